@@ -1,0 +1,245 @@
+"""Benchmark: dynamics-gradient knot points/s on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+A step is one pass of the hot path over one batch of synthetic knot points
+already resident in HBM.  The headline workload (`value`) is the plain per-call
+kernel on one trajectory of N = 128 knots, one launch per call; the large batch
+(2^20 knots, 10% stretched-arm poses) and the 64-trajectory mixed-length batch
+are reported beside it under "workloads".  Knot points/s is also given as a
+fraction of the HBM roofline (504 algorithmic bytes per knot: 84 in, 420 out)
+and of the FP32 roofline (31,340 algorithmic FLOP per knot, the op count of
+the oracle's restatement of the reference method).  The CPU baseline is the
+oracle's FP32 restatement of the reference's multithreaded path (time_CPU.cpp
+ReusableThreads loop) timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "fast-rbd-gradients_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import rbdgrad  # noqa: E402
+
+METRIC = "dynamics-gradient knot points/s @<=1e-4 rel err vs FP64; served p50/p99 latency"
+BYTES_PER_KNOT = 504          # 21 FP32 in + 105 FP32 out
+FLOPS_PER_KNOT = 31340        # algorithmic count of the reference method (oracle op count)
+HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 (packed vector / matrix) peak
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def knots(n, seed, near_singular_frac=0.0):
+    rng = np.random.default_rng(seed)
+    q = rng.uniform(-np.pi, np.pi, (n, 7))
+    qd = rng.uniform(-2.0, 2.0, (n, 7))
+    u = rng.uniform(-50.0, 50.0, (n, 7))
+    if near_singular_frac > 0:
+        m = rng.random(n) < near_singular_frac
+        for j in (1, 3, 5):
+            q[m, j] = rng.uniform(-1e-3, 1e-3, int(m.sum()))
+    return np.concatenate([q, qd, u], axis=1).astype(np.float32)
+
+
+def time_launches(launch, steps, warmup, stream):
+    """Warm up, then time exactly `steps` launches on `stream` with HIP events,
+    synchronising the device on both sides.  Returns total milliseconds."""
+    for _ in range(warmup):
+        launch()
+    rbdgrad.hip().hipDeviceSynchronize()
+    t0, t1 = rbdgrad.Event(), rbdgrad.Event()
+    t0.record(stream)
+    for _ in range(steps):
+        launch()
+    t1.record(stream)
+    t1.synchronize()
+    rbdgrad.hip().hipDeviceSynchronize()
+    ms = t0.elapsed_ms(t1)
+    t0.destroy()
+    t1.destroy()
+    return ms
+
+
+def accuracy(out, x, sample=None):
+    """Max relative error per block vs the FP64 oracle on the same FP32 inputs."""
+    from oracle import pyoracle
+    idx = np.arange(len(x)) if sample is None or sample >= len(x) else \
+        np.random.default_rng(5).choice(len(x), sample, replace=False)
+    ref = pyoracle.fdgrad(x[idx].astype(np.float64), threads=min(16, os.cpu_count() or 1))
+    o = out[idx].astype(np.float64)
+    res = {}
+    for name, sl in (("qdd", slice(0, 7)), ("dqdd_dq", slice(7, 56)), ("dqdd_dqd", slice(56, 105))):
+        e = np.max(np.abs(o[:, sl] - ref[:, sl]), axis=1) / np.maximum(np.max(np.abs(ref[:, sl]), axis=1), 1e-6)
+        res[name] = {"max": float(e.max()), "p99.9": float(np.percentile(e, 99.9))}
+    res["knots_checked"] = int(len(idx))
+    return res
+
+
+def roofline(knots_per_launch, ms_per_launch):
+    sec = ms_per_launch * 1e-3
+    gbs = knots_per_launch * BYTES_PER_KNOT / sec / 1e9
+    tfl = knots_per_launch * FLOPS_PER_KNOT / sec / 1e12
+    return gbs, tfl
+
+
+def run_flat(ctx, x, steps, warmup, stream, layout="aos"):
+    n = len(x)
+    if layout == "soa":
+        xs = np.ascontiguousarray(x.T)
+        d_in = rbdgrad.DeviceBuffer.from_array(xs)
+        d_out = rbdgrad.DeviceBuffer(n * rbdgrad.OUT_PER_KNOT * 4)
+        launch = lambda: ctx.fdgrad_soa(n, d_in, n, d_out, n, stream)  # noqa: E731
+    else:
+        d_in = rbdgrad.DeviceBuffer.from_array(x)
+        d_out = rbdgrad.DeviceBuffer(n * rbdgrad.OUT_PER_KNOT * 4)
+        launch = lambda: ctx.fdgrad(n, d_in, d_out, stream)  # noqa: E731
+    ms = time_launches(launch, steps, warmup, stream)
+    if layout == "soa":
+        out = d_out.download((rbdgrad.OUT_PER_KNOT, n)).T.copy()
+    else:
+        out = d_out.download((n, rbdgrad.OUT_PER_KNOT))
+    d_in.free()
+    d_out.free()
+    return ms, out
+
+
+def cpu_baseline(x, seconds):
+    """The oracle's FP32 restatement of the reference's ReusableThreads loop on
+    this host: 4 threads (the reference's CPU_THREADS_GLOBAL) and all usable
+    cores (capped at 16, the GPU host's CPU share); the faster is reported."""
+    from oracle import pyoracle
+    n = len(x)
+    cores = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    results = []
+    for threads in sorted({4, cores}):
+        times, _ = pyoracle.cpu_bench(x, threads, iters=10 ** 7, mode=0, time_cap_s=seconds)
+        times = times[len(times) // 10:]          # drop the first 10% (pool warm-up)
+        mean_us = float(np.mean(times))
+        results.append({"value": n / (mean_us * 1e-6), "unit": "knots/s", "cores": threads, "kind": "port",
+                        "sample": f"{len(times)} calls x {n} knots (config 1 inputs), FP32, ReusableThreads-style "
+                                  f"pool, {threads} threads, ~{seconds:.0f} s",
+                        "p50_us": float(np.percentile(times, 50)), "p99_us": float(np.percentile(times, 99))})
+        log(f"cpu baseline {threads} threads: {results[-1]['value']:.3e} knots/s")
+    best = max(results, key=lambda r: r["value"])
+    best = dict(best)
+    best["other"] = [r for r in results if r is not best and r["cores"] != best["cores"]]
+    return best
+
+
+def mixed_trajectories(seed=7, count=64):
+    rng = np.random.default_rng(seed)
+    return rng.integers(8, 257, size=count)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000, help="timed launches of the per-call workload")
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--precision", default="fp32", choices=list(rbdgrad.PRECISIONS))
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time per thread count")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="skip the large-batch and mixed workloads")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "knot points are independent, so N GPUs would each run their own share of knots", file=sys.stderr)
+        sys.exit(2)
+    if rbdgrad.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    stream = rbdgrad.Stream()
+    ctx = rbdgrad.Context(device=0, precision=args.precision)
+    kname = ctx.kernel_name()
+
+    # ---- headline: per-call kernel, one trajectory of 128 knots per launch ----
+    n1 = 128
+    x1 = knots(n1, seed=101)
+    log(f"[c1] per-call N={n1}: {args.warmup} warmup + {args.steps} timed launches")
+    ms1, out1 = run_flat(ctx, x1, args.steps, args.warmup, stream)
+    ms_step = ms1 / args.steps
+    value = n1 * args.steps / (ms1 * 1e-3)
+    gbs1, tfl1 = roofline(n1, ms_step)
+    acc1 = accuracy(out1, x1)
+    log(f"[c1] {value:.3e} knots/s, {ms_step * 1e3:.2f} us/launch, acc {acc1}")
+
+    workloads = {}
+    if not args.quick:
+        # ---- large batch: 2^20 knots, 10% stretched-arm poses ----
+        n4 = 1 << 20
+        x4 = knots(n4, seed=104, near_singular_frac=0.1)
+        steps4 = 20
+        for prec in ("fp32", "fp64_columns"):
+            c4 = ctx if prec == args.precision else rbdgrad.Context(device=0, precision=prec)
+            for layout in ("aos", "soa"):
+                log(f"[c4] 2^20 knots prec={prec} layout={layout}: 3 warmup + {steps4} timed")
+                ms4, out4 = run_flat(c4, x4, steps4, 3, stream, layout)
+                per = ms4 / steps4
+                gbs, tfl = roofline(n4, per)
+                workloads[f"large_batch_2p20_{prec}_{layout}"] = {
+                    "knots_per_launch": n4, "ms_per_launch": per, "knots_per_s": n4 / (per * 1e-3),
+                    "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_tflops": tfl,
+                    "fp32_frac": tfl / FP32_PEAK_TFLOPS, "accuracy": accuracy(out4, x4, sample=8192)}
+                log(f"[c4] {workloads[f'large_batch_2p20_{prec}_{layout}']}")
+            if c4 is not ctx:
+                c4.close()
+        # ---- 64 trajectories of uneven length U{8..256}, one launch over all knots ----
+        lens = mixed_trajectories()
+        n2 = int(lens.sum())
+        x2 = knots(n2, seed=102)
+        steps2 = 500
+        log(f"[c2] mixed 64 trajectories ({n2} knots): one launch per batch, {steps2} timed")
+        ms2, out2 = run_flat(ctx, x2, steps2, 20, stream)
+        per2 = ms2 / steps2
+        gbs, tfl = roofline(n2, per2)
+        workloads["mixed_64_traj_flat"] = {
+            "knots_per_launch": n2, "ms_per_launch": per2, "knots_per_s": n2 / (per2 * 1e-3),
+            "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_tflops": tfl, "fp32_frac": tfl / FP32_PEAK_TFLOPS,
+            "accuracy": accuracy(out2, x2, sample=4096)}
+        log(f"[c2] {workloads['mixed_64_traj_flat']}")
+
+    cpu = None if args.no_cpu else cpu_baseline(x1, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "knots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else args.precision,
+        "data": "synthetic: seeded uniform q~U(-pi,pi), qd~U(-2,2), u~U(-50,50); iiwa 14 model constants",
+        "config": {"workload": "per-call kernel, one trajectory of 128 knot points per launch (config 1)",
+                   "knots_per_call": n1, "layout": "aos", "precision": args.precision, "kernel": kname},
+        "roofline": {"bound": "hbm", "achieved": gbs1, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs1 / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_knot": BYTES_PER_KNOT, "knots_per_launch": n1},
+        "roofline_fp32": {"bound": "fp32-valu", "achieved": tfl1, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": tfl1 / FP32_PEAK_TFLOPS, "flops_per_knot": FLOPS_PER_KNOT},
+        "accuracy": acc1,
+        "workloads": workloads,
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    stream.destroy()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

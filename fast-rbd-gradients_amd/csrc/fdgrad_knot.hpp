@@ -1,0 +1,636 @@
+// Per-knot-point forward-dynamics gradient of the KUKA iiwa 14, register resident.
+//
+// One call computes, for one knot point, q, qd, u -> qdd, dqdd/dq, dqdd/dqd with
+// the reference's method (plancherb1/fast-rbd-gradients):
+//   1. sin/cos of q and the seven joint transforms  (dynamicsGradient.h:507-587)
+//   2. Minv by the articulated-body recursion         (minv.h:423-478)
+//   3. RNEA with qdd = 0 -> bias c                    (dynamicsGradient.h:619-683)
+//   4. qdd = Minv (u - c)                             (dynamicsGradient.h:704)
+//   5. RNEA with qdd: v, a, f                         (dynamicsGradient.h:893-895)
+//   6. dtau/dq, dtau/dqd: forward sweep over links per column, backward sweep
+//                                                     (dynamicsGradient.h:786-857)
+//   7. dqdd/d(q,qd) = -Minv dtau/d(q,qd)              (dynamicsGradient.h:921)
+// Every link index is a compile-time constant (static_for), so each joint's
+// transform, inertia and offset are folded into straight-line code: structural
+// zeros of E, r and I cost nothing when the file is compiled with
+// -fno-signed-zeros -ffinite-math-only (these let x*0 and x+0 fold away).
+// The column loop of step 6 runs at run time so that its body stays small; a
+// column's d/dq and d/dqd sweeps share one pass over the links, and each output
+// column -Minv dtau[:, c] is stored as soon as it is known, so dtau never has
+// to be held whole.
+//
+// The file is plain C++17 when RBD_HD is empty, so the same arithmetic is also
+// compiled for the host by the CPU test harness.
+#pragma once
+#include "iiwa_model.hpp"
+#include <type_traits>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define RBD_HD __host__ __device__
+#else
+#include <cmath>
+#define RBD_HD
+#endif
+#define RBD_INLINE __attribute__((always_inline)) inline
+
+namespace rbd {
+
+template <int N> struct IC { static constexpr int value = N; };
+
+// f(IC<B>), f(IC<B+1>), ..., f(IC<E-1>)
+template <int B, int E, class F>
+RBD_HD RBD_INLINE void static_for(F&& f) {
+  if constexpr (B < E) { f(IC<B>{}); static_for<B + 1, E>(f); }
+}
+// f(IC<B-1>), f(IC<B-2>), ..., f(IC<E>)
+template <int B, int E, class F>
+RBD_HD RBD_INLINE void static_for_down(F&& f) {
+  if constexpr (B > E) { f(IC<B - 1>{}); static_for_down<B - 1, E>(f); }
+}
+
+template <typename S> struct CS { S c, s; };
+
+RBD_HD RBD_INLINE void sincos_s(float x, float* s, float* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  ::sincosf(x, s, c);
+#else
+  *s = std::sin(x); *c = std::cos(x);
+#endif
+}
+RBD_HD RBD_INLINE void sincos_s(double x, double* s, double* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  ::sincos(x, s, c);
+#else
+  *s = std::sin(x); *c = std::cos(x);
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Joint transform of link K: T = [E 0; -E r^ E]
+// ---------------------------------------------------------------------------
+template <int K, int I, int J, typename S>
+RBD_HD RBD_INLINE S Eij(const CS<S>& t) {
+  constexpr S a = S(ecoef_c(K, I, J)), b = S(ecoef_s(K, I, J)), d = S(ecoef_1(K, I, J));
+  return a * t.c + b * t.s + d;
+}
+// y = E x
+template <int K, typename S>
+RBD_HD RBD_INLINE void rot(const CS<S>& t, const S* x, S* y) {
+  static_for<0, 3>([&](auto Ic) {
+    constexpr int i = decltype(Ic)::value;
+    y[i] = Eij<K, i, 0>(t) * x[0] + Eij<K, i, 1>(t) * x[1] + Eij<K, i, 2>(t) * x[2];
+  });
+}
+// y = E^T x
+template <int K, typename S>
+RBD_HD RBD_INLINE void rot_t(const CS<S>& t, const S* x, S* y) {
+  static_for<0, 3>([&](auto Jc) {
+    constexpr int j = decltype(Jc)::value;
+    y[j] = Eij<K, 0, j>(t) * x[0] + Eij<K, 1, j>(t) * x[1] + Eij<K, 2, j>(t) * x[2];
+  });
+}
+template <int K, typename S> constexpr S roff(int i) { return S(kJointOffset[K][i]); }
+
+// y = T x for a motion vector: [E w; E (u - r x w)]
+template <int K, typename S>
+RBD_HD RBD_INLINE void xmotion(const CS<S>& t, const S* x, S* y) {
+  constexpr S rx = roff<K, S>(0), ry = roff<K, S>(1), rz = roff<K, S>(2);
+  const S d[3] = {x[3] - (ry * x[2] - rz * x[1]),
+                      x[4] - (rz * x[0] - rx * x[2]),
+                      x[5] - (rx * x[1] - ry * x[0])};
+  S y0[3], y1[3];
+  rot<K>(t, x, y0);
+  rot<K>(t, d, y1);
+  y[0] = y0[0]; y[1] = y0[1]; y[2] = y0[2];
+  y[3] = y1[0]; y[4] = y1[1]; y[5] = y1[2];
+}
+// y = T^T x for a force vector: [E^T n + r x (E^T f); E^T f]
+template <int K, typename S>
+RBD_HD RBD_INLINE void xforce_t(const CS<S>& t, const S* x, S* y) {
+  constexpr S rx = roff<K, S>(0), ry = roff<K, S>(1), rz = roff<K, S>(2);
+  S n[3], fl[3];
+  rot_t<K>(t, x, n);
+  rot_t<K>(t, x + 3, fl);
+  y[0] = n[0] + (ry * fl[2] - rz * fl[1]);
+  y[1] = n[1] + (rz * fl[0] - rx * fl[2]);
+  y[2] = n[2] + (rx * fl[1] - ry * fl[0]);
+  y[3] = fl[0]; y[4] = fl[1]; y[5] = fl[2];
+}
+
+// entry (R, C) of link K's spatial inertia, as a compile-time constant
+template <typename S, int K, int R, int C>
+constexpr S kInertia = S(inertia_entry(K, R, C));
+
+// y (+)= I_K x, I_K the constant spatial inertia of link K
+template <int K, bool ACC = false, typename S>
+RBD_HD RBD_INLINE void imv(const S* x, S* y) {
+  static_for<0, 6>([&](auto Rc) {
+    constexpr int r = decltype(Rc)::value;
+    S acc = kInertia<S, K, r, 0> * x[0];
+    static_for<1, 6>([&](auto Cc) {
+      constexpr int c = decltype(Cc)::value;
+      acc += kInertia<S, K, r, c> * x[c];
+    });
+    y[r] = ACC ? y[r] + acc : acc;
+  });
+}
+// A = I_K
+template <int K, typename S>
+RBD_HD RBD_INLINE void iload(S A[6][6]) {
+  static_for<0, 6>([&](auto Rc) {
+    constexpr int r = decltype(Rc)::value;
+    static_for<0, 6>([&](auto Cc) {
+      constexpr int c = decltype(Cc)::value;
+      A[r][c] = kInertia<S, K, r, c>;
+    });
+  });
+}
+
+// y (+)= x x* z  =  [w x za + u x zl ; w x zl]   (spatial force cross product)
+template <bool ACC = false, typename S>
+RBD_HD RBD_INLINE void crf(const S* x, const S* z, S* y) {
+  const S y0 = x[1] * z[2] - x[2] * z[1] + x[4] * z[5] - x[5] * z[4];
+  const S y1 = x[2] * z[0] - x[0] * z[2] + x[5] * z[3] - x[3] * z[5];
+  const S y2 = x[0] * z[1] - x[1] * z[0] + x[3] * z[4] - x[4] * z[3];
+  const S y3 = x[1] * z[5] - x[2] * z[4];
+  const S y4 = x[2] * z[3] - x[0] * z[5];
+  const S y5 = x[0] * z[4] - x[1] * z[3];
+  if (ACC) { y[0] += y0; y[1] += y1; y[2] += y2; y[3] += y3; y[4] += y4; y[5] += y5; }
+  else     { y[0] = y0;  y[1] = y1;  y[2] = y2;  y[3] = y3;  y[4] = y4;  y[5] = y5; }
+}
+
+// ---------------------------------------------------------------------------
+// Articulated-inertia transport: P = T^T Ia T for a symmetric Ia whose row and
+// column 2 are zero (revolute joint about z).  T = [E 0; 0 E][1 0; -r^ 1]:
+// rotate the 3x3 blocks (A' = E^T A E, C' = E^T C E, M' = E^T M E), then
+// translate: TL = A' + r^ C'^T - C' r^ - r^ M' r^, TR = C' + r^ M', BR = M'.
+// (Same product as the reference's ctransform_Ia_revolute, minv.h:221-386.)
+// ---------------------------------------------------------------------------
+template <int K, typename S>
+RBD_HD RBD_INLINE void transport(const CS<S>& t, const S Ia[6][6], S P[6][6]) {
+  S A[3][3], C[3][3], M[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) { A[i][j] = Ia[i][j]; C[i][j] = Ia[i][j + 3]; M[i][j] = Ia[i + 3][j + 3]; }
+  // X' = E^T X E : first Y = X E (rows of X times E), then E^T Y
+  auto sandwich = [&](const S X[3][3], S out[3][3]) {
+    S Y[3][3];
+    for (int i = 0; i < 3; ++i) {
+      S row_e[3];
+      // (X E)[i][j] = sum_l X[i][l] E[l][j] = (E^T x_i)_j  with x_i the i-th row
+      const S xi[3] = {X[i][0], X[i][1], X[i][2]};
+      rot_t<K>(t, xi, row_e);
+      Y[i][0] = row_e[0]; Y[i][1] = row_e[1]; Y[i][2] = row_e[2];
+    }
+    for (int j = 0; j < 3; ++j) {
+      const S yj[3] = {Y[0][j], Y[1][j], Y[2][j]};
+      S col[3];
+      rot_t<K>(t, yj, col);            // (E^T Y)[:, j] = E^T y_j
+      out[0][j] = col[0]; out[1][j] = col[1]; out[2][j] = col[2];
+    }
+  };
+  S Ar[3][3], Cr[3][3], Mr[3][3];
+  sandwich(A, Ar);
+  sandwich(C, Cr);
+  sandwich(M, Mr);
+  constexpr S rx = roff<K, S>(0), ry = roff<K, S>(1), rz = roff<K, S>(2);
+  const S R[3][3] = {{S(0), -rz, ry}, {rz, S(0), -rx}, {-ry, rx, S(0)}};
+  // TR = Cr + R Mr
+  S TR[3][3], RM[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      RM[i][j] = R[i][0] * Mr[0][j] + R[i][1] * Mr[1][j] + R[i][2] * Mr[2][j];
+      TR[i][j] = Cr[i][j] + RM[i][j];
+    }
+  // TL = Ar + R Cr^T - Cr R - R Mr R = Ar + R Cr^T - (Cr + R Mr) R = Ar + R Cr^T - TR R
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const S rct = R[i][0] * Cr[j][0] + R[i][1] * Cr[j][1] + R[i][2] * Cr[j][2];
+      const S trr = TR[i][0] * R[0][j] + TR[i][1] * R[1][j] + TR[i][2] * R[2][j];
+      const S e = Ar[i][j] + rct - trr;
+      P[i][j] = e; P[j][i] = e;
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      P[i][j + 3] = TR[i][j];
+      P[j + 3][i] = TR[i][j];
+      P[i + 3][j + 3] = Mr[i][j];
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Arithmetic precision per phase.  Inputs and outputs are FP32 in every mode.
+//   Minv  : the articulated-body recursion for Minv
+//   Pivot : the pivots D_k = S^T IA_k S and 1/D_k inside it
+//   Rnea  : bias RNEA, qdd and the RNEA with qdd
+//   Col   : the derivative columns dtau/d(q,qd) and -Minv dtau
+// ---------------------------------------------------------------------------
+template <typename SMinv, typename SPivot, typename SRnea, typename SCol>
+struct Precision {
+  using Minv = SMinv; using Pivot = SPivot; using Rnea = SRnea; using Col = SCol;
+};
+using PrecFP32      = Precision<float, float, float, float>;     // all FP32
+using PrecFP64Pivot = Precision<float, double, float, float>;    // FP64 pivots only
+using PrecFP64Col   = Precision<float, float, float, double>;    // FP64 derivative columns
+using PrecFP64      = Precision<double, double, double, double>; // all FP64
+
+// ---------------------------------------------------------------------------
+// Minv = M^-1 (upper triangle, Mi[r][c] for r <= c) by the articulated-body
+// recursion (computeMinv, minv.h:423-478; equation numbers as in its comments).
+// The pivots D_k and 1/D_k are formed in SP (FP64 in the pivot variant).
+// ---------------------------------------------------------------------------
+template <typename S, typename SP>
+RBD_HD RBD_INLINE void compute_minv(const CS<S>* t, S Mi[NQ][NQ]) {
+  S IA[6][6];
+  SP IA22;                    // IA[2][2] (the pivot) carried in SP
+  iload<NQ - 1>(IA);
+  IA22 = SP(IA[2][2]);
+  S F[NQ][6];                 // F[j] = F_k[:, j], columns j >= k meaningful
+  S U[NQ][6], Dinv[NQ];
+  for (int j = 0; j < NQ; ++j)
+    for (int r = 0; r < 6; ++r) F[j][r] = S(0);
+  static_for_down<NQ, 0>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    for (int r = 0; r < 6; ++r) U[k][r] = IA[r][2];        // U = IA S
+    const SP dinv = SP(1) / IA22;                            // 1 / (S^T IA S)
+    Dinv[k] = S(dinv);
+    Mi[k][k] = Dinv[k];                                      // (21)
+    static_for<k + 1, NQ>([&](auto Jc) {                     // (12)
+      constexpr int j = decltype(Jc)::value;
+      Mi[k][j] = -Dinv[k] * F[j][2];
+    });
+    static_for<k, NQ>([&](auto Jc) {                         // (14), (19)
+      constexpr int j = decltype(Jc)::value;
+      for (int r = 0; r < 6; ++r) F[j][r] += U[k][r] * Mi[k][j];
+    });
+    if constexpr (k > 0) {
+      static_for<k, NQ>([&](auto Jc) {                       // (15): F_{k-1} = T^T F_k
+        constexpr int j = decltype(Jc)::value;
+        S y[6];
+        xforce_t<k>(t[k], F[j], y);
+        for (int r = 0; r < 6; ++r) F[j][r] = y[r];
+      });
+      S Ia[6][6];                                            // IA - U U^T / D
+      S UD[6];
+      for (int r = 0; r < 6; ++r) UD[r] = U[k][r] * Dinv[k];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c)
+          Ia[r][c] = (r == 2 || c == 2) ? S(0) : IA[r][c] - U[k][r] * UD[c];
+      S P[6][6];
+      transport<k>(t[k], Ia, P);
+      iload<k - 1>(IA);
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) IA[r][c] += P[r][c];
+      if constexpr (sizeof(SP) > sizeof(S)) {
+        // the next pivot S^T IA_{k-1} S formed in SP from the same (rounded) terms;
+        // only entry (2,2) of this second transport is live, the rest is dead code
+        SP IaW[6][6], PW[6][6];
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 6; ++c) IaW[r][c] = SP(Ia[r][c]);
+        const CS<SP> tw{SP(t[k].c), SP(t[k].s)};
+        transport<k>(tw, IaW, PW);
+        IA22 = SP(inertia_entry(k - 1, 2, 2)) + PW[2][2];
+      } else {
+        IA22 = SP(IA[2][2]);
+      }
+    }
+  });
+  // forward pass (Minv_fp, minv.h:439-469): F_k[:, j] = S Minv[k, j] + T F_{k-1}[:, j]
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    if constexpr (k == 0) {
+      for (int j = 0; j < NQ; ++j) {
+        for (int r = 0; r < 6; ++r) F[j][r] = S(0);
+        F[j][2] = Mi[0][j];
+      }
+    } else {
+      static_for<k, NQ>([&](auto Jc) {
+        constexpr int j = decltype(Jc)::value;
+        S w[6];
+        xmotion<k>(t[k], F[j], w);                           // T F_{k-1}[:, j]
+        const S uw = U[k][0] * w[0] + U[k][1] * w[1] + U[k][2] * w[2] +
+                     U[k][3] * w[3] + U[k][4] * w[4] + U[k][5] * w[5];
+        Mi[k][j] -= Dinv[k] * uw;                            // (30)
+        w[2] += Mi[k][j];                                    // (32) + (34)
+        for (int r = 0; r < 6; ++r) F[j][r] = w[r];
+      });
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// RNEA forward pass (RNEA_fp, dynamicsGradient.h:619-659): v, a, f per link.
+// The gravity trick sets the base acceleration to +g along z.
+// ---------------------------------------------------------------------------
+template <typename S>
+RBD_HD RBD_INLINE void rnea_fwd(const CS<S>* t, const S* qd, const S* qdd, S gravity,
+                                S v[NQ][6], S a[NQ][6], S f[NQ][6]) {
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    if constexpr (k == 0) {
+      for (int i = 0; i < 6; ++i) { v[0][i] = S(0); a[0][i] = S(0); }
+      v[0][2] = qd[0];
+      a[0][2] = qdd[0];
+      const S g[3] = {S(0), S(0), gravity};
+      S ga[3];
+      rot<0>(t[0], g, ga);                   // T_0 [0,0,0,0,0,g] = [0; E g]
+      a[0][3] = ga[0]; a[0][4] = ga[1]; a[0][5] = ga[2];
+    } else {
+      xmotion<k>(t[k], v[k - 1], v[k]);
+      v[k][2] += qd[k];
+      xmotion<k>(t[k], a[k - 1], a[k]);
+      a[k][0] += v[k][1] * qd[k];            // + (v x S) qd
+      a[k][1] -= v[k][0] * qd[k];
+      a[k][3] += v[k][4] * qd[k];
+      a[k][4] -= v[k][3] * qd[k];
+      a[k][2] += qdd[k];
+    }
+    S Iv[6];
+    imv<k>(v[k], Iv);
+    imv<k>(a[k], f[k]);                      // f = I a + v x* I v
+    crf<true>(v[k], Iv, f[k]);
+  });
+}
+
+// RNEA backward pass (RNEA_bp, dynamicsGradient.h:661-670): f_{k-1} += T^T f_k
+template <typename S>
+RBD_HD RBD_INLINE void rnea_bwd(const CS<S>* t, S f[NQ][6]) {
+  static_for_down<NQ, 1>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    S y[6];
+    xforce_t<k>(t[k], f[k], y);
+    for (int i = 0; i < 6; ++i) f[k - 1][i] += y[i];
+  });
+}
+
+// ---------------------------------------------------------------------------
+// One derivative column.  Forward sweep (compute_vafdu, dynamicsGradient.h:786-829)
+// from link c: at k == c the seed is dv = S, da = v_c x S for d/dqd, and
+// dv = v_c x S, da = a_c x S for d/dq (the reference's (T v_{c-1}) x S and
+// (v x S) qd + (T a_{c-1}) x S reduce to these because S x S = 0);
+// for k > c: dv = T dv, da = T da + (dv x S) qd_k.  Then
+// df_k = I da + dv x* (I v_k) + v_k x* (I dv).
+// Backward sweep (compute_cdu, :831-857): g_{k-1} = df_{k-1} + T^T g_k, with the
+// extra T^T (-(f_k x S)) at k == c for d/dq; dtau_k = g_k[2].
+// ---------------------------------------------------------------------------
+template <typename S>
+struct PhaseC {
+  CS<S> t[NQ];
+  S v[NQ][6];      // link velocities
+  S Iv[NQ][6];     // I_k v_k
+  S axs[NQ][4];    // a_k x S  -> (a1, -a0, a4, -a3)   (entries 2 and 5 are zero)
+  S nfxs[NQ][4];   // -(f_k x S) -> (-f1, f0, -f4, f3)
+  S qd[NQ];
+  S Mi[NQ][NQ];    // upper triangle of Minv
+};
+
+// dtau[:, c] for the d/dqd (DQ == false) or d/dq (DQ == true) column c.
+template <bool DQ, typename S>
+RBD_HD RBD_INLINE void dtau_column(const PhaseC<S>& P, int c, S tau[NQ]) {
+  S dv[6], da[6], df[NQ][6];
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    if (k < c) {
+      for (int i = 0; i < 6; ++i) df[k][i] = S(0);
+      return;
+    }
+    if (k == c) {
+      if constexpr (DQ) {
+        dv[0] = P.v[k][1]; dv[1] = -P.v[k][0]; dv[2] = S(0);
+        dv[3] = P.v[k][4]; dv[4] = -P.v[k][3]; dv[5] = S(0);
+        da[0] = P.axs[k][0]; da[1] = P.axs[k][1]; da[2] = S(0);
+        da[3] = P.axs[k][2]; da[4] = P.axs[k][3]; da[5] = S(0);
+      } else {
+        for (int i = 0; i < 6; ++i) dv[i] = S(0);
+        dv[2] = S(1);
+        da[0] = P.v[k][1]; da[1] = -P.v[k][0]; da[2] = S(0);
+        da[3] = P.v[k][4]; da[4] = -P.v[k][3]; da[5] = S(0);
+      }
+    } else {
+      if constexpr (k > 0) {
+        S y[6];
+        xmotion<k>(P.t[k], dv, y);
+        for (int i = 0; i < 6; ++i) dv[i] = y[i];
+        xmotion<k>(P.t[k], da, y);
+        const S qk = P.qd[k];
+        da[0] = y[0] + dv[1] * qk; da[1] = y[1] - dv[0] * qk; da[2] = y[2];
+        da[3] = y[3] + dv[4] * qk; da[4] = y[4] - dv[3] * qk; da[5] = y[5];
+      }
+    }
+    S Idv[6];
+    imv<k>(dv, Idv);
+    imv<k>(da, df[k]);
+    crf<true>(dv, P.Iv[k], df[k]);
+    crf<true>(P.v[k], Idv, df[k]);
+  });
+  S g[6];
+  for (int i = 0; i < 6; ++i) g[i] = df[NQ - 1][i];
+  static_for_down<NQ, 1>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    tau[k] = g[2];
+    if constexpr (DQ) {
+      if (k == c) {
+        g[0] += P.nfxs[k][0]; g[1] += P.nfxs[k][1];
+        g[3] += P.nfxs[k][2]; g[4] += P.nfxs[k][3];
+      }
+    }
+    S y[6];
+    xforce_t<k>(P.t[k], g, y);
+    if (k - 1 >= c) for (int i = 0; i < 6; ++i) g[i] = y[i] + df[k - 1][i];
+    else            for (int i = 0; i < 6; ++i) g[i] = y[i];
+  });
+  tau[0] = g[2];
+}
+
+// tau[c] += x for a run-time c without indexing a register array
+template <typename S>
+RBD_HD RBD_INLINE void add_on_diagonal(S tau[NQ], int c, S x) {
+  static_for<0, NQ>([&](auto Rc) {
+    constexpr int r = decltype(Rc)::value;
+    tau[r] += (r == c) ? x : S(0);
+  });
+}
+
+// out[r] = (Minv x)[r] (NEG: negated), Minv upper-stored
+template <bool NEG, typename S, typename SX>
+RBD_HD RBD_INLINE void minv_times(const S Mi[NQ][NQ], const SX x[NQ], SX out[NQ]) {
+  static_for<0, NQ>([&](auto Rc) {
+    constexpr int r = decltype(Rc)::value;
+    SX acc = SX(0);
+    static_for<0, NQ>([&](auto Cc) {
+      constexpr int c = decltype(Cc)::value;
+      acc += SX(r <= c ? Mi[r][c] : Mi[c][r]) * x[c];
+    });
+    out[r] = NEG ? -acc : acc;
+  });
+}
+
+// Phase C from v, a (with qdd), f (after the backward pass) and Minv:
+// the derivative columns and the stores of dqdd/dq, dqdd/dqd.
+template <typename SC, typename SR, typename SM, class Out>
+RBD_HD RBD_INLINE void gradient_columns(const CS<SC>* t, const SR* qd, const SR v[NQ][6], const SR a[NQ][6],
+                                        const SR f[NQ][6], const SM Mi[NQ][NQ], float damping, Out& out) {
+  PhaseC<SC> P;
+  for (int k = 0; k < NQ; ++k) { P.t[k] = t[k]; P.qd[k] = SC(qd[k]); }
+  for (int r = 0; r < NQ; ++r)
+    for (int c = r; c < NQ; ++c) P.Mi[r][c] = SC(Mi[r][c]);
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    for (int i = 0; i < 6; ++i) P.v[k][i] = SC(v[k][i]);
+    imv<k>(P.v[k], P.Iv[k]);
+    P.axs[k][0] = SC(a[k][1]); P.axs[k][1] = -SC(a[k][0]); P.axs[k][2] = SC(a[k][4]); P.axs[k][3] = -SC(a[k][3]);
+    P.nfxs[k][0] = -SC(f[k][1]); P.nfxs[k][1] = SC(f[k][0]); P.nfxs[k][2] = -SC(f[k][4]); P.nfxs[k][3] = SC(f[k][3]);
+  });
+  // d/dq column 0 is identically zero (rotation about the gravity axis)
+  for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
+#pragma clang loop unroll(disable)
+  for (int c = 0; c < NQ; ++c) {
+    SC tau[NQ], col[NQ];
+    dtau_column<false>(P, c, tau);
+    add_on_diagonal(tau, c, SC(damping));
+    minv_times<true>(P.Mi, tau, col);
+    for (int r = 0; r < NQ; ++r) out.dqd(r, c, float(col[r]));
+    if (c > 0) {
+      dtau_column<true>(P, c, tau);
+      minv_times<true>(P.Mi, tau, col);
+      for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
+    }
+  }
+}
+
+// Largest of two scalar types
+template <typename A, typename B>
+using Wider = typename std::conditional<(sizeof(A) >= sizeof(B)), A, B>::type;
+
+// ---------------------------------------------------------------------------
+// Accessors.  In: q(i), qd(i), u(i) [, qdd(i), minv(r, c) in the reference-
+// compatible mode].  Out: qdd(i, x), dq(r, c, x), dqd(r, c, x).  All FP32.
+// ---------------------------------------------------------------------------
+
+// Run-time model options (the reference's compile-time VEL_DAMPING and MPC_MODE).
+struct Options {
+  float damping = 0.0f;               // joint viscous damping: 0.5 with VEL_DAMPING (dynamicsGradient.h:680,840)
+  float gravity = float(kGravity);    // 0 in MPC_MODE (dynamicsGradient.h:635-637)
+};
+
+// Full path, P: arithmetic precision per phase.
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void fdgrad_knot(const In& in, Out& out, const Options& opt) {
+  using SM = typename P::Minv; using SP = typename P::Pivot;
+  using SR = typename P::Rnea; using SC = typename P::Col;
+  using ST = Wider<Wider<SM, SR>, SC>;     // sin/cos computed once in the widest type
+  CS<SM> tm[NQ]; CS<SR> tr[NQ]; CS<SC> tc[NQ];
+  SR qd[NQ], u[NQ];
+  for (int k = 0; k < NQ; ++k) {
+    ST s, c;
+    sincos_s(ST(in.q(k)), &s, &c);
+    tm[k] = CS<SM>{SM(c), SM(s)}; tr[k] = CS<SR>{SR(c), SR(s)}; tc[k] = CS<SC>{SC(c), SC(s)};
+    qd[k] = SR(in.qd(k));
+    u[k] = SR(in.u(k));
+  }
+  SM Mi[NQ][NQ];
+  compute_minv<SM, SP>(tm, Mi);
+  // bias: RNEA with qdd = 0
+  SR v[NQ][6], a[NQ][6], f[NQ][6];
+  const SR zero[NQ] = {SR(0), SR(0), SR(0), SR(0), SR(0), SR(0), SR(0)};
+  rnea_fwd(tr, qd, zero, SR(opt.gravity), v, a, f);
+  rnea_bwd(tr, f);
+  SR bias[NQ], qdd[NQ];
+  for (int k = 0; k < NQ; ++k) bias[k] = u[k] - f[k][2] - SR(opt.damping) * qd[k];
+  minv_times<false>(Mi, bias, qdd);
+  for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
+  // RNEA with qdd, added incrementally: a += da, f += bwd(I da), da_k = T da_{k-1} + S qdd_k
+  {
+    SR dA[6] = {SR(0), SR(0), qdd[0], SR(0), SR(0), SR(0)};
+    SR dF[NQ][6];
+    static_for<0, NQ>([&](auto Kc) {
+      constexpr int k = decltype(Kc)::value;
+      if constexpr (k > 0) {
+        SR y[6];
+        xmotion<k>(tr[k], dA, y);
+        for (int i = 0; i < 6; ++i) dA[i] = y[i];
+        dA[2] += qdd[k];
+      }
+      for (int i = 0; i < 6; ++i) a[k][i] += dA[i];
+      imv<k>(dA, dF[k]);
+    });
+    rnea_bwd(tr, dF);
+    for (int k = 0; k < NQ; ++k)
+      for (int i = 0; i < 6; ++i) f[k][i] += dF[k][i];
+  }
+  gradient_columns(tc, qd, v, a, f, Mi, opt.damping, out);
+}
+
+// Reference-compatible mode (QDD_MINV_PASSED_IN, the reference's timed
+// configuration; dynamicsGradientKernel, helpers_GPU/dynamicsGradient.cuh:1257-1291):
+// qdd and a dense Minv are inputs; out is dqdd/dq, dqdd/dqd.  The upper triangle
+// of the given Minv is read (the same matrix for the symmetric Minv that
+// forwardDynamics produces).
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void fdgrad_knot_given_qdd_minv(const In& in, Out& out, const Options& opt) {
+  using SR = typename P::Rnea; using SC = typename P::Col;
+  using ST = Wider<SR, SC>;
+  CS<SR> tr[NQ]; CS<SC> tc[NQ];
+  SR qd[NQ], qdd[NQ];
+  for (int k = 0; k < NQ; ++k) {
+    ST s, c;
+    sincos_s(ST(in.q(k)), &s, &c);
+    tr[k] = CS<SR>{SR(c), SR(s)}; tc[k] = CS<SC>{SC(c), SC(s)};
+    qd[k] = SR(in.qd(k));
+    qdd[k] = SR(in.qdd(k));
+  }
+  SC Mi[NQ][NQ];
+  for (int r = 0; r < NQ; ++r)
+    for (int c = r; c < NQ; ++c) Mi[r][c] = SC(in.minv(r, c));
+  SR v[NQ][6], a[NQ][6], f[NQ][6];
+  rnea_fwd(tr, qd, qdd, SR(opt.gravity), v, a, f);
+  rnea_bwd(tr, f);
+  gradient_columns(tc, qd, v, a, f, Mi, opt.damping, out);
+}
+
+
+// ---------------------------------------------------------------------------
+// Memory layouts
+// ---------------------------------------------------------------------------
+constexpr int kInPerKnot = 21;     // q, qd, u
+constexpr int kOutPerKnot = 105;   // qdd, dqdd/dq (col-major), dqdd/dqd
+constexpr int kVolPerKnot = 77;    // q, qd, u, qdd, Minv dense (COMPLETELY_FUSED_VALS, .cuh:1145)
+constexpr int kDqddPerKnot = 98;   // dqdd/dq, dqdd/dqd
+
+// array-of-structures: one knot's values contiguous (the reference's knot<T> order)
+struct InAoS {
+  const float* p;
+  RBD_HD float q(int i) const { return p[i]; }
+  RBD_HD float qd(int i) const { return p[NQ + i]; }
+  RBD_HD float u(int i) const { return p[2 * NQ + i]; }
+  RBD_HD float qdd(int i) const { return p[3 * NQ + i]; }
+  RBD_HD float minv(int r, int c) const { return p[4 * NQ + r + NQ * c]; }
+};
+struct OutAoS {
+  float* p;
+  RBD_HD void qdd(int i, float x) const { p[i] = x; }
+  RBD_HD void dq(int r, int c, float x) const { p[NQ + NQ * c + r] = x; }
+  RBD_HD void dqd(int r, int c, float x) const { p[NQ + NQ * NQ + NQ * c + r] = x; }
+};
+struct OutDqddAoS {   // 98 per knot, no qdd
+  float* p;
+  RBD_HD void qdd(int, float) const {}
+  RBD_HD void dq(int r, int c, float x) const { p[NQ * c + r] = x; }
+  RBD_HD void dqd(int r, int c, float x) const { p[NQ * NQ + NQ * c + r] = x; }
+};
+// structure-of-arrays: value i of knot k at p[i * ld + k]
+struct InSoA {
+  const float* p; long ld;
+  RBD_HD float q(int i) const { return p[i * ld]; }
+  RBD_HD float qd(int i) const { return p[(NQ + i) * ld]; }
+  RBD_HD float u(int i) const { return p[(2 * NQ + i) * ld]; }
+};
+struct OutSoA {
+  float* p; long ld;
+  RBD_HD void qdd(int i, float x) const { p[i * ld] = x; }
+  RBD_HD void dq(int r, int c, float x) const { p[(NQ + NQ * c + r) * ld] = x; }
+  RBD_HD void dqd(int r, int c, float x) const { p[(NQ + NQ * NQ + NQ * c + r) * ld] = x; }
+};
+
+}  // namespace rbd

@@ -1,0 +1,131 @@
+/*
+ * rbdgrad.h -- C ABI of the MI355X (gfx950) forward-dynamics-gradient library
+ * for the 7-DoF KUKA iiwa 14 (librbdgrad.so).
+ *
+ * Per knot point:  q, qd, u  ->  qdd, dqdd/dq, dqdd/dqd   (FP32 in and out)
+ *
+ * The reference (plancherb1/fast-rbd-gradients) has no library or FFI: its
+ * per-trajectory entry points are C++ templates and a CUDA kernel over raw
+ * buffers.  Each function below names the reference interface it replaces.
+ *
+ * Layouts (FP32):
+ *   knot input   [q(7) | qd(7) | u(7)]                                21 floats
+ *   knot output  [qdd(7) | dqdd/dq (7x7) | dqdd/dqd (7x7)]            105 floats
+ *                7x7 blocks column-major: entry (r, c) = d qdd_r / d x_c at 7c + r
+ *                (the reference's dqdd layout, helpers_CPU/dynamicsGradient.h:839,921)
+ *   reference-compatible input  [q | qd | u | qdd | Minv dense 7x7]   77 floats
+ *   reference-compatible output [dqdd/dq | dqdd/dqd]                  98 floats
+ *                (COMPLETELY_FUSED_VALS, helpers_GPU/dynamicsGradient.cuh:1142-1146)
+ *
+ * Conventions:
+ *   - Status codes, never exit(); rbd_last_error() gives the detail of the last
+ *     failure on the calling thread.
+ *   - Device pointers (d_*) are caller-owned HIP device memory; host pointers
+ *     (h_*) are caller-owned host memory.  The library allocates no device memory
+ *     except in the host-pointer convenience call and the batch/serving objects,
+ *     which own what they allocate.
+ *   - Every launch goes on the caller's stream (a hipStream_t; NULL = default
+ *     stream) and returns without synchronising, like a kernel launch.
+ *   - A context is immutable after rbd_create and may be used from several
+ *     threads at once.
+ *   - Inputs must be finite.  A knot with a non-finite input gets unspecified
+ *     output values; no other knot is affected.
+ */
+#ifndef RBDGRAD_H
+#define RBDGRAD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rbd_context rbd_context;
+typedef void* rbd_stream_t; /* hipStream_t */
+
+typedef enum rbd_status {
+  RBD_OK = 0,
+  RBD_ERR_INVALID_ARGUMENT = 1,
+  RBD_ERR_HIP = 2,
+  RBD_ERR_NO_DEVICE = 3,
+  RBD_ERR_OUT_OF_MEMORY = 4,
+  RBD_ERR_UNSUPPORTED = 5,
+  RBD_ERR_CAPACITY = 6
+} rbd_status;
+
+/* model options: the reference's compile-time VEL_DAMPING and MPC_MODE
+ * (utils/experiment_helpers.h:19-22; both off by default there) */
+#define RBD_FLAG_VEL_DAMPING 1u /* joint damping 0.5*qd (helpers_CPU/dynamicsGradient.h:680,840) */
+#define RBD_FLAG_NO_GRAVITY 2u  /* gravity off, MPC_MODE (helpers_CPU/dynamicsGradient.h:635-637) */
+
+/* arithmetic precision (inputs and outputs are FP32 in every mode) */
+typedef enum rbd_precision {
+  RBD_PREC_FP32 = 0,         /* everything FP32 */
+  RBD_PREC_FP64_PIVOT = 1,   /* FP32, but the Minv pivots D_k = S^T IA_k S and 1/D_k in FP64 */
+  RBD_PREC_FP64_COLUMNS = 2, /* FP32 Minv and RNEA, FP64 derivative columns and -Minv*dtau */
+  RBD_PREC_FP64 = 3          /* everything FP64 */
+} rbd_precision;
+
+typedef struct rbd_config {
+  int device;        /* HIP device ordinal */
+  unsigned flags;    /* RBD_FLAG_* */
+  int precision;     /* rbd_precision */
+  int reserved[5];   /* must be zero */
+} rbd_config;
+
+/* rbd_config with device 0, flags 0 (gravity on, no damping), RBD_PREC_FP32. */
+void rbd_config_default(rbd_config* cfg);
+
+/* Context: device, model options, precision.  The robot constants are compiled
+ * into the kernels' instruction stream, so nothing is uploaded per call
+ * (the reference uploads its 504-float mem_const once, time_GPU.cu:174). */
+rbd_status rbd_create(const rbd_config* cfg, rbd_context** ctx);
+rbd_status rbd_destroy(rbd_context* ctx);
+
+const char* rbd_status_string(rbd_status s);
+const char* rbd_last_error(void);
+/* Library ABI version (incremented on incompatible changes). */
+int rbd_abi_version(void);
+
+/* Full path for n knot points, array-of-structures (21 in / 105 out per knot),
+ * one launch on `stream`.
+ * Replaces: the per-trajectory forward-dynamics gradient -- the reference
+ * computes it only on the CPU, forwardDynamicsGradientSetup with
+ * QDD_MINV_PASSED_IN = false + forwardDynamicsGradientThreaded
+ * (helpers_CPU/dynamicsGradient.h:868-897, :924-934), driven per trajectory by
+ * dynamicsGradientReusableThreaded (time_CPU.cpp:47-54). */
+rbd_status rbd_fdgrad_traj_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, float* d_out,
+                               rbd_stream_t stream);
+
+/* Same, structure-of-arrays: value i of knot k at d_in[i*ld_in + k] (i < 21) and
+ * output value j at d_out[j*ld_out + k] (j < 105); ld_* >= n_knots.
+ * Replaces: as rbd_fdgrad_traj_f32 (a layout the reference does not have). */
+rbd_status rbd_fdgrad_traj_soa_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, int64_t ld_in,
+                                   float* d_out, int64_t ld_out, rbd_stream_t stream);
+
+/* Reference-compatible mode: qdd and a dense Minv are inputs (77 per knot),
+ * dqdd/dq and dqdd/dqd out (98 per knot).  d_mem_const may be NULL; if given it
+ * must hold the reference's 504-float T|I constant block and is only checked
+ * for size by the caller's contract (the constants are compiled in).
+ * Replaces: dynamicsGradientKernel<T,N,MPC,VD><<<N,(32,7)>>>(d_dqdd, d_mem_vol,
+ * d_mem_const) (helpers_GPU/dynamicsGradient.cuh:1257-1291, launched at
+ * time_GPU.cu:314,325). */
+rbd_status rbd_dynamics_gradient_kernel_f32(rbd_context* ctx, int64_t n_knots, float* d_dqdd,
+                                            const float* d_mem_vol, const float* d_mem_const,
+                                            rbd_stream_t stream);
+
+/* Host-pointer convenience: copies h_in to the device, runs the full path,
+ * copies the result to h_out and waits (the reference's "GPU Compute + I/O"
+ * timing, time_GPU.cu:311-320).  Uses and keeps a per-context device buffer.
+ * Replaces: the H2D copy + kernel + D2H copy sequence of time_GPU.cu:313-316. */
+rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n_knots, const float* h_in, float* h_out);
+
+/* Name of the kernel the context dispatches to (full path, or the reference-
+ * compatible mode when reference_mode != 0), as profilers list it. */
+const char* rbd_kernel_name(rbd_context* ctx, int reference_mode);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* RBDGRAD_H */

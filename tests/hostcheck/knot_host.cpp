@@ -1,0 +1,80 @@
+// Test-only host build of the device per-knot arithmetic
+// (fast-rbd-gradients_amd/csrc/fdgrad_knot.hpp compiled by g++ with the same
+// floating-point flags as the HIP build).  Lets the CPU test suite check the
+// kernel's arithmetic, in every precision mode, against the oracle before any
+// GPU run; it is not a product path and nothing outside tests/ loads it.
+#include "../../fast-rbd-gradients_amd/csrc/fdgrad_knot.hpp"
+
+using namespace rbd;
+
+static Options options(int flags) {
+  Options o;
+  o.damping = (flags & 1) ? 0.5f : 0.0f;
+  o.gravity = (flags & 2) ? 0.0f : float(kGravity);
+  return o;
+}
+
+template <class P>
+static void run(bool ref, int flags, long n, const float* in, float* out) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    if (ref) {
+      InAoS i{in + kVolPerKnot * k};
+      OutDqddAoS o{out + kDqddPerKnot * k};
+      fdgrad_knot_given_qdd_minv<P>(i, o, opt);
+    } else {
+      InAoS i{in + kInPerKnot * k};
+      OutAoS o{out + kOutPerKnot * k};
+      fdgrad_knot<P>(i, o, opt);
+    }
+  }
+}
+
+static void by_precision(int prec, bool ref, int flags, long n, const float* in, float* out) {
+  switch (prec) {
+    case 0: run<PrecFP32>(ref, flags, n, in, out); break;
+    case 1: run<PrecFP64Pivot>(ref, flags, n, in, out); break;
+    case 2: run<PrecFP64Col>(ref, flags, n, in, out); break;
+    case 3: run<PrecFP64>(ref, flags, n, in, out); break;
+    default: break;
+  }
+}
+
+extern "C" {
+// The device model tables: spatial inertia of link k (6x6 column-major) and the
+// joint transform T_k(q) (6x6 column-major, built by applying T to unit vectors).
+void host_model_inertia(int k, double* out) {
+  float A[6][6];
+  auto load = [&](auto Kc) { iload<decltype(Kc)::value>(A); };
+  switch (k) {
+    case 0: load(IC<0>{}); break; case 1: load(IC<1>{}); break; case 2: load(IC<2>{}); break;
+    case 3: load(IC<3>{}); break; case 4: load(IC<4>{}); break; case 5: load(IC<5>{}); break;
+    default: load(IC<6>{}); break;
+  }
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) out[6 * c + r] = A[r][c];
+}
+void host_model_transform(int k, double q, double* out) {
+  const CS<double> t{std::cos(q), std::sin(q)};
+  auto apply = [&](auto Kc) {
+    for (int c = 0; c < 6; ++c) {
+      double e[6] = {0, 0, 0, 0, 0, 0}, y[6];
+      e[c] = 1.0;
+      xmotion<decltype(Kc)::value>(t, e, y);
+      for (int r = 0; r < 6; ++r) out[6 * c + r] = y[r];
+    }
+  };
+  switch (k) {
+    case 0: apply(IC<0>{}); break; case 1: apply(IC<1>{}); break; case 2: apply(IC<2>{}); break;
+    case 3: apply(IC<3>{}); break; case 4: apply(IC<4>{}); break; case 5: apply(IC<5>{}); break;
+    default: apply(IC<6>{}); break;
+  }
+}
+
+void host_fdgrad_f32(long n, const float* in, float* out, int flags, int prec) {
+  by_precision(prec, false, flags, n, in, out);
+}
+void host_fdgrad_refmode_f32(long n, const float* vol, float* out, int flags, int prec) {
+  by_precision(prec, true, flags, n, vol, out);
+}
+}

@@ -1,0 +1,77 @@
+"""The C-ABI library loads and exports every function include/rbdgrad.h declares
+(no GPU needed: nothing here launches work)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import common
+
+
+HEADER = os.path.join(common.ROOT, "include", "rbdgrad.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rbd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_what_the_binding_lists():
+    import rbdgrad
+    assert declared_functions() == sorted(rbdgrad.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    import rbdgrad
+    lib = rbdgrad.lib()                       # raises if librbdgrad.so is missing
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", rbdgrad.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\bT (rbd_\w+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_library_is_built_for_gfx950():
+    import rbdgrad
+    out = subprocess.run(["/opt/rocm/bin/roc-obj-ls", rbdgrad.LIB_PATH], capture_output=True, text=True)
+    if out.returncode != 0:
+        # fall back to scanning the offload bundle's target triple
+        data = open(rbdgrad.LIB_PATH, "rb").read()
+        assert b"gfx950" in data
+    else:
+        assert "gfx950" in out.stdout
+
+
+def test_status_strings_and_config_default():
+    import rbdgrad
+    L = rbdgrad.lib()
+    assert L.rbd_abi_version() == 1
+    for s, text in [(0, "ok"), (1, "invalid argument"), (3, "no HIP device")]:
+        assert L.rbd_status_string(s).decode() == text
+    cfg = rbdgrad.Config()
+    L.rbd_config_default(ctypes.byref(cfg))
+    assert (cfg.device, cfg.flags, cfg.precision) == (0, 0, 0)
+
+
+def test_invalid_arguments_are_reported_without_a_device():
+    import rbdgrad
+    L = rbdgrad.lib()
+    h = ctypes.c_void_p()
+    # null output pointer and bad config are rejected before any device call
+    assert L.rbd_create(None, None) == 1
+    cfg = rbdgrad.Config()
+    L.rbd_config_default(ctypes.byref(cfg))
+    cfg.precision = 9
+    assert L.rbd_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
+    assert "precision" in L.rbd_last_error().decode()
+    cfg.precision = 0
+    cfg.flags = 64
+    assert L.rbd_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
+    # null context on every entry point
+    assert L.rbd_fdgrad_traj_f32(None, 1, None, None, None) == 1
+    assert L.rbd_fdgrad_traj_soa_f32(None, 1, None, 1, None, 1, None) == 1
+    assert L.rbd_dynamics_gradient_kernel_f32(None, 1, None, None, None, None) == 1
+    assert L.rbd_fdgrad_traj_host_f32(None, 1, None, None) == 1
+    assert L.rbd_destroy(None) == 0

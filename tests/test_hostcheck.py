@@ -1,0 +1,88 @@
+"""CPU check of the kernel arithmetic: fast-rbd-gradients_amd/csrc/fdgrad_knot.hpp
+compiled for the host (tests/hostcheck) against the URDF data and the FP64
+oracle, in every precision mode, before anything runs on the GPU."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import common
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOSTLIB = os.path.join(HERE, "hostcheck", "libknot_host.so")
+
+
+@pytest.fixture(scope="module")
+def host():
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "hostcheck")], check=True)
+    L = ctypes.CDLL(HOSTLIB)
+    fp = ctypes.POINTER(ctypes.c_float)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.host_fdgrad_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int]
+    L.host_fdgrad_refmode_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int]
+    L.host_model_inertia.argtypes = [ctypes.c_int, dp]
+    L.host_model_transform.argtypes = [ctypes.c_int, ctypes.c_double, dp]
+    return L
+
+
+def run(host, x, flags=0, prec=0):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros((x.shape[0], 105), dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    host.host_fdgrad_f32(x.shape[0], x.ctypes.data_as(fp), out.ctypes.data_as(fp), flags, prec)
+    return out
+
+
+def test_device_model_tables_match_urdf(host):
+    model = common.UrdfModel()
+    buf = np.zeros(36)
+    dp = ctypes.POINTER(ctypes.c_double)
+    for k in range(7):
+        host.host_model_inertia(k, buf.ctypes.data_as(dp))
+        np.testing.assert_allclose(buf.reshape(6, 6).T, model.inertia(k), atol=1e-7)   # float32 constants
+        for q in (-2.5, -0.3, 0.0, 1.1, 3.0):
+            host.host_model_transform(k, q, buf.ctypes.data_as(dp))
+            np.testing.assert_allclose(buf.reshape(6, 6).T, model.transform(k, q), atol=1e-15)
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_host_build_matches_oracle(host, oracle, flags, prec):
+    x = common.knots(400, seed=200 + flags)
+    ref = oracle.fdgrad(x.astype(np.float64), flags)
+    err = common.max_block_errors(run(host, x, flags, prec), ref)
+    bound = common.REL_TOL if prec < 3 else 1e-6
+    assert max(err.values()) < bound, err
+
+
+def test_fp64_columns_fix_near_singular_poses(host, oracle):
+    """Stretched-arm poses: FP32 sits near the bar on dqdd/dqd; FP64 columns are
+    far below it (the measured reason for RBD_PREC_FP64_COLUMNS)."""
+    x = common.knots(3000, seed=210, near_singular_frac=1.0)
+    ref = oracle.fdgrad(x.astype(np.float64), threads=8)
+    e32 = common.max_block_errors(run(host, x, 0, 0), ref)
+    e64c = common.max_block_errors(run(host, x, 0, 2), ref)
+    assert max(e64c.values()) < 1e-5, e64c
+    assert e64c["dqdd_dqd"] < e32["dqdd_dqd"] / 4
+
+
+def test_reference_mode(host, oracle):
+    g = np.load(os.path.join(common.GOLDEN, "fdgrad_golden.npz"))
+    vol, ref = g["refmode_n128__in"], g["refmode_n128__out"]
+    out = np.zeros((vol.shape[0], 98), dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    host.host_fdgrad_refmode_f32(vol.shape[0], np.ascontiguousarray(vol).ctypes.data_as(fp),
+                                 out.ctypes.data_as(fp), 0, 0)
+    err = common.dqdd_block_errors(out, ref)
+    assert max(float(v.max()) for v in err.values()) < common.REL_TOL
+
+
+def test_golden_sets(host):
+    g = np.load(os.path.join(common.GOLDEN, "fdgrad_golden.npz"))
+    names = sorted({k.split("__")[0] for k in g.files if not k.startswith("refmode")})
+    for name in names:
+        x, flags, ref = g[name + "__in"], int(g[name + "__flags"]), g[name + "__out"]
+        err = common.max_block_errors(run(host, x, flags, 2), ref)
+        assert max(err.values()) < common.REL_TOL, (name, err)
