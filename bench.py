@@ -162,7 +162,9 @@ def main():
 
     stream = rbdgrad.Stream()
     ctx = rbdgrad.Context(device=0, precision=args.precision)
-    kname = ctx.kernel_name()
+    threshold, tune_table = ctx.autotune(stream)
+    log(f"autotune: lane-group mapping up to {threshold} knots; table (n: us one-lane, us lane-group) "
+        f"{ {k: (round(a, 2), round(b, 2)) for k, (a, b) in tune_table.items()} }")
 
     # ---- headline: per-call kernel, one trajectory of 128 knots per launch ----
     n1 = 128
@@ -173,9 +175,20 @@ def main():
     value = n1 * args.steps / (ms1 * 1e-3)
     gbs1, tfl1 = roofline(n1, ms_step)
     acc1 = accuracy(out1, x1)
-    log(f"[c1] {value:.3e} knots/s, {ms_step * 1e3:.2f} us/launch, acc {acc1}")
+    kname = ctx.kernel_name(n1)
+    log(f"[c1] {kname}: {value:.3e} knots/s, {ms_step * 1e3:.2f} us/launch, acc {acc1}")
 
-    workloads = {}
+    workloads = {"autotune": {"group_threshold_knots": threshold,
+                              "us_per_launch": {str(k): {"one_lane": a, "lane_group": b}
+                                                for k, (a, b) in tune_table.items()}}}
+    # the same per-call workload with the mapping forced to one lane per knot
+    one = rbdgrad.Context(device=0, precision=args.precision, lanes_per_knot=1)
+    ms1b, _ = run_flat(one, x1, args.steps // 4, args.warmup, stream)
+    workloads["per_call_n128_one_lane_per_knot"] = {
+        "kernel": one.kernel_name(n1), "us_per_launch": ms1b / (args.steps // 4) * 1e3,
+        "knots_per_s": n1 * (args.steps // 4) / (ms1b * 1e-3)}
+    log(f"[c1] one lane per knot: {workloads['per_call_n128_one_lane_per_knot']}")
+    one.close()
     if not args.quick:
         # ---- large batch: 2^20 knots, 10% stretched-arm poses ----
         n4 = 1 << 20
@@ -226,7 +239,8 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else args.precision,
         "data": "synthetic: seeded uniform q~U(-pi,pi), qd~U(-2,2), u~U(-50,50); iiwa 14 model constants",
         "config": {"workload": "per-call kernel, one trajectory of 128 knot points per launch (config 1)",
-                   "knots_per_call": n1, "layout": "aos", "precision": args.precision, "kernel": kname},
+                   "knots_per_call": n1, "layout": "aos", "precision": args.precision, "kernel": kname,
+                   "lane_group_threshold": threshold},
         "roofline": {"bound": "hbm", "achieved": gbs1, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs1 / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_knot": BYTES_PER_KNOT, "knots_per_launch": n1},

@@ -466,12 +466,61 @@ RBD_HD RBD_INLINE void minv_times(const S Mi[NQ][NQ], const SX x[NQ], SX out[NQ]
   });
 }
 
-// Phase C from v, a (with qdd), f (after the backward pass) and Minv:
-// the derivative columns and the stores of dqdd/dq, dqdd/dqd.
-template <typename SC, typename SR, typename SM, class Out>
-RBD_HD RBD_INLINE void gradient_columns(const CS<SC>* t, const SR* qd, const SR v[NQ][6], const SR a[NQ][6],
-                                        const SR f[NQ][6], const SM Mi[NQ][NQ], float damping, Out& out) {
-  PhaseC<SC> P;
+// Uniform-sweep variant of dtau_column for lane-parallel use: the column c and
+// its kind (dq) may differ from lane to lane while every lane executes the same
+// instructions.  dv, da start at zero and stay zero until k == c, where the seed
+// replaces them by select; the zero df_k for k < c then adds exactly nothing in
+// the backward sweep, so the result equals dtau_column's.
+template <typename S>
+RBD_HD RBD_INLINE void dtau_column_uniform(const PhaseC<S>& P, int c, bool dq, S tau[NQ]) {
+  S dv[6], da[6], df[NQ][6];
+  for (int i = 0; i < 6; ++i) { dv[i] = S(0); da[i] = S(0); }
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    S pv[6], pa[6];
+    if constexpr (k > 0) {
+      xmotion<k>(P.t[k], dv, pv);
+      xmotion<k>(P.t[k], da, pa);
+      const S qk = P.qd[k];
+      pa[0] += pv[1] * qk; pa[1] -= pv[0] * qk; pa[3] += pv[4] * qk; pa[4] -= pv[3] * qk;
+    } else {
+      for (int i = 0; i < 6; ++i) { pv[i] = S(0); pa[i] = S(0); }
+    }
+    // seeds: d/dqd -> dv = S, da = v_k x S;  d/dq -> dv = v_k x S, da = a_k x S
+    const S vx[6] = {P.v[k][1], -P.v[k][0], S(0), P.v[k][4], -P.v[k][3], S(0)};
+    const S ax[6] = {P.axs[k][0], P.axs[k][1], S(0), P.axs[k][2], P.axs[k][3], S(0)};
+    const bool seed = (k == c);
+    for (int i = 0; i < 6; ++i) {
+      const S sv = dq ? vx[i] : (i == 2 ? S(1) : S(0));
+      const S sa = dq ? ax[i] : vx[i];
+      dv[i] = seed ? sv : pv[i];
+      da[i] = seed ? sa : pa[i];
+    }
+    S Idv[6];
+    imv<k>(dv, Idv);
+    imv<k>(da, df[k]);
+    crf<true>(dv, P.Iv[k], df[k]);
+    crf<true>(P.v[k], Idv, df[k]);
+  });
+  S g[6];
+  for (int i = 0; i < 6; ++i) g[i] = df[NQ - 1][i];
+  static_for_down<NQ, 1>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    tau[k] = g[2];
+    const bool extra = dq && (k == c);
+    g[0] += extra ? P.nfxs[k][0] : S(0); g[1] += extra ? P.nfxs[k][1] : S(0);
+    g[3] += extra ? P.nfxs[k][2] : S(0); g[4] += extra ? P.nfxs[k][3] : S(0);
+    S y[6];
+    xforce_t<k>(P.t[k], g, y);
+    for (int i = 0; i < 6; ++i) g[i] = y[i] + df[k - 1][i];
+  });
+  tau[0] = g[2];
+}
+
+// Phase-C inputs from v, a (with qdd), f (after the backward pass) and Minv.
+template <typename SC, typename SR, typename SM>
+RBD_HD RBD_INLINE void make_phase_c(PhaseC<SC>& P, const CS<SC>* t, const SR* qd, const SR v[NQ][6],
+                                    const SR a[NQ][6], const SR f[NQ][6], const SM Mi[NQ][NQ]) {
   for (int k = 0; k < NQ; ++k) { P.t[k] = t[k]; P.qd[k] = SC(qd[k]); }
   for (int r = 0; r < NQ; ++r)
     for (int c = r; c < NQ; ++c) P.Mi[r][c] = SC(Mi[r][c]);
@@ -482,6 +531,12 @@ RBD_HD RBD_INLINE void gradient_columns(const CS<SC>* t, const SR* qd, const SR 
     P.axs[k][0] = SC(a[k][1]); P.axs[k][1] = -SC(a[k][0]); P.axs[k][2] = SC(a[k][4]); P.axs[k][3] = -SC(a[k][3]);
     P.nfxs[k][0] = -SC(f[k][1]); P.nfxs[k][1] = SC(f[k][0]); P.nfxs[k][2] = -SC(f[k][4]); P.nfxs[k][3] = SC(f[k][3]);
   });
+}
+
+// All derivative columns in one lane (one lane per knot) and the stores of
+// dqdd/dq, dqdd/dqd.
+template <typename SC, class Out>
+RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out& out) {
   // d/dq column 0 is identically zero (rotation about the gravity axis)
   for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
 #pragma clang loop unroll(disable)
@@ -496,6 +551,28 @@ RBD_HD RBD_INLINE void gradient_columns(const CS<SC>* t, const SR* qd, const SR 
       minv_times<true>(P.Mi, tau, col);
       for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
     }
+  }
+}
+
+// Lane roles when one knot is spread over a group of kLanesPerKnotGroup lanes:
+// role j < 7: d/dqd column j; 7 <= j < 13: d/dq column j - 6; j == 13: qdd and
+// the zero d/dq column 0; j >= 14: no stores.
+constexpr int kLanesPerKnotGroup = 16;
+constexpr int kRoleQdd = 13;
+
+// One derivative column for lane role `role` of a knot group and its store.
+template <typename SC, class Out>
+RBD_HD RBD_INLINE void gradient_column_lane(const PhaseC<SC>& P, float damping, int role, Out& out) {
+  const bool dq = role >= NQ;
+  const int c = dq ? role - (NQ - 1) : role;
+  SC tau[NQ], col[NQ];
+  dtau_column_uniform(P, c, dq, tau);
+  add_on_diagonal(tau, dq ? -1 : c, SC(damping));
+  minv_times<true>(P.Mi, tau, col);
+  if (role < NQ) {
+    for (int r = 0; r < NQ; ++r) out.dqd(r, c, float(col[r]));
+  } else if (role < 2 * NQ - 1) {
+    for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
   }
 }
 
@@ -514,9 +591,11 @@ struct Options {
   float gravity = float(kGravity);    // 0 in MPC_MODE (dynamicsGradient.h:635-637)
 };
 
-// Full path, P: arithmetic precision per phase.
-template <class P = PrecFP32, class In, class Out>
-RBD_HD RBD_INLINE void fdgrad_knot(const In& in, Out& out, const Options& opt) {
+// Steps 1-5 of the full path: sin/cos, Minv, bias RNEA, qdd = Minv (u - c), RNEA
+// with qdd; fills the phase-C inputs and qdd.
+template <class P, class In>
+RBD_HD RBD_INLINE void phase_a(const In& in, const Options& opt, PhaseC<typename P::Col>& PC,
+                               typename P::Rnea qdd[NQ]) {
   using SM = typename P::Minv; using SP = typename P::Pivot;
   using SR = typename P::Rnea; using SC = typename P::Col;
   using ST = Wider<Wider<SM, SR>, SC>;     // sin/cos computed once in the widest type
@@ -536,10 +615,9 @@ RBD_HD RBD_INLINE void fdgrad_knot(const In& in, Out& out, const Options& opt) {
   const SR zero[NQ] = {SR(0), SR(0), SR(0), SR(0), SR(0), SR(0), SR(0)};
   rnea_fwd(tr, qd, zero, SR(opt.gravity), v, a, f);
   rnea_bwd(tr, f);
-  SR bias[NQ], qdd[NQ];
+  SR bias[NQ];
   for (int k = 0; k < NQ; ++k) bias[k] = u[k] - f[k][2] - SR(opt.damping) * qd[k];
   minv_times<false>(Mi, bias, qdd);
-  for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
   // RNEA with qdd, added incrementally: a += da, f += bwd(I da), da_k = T da_{k-1} + S qdd_k
   {
     SR dA[6] = {SR(0), SR(0), qdd[0], SR(0), SR(0), SR(0)};
@@ -559,16 +637,12 @@ RBD_HD RBD_INLINE void fdgrad_knot(const In& in, Out& out, const Options& opt) {
     for (int k = 0; k < NQ; ++k)
       for (int i = 0; i < 6; ++i) f[k][i] += dF[k][i];
   }
-  gradient_columns(tc, qd, v, a, f, Mi, opt.damping, out);
+  make_phase_c(PC, tc, qd, v, a, f, Mi);
 }
 
-// Reference-compatible mode (QDD_MINV_PASSED_IN, the reference's timed
-// configuration; dynamicsGradientKernel, helpers_GPU/dynamicsGradient.cuh:1257-1291):
-// qdd and a dense Minv are inputs; out is dqdd/dq, dqdd/dqd.  The upper triangle
-// of the given Minv is read (the same matrix for the symmetric Minv that
-// forwardDynamics produces).
-template <class P = PrecFP32, class In, class Out>
-RBD_HD RBD_INLINE void fdgrad_knot_given_qdd_minv(const In& in, Out& out, const Options& opt) {
+// Steps 1 and 5 of the reference-compatible mode (qdd and Minv given).
+template <class P, class In>
+RBD_HD RBD_INLINE void phase_a_given_qdd_minv(const In& in, const Options& opt, PhaseC<typename P::Col>& PC) {
   using SR = typename P::Rnea; using SC = typename P::Col;
   using ST = Wider<SR, SC>;
   CS<SR> tr[NQ]; CS<SC> tc[NQ];
@@ -586,9 +660,55 @@ RBD_HD RBD_INLINE void fdgrad_knot_given_qdd_minv(const In& in, Out& out, const 
   SR v[NQ][6], a[NQ][6], f[NQ][6];
   rnea_fwd(tr, qd, qdd, SR(opt.gravity), v, a, f);
   rnea_bwd(tr, f);
-  gradient_columns(tc, qd, v, a, f, Mi, opt.damping, out);
+  make_phase_c(PC, tc, qd, v, a, f, Mi);
 }
 
+// Full path for one knot in one lane.  P: arithmetic precision per phase.
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void fdgrad_knot(const In& in, Out& out, const Options& opt) {
+  PhaseC<typename P::Col> PC;
+  typename P::Rnea qdd[NQ];
+  phase_a<P>(in, opt, PC, qdd);
+  for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
+  gradient_columns(PC, opt.damping, out);
+}
+
+// Full path for one knot spread over kLanesPerKnotGroup lanes: every lane of
+// the group runs steps 1-5 (identical values), then lane `role` computes one
+// derivative column (steps 6-7) and stores it.
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void fdgrad_knot_lanes(const In& in, Out& out, const Options& opt, int role) {
+  PhaseC<typename P::Col> PC;
+  typename P::Rnea qdd[NQ];
+  phase_a<P>(in, opt, PC, qdd);
+  if (role == kRoleQdd) {
+    for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
+    for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
+  }
+  gradient_column_lane(PC, opt.damping, role, out);
+}
+
+// Reference-compatible mode (QDD_MINV_PASSED_IN, the reference's timed
+// configuration; dynamicsGradientKernel, helpers_GPU/dynamicsGradient.cuh:1257-1291):
+// qdd and a dense Minv are inputs; out is dqdd/dq, dqdd/dqd.  The upper triangle
+// of the given Minv is read (the same matrix for the symmetric Minv that
+// forwardDynamics produces).
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void fdgrad_knot_given_qdd_minv(const In& in, Out& out, const Options& opt) {
+  PhaseC<typename P::Col> PC;
+  phase_a_given_qdd_minv<P>(in, opt, PC);
+  gradient_columns(PC, opt.damping, out);
+}
+
+// The same spread over a lane group.
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void fdgrad_knot_given_qdd_minv_lanes(const In& in, Out& out, const Options& opt, int role) {
+  PhaseC<typename P::Col> PC;
+  phase_a_given_qdd_minv<P>(in, opt, PC);
+  if (role == kRoleQdd)
+    for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
+  gradient_column_lane(PC, opt.damping, role, out);
+}
 
 // ---------------------------------------------------------------------------
 // Memory layouts

@@ -1,11 +1,17 @@
 // librbdgrad.so: HIP kernels for gfx950 and the C ABI declared in include/rbdgrad.h.
 //
-// Mapping: one lane per knot point.  Each lane runs the whole per-knot solver
-// (fdgrad_knot.hpp) out of registers: no LDS, no barriers, no cross-lane
+// Two mappings of knot points onto lanes, both running the per-knot solver
+// (fdgrad_knot.hpp) out of registers -- no LDS, no barriers, no cross-lane
 // traffic; the only memory traffic is the knot's 84 input bytes and its 420
-// output bytes.  A wave therefore processes 64 knots in lock step with uniform
-// control flow (the derivative-column loop runs over the same column index in
-// every lane).
+// output bytes:
+//   * one lane per knot (throughput): a wave computes 64 knots in lock step
+//     with uniform control flow (the derivative-column loop runs over the same
+//     column index in every lane);
+//   * one knot per 16-lane group (latency): every lane of the group runs the
+//     shared steps and then one derivative column, so a knot's critical path is
+//     the shared steps plus one column instead of all thirteen.
+// The context picks the mapping per call from the batch size (crossover set by
+// rbd_autotune or the built-in default).
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -34,10 +40,17 @@ constexpr int kBlock = 64;   // one wave per workgroup
 
 }  // namespace
 
+// Below this many knots a call uses the 16-lane-group mapping (when the
+// context's lanes_per_knot is 0); measured crossover on MI355X, refined by
+// rbd_autotune.
+constexpr int64_t kDefaultGroupThreshold = 8192;
+
 struct rbd_context {
   int device = 0;
   unsigned flags = 0;
   int precision = RBD_PREC_FP32;
+  int lanes_per_knot = 0;                        // 0 = by batch size, 1, or 16
+  std::atomic<int64_t> group_threshold{kDefaultGroupThreshold};
   rbd::Options opt;
   // device buffer of the host-pointer convenience call
   std::mutex host_mu;
@@ -83,51 +96,118 @@ __global__ __launch_bounds__(kBlock) void fdgrad_refmode_kernel(long n, const fl
   }
 }
 
+// One knot per group of kLanesPerKnotGroup lanes (4 knots per wave): the
+// low-latency mapping for small batches.  Every lane of a group runs the
+// shared steps, then one derivative column; the group's lanes finish together.
+template <class P>
+__global__ __launch_bounds__(kBlock) void fdgrad_aos_lanes_kernel(long n, const float* __restrict__ in,
+                                                                  float* __restrict__ out, Options opt) {
+  const int role = threadIdx.x % kLanesPerKnotGroup;
+  const long groups = long(gridDim.x) * (blockDim.x / kLanesPerKnotGroup);
+  for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / kLanesPerKnotGroup; k < n; k += groups) {
+    InAoS i{in + kInPerKnot * k};
+    OutAoS o{out + kOutPerKnot * k};
+    fdgrad_knot_lanes<P>(i, o, opt, role);
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void fdgrad_soa_lanes_kernel(long n, const float* __restrict__ in, long ld_in,
+                                                                  float* __restrict__ out, long ld_out, Options opt) {
+  const int role = threadIdx.x % kLanesPerKnotGroup;
+  const long groups = long(gridDim.x) * (blockDim.x / kLanesPerKnotGroup);
+  for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / kLanesPerKnotGroup; k < n; k += groups) {
+    InSoA i{in + k, ld_in};
+    OutSoA o{out + k, ld_out};
+    fdgrad_knot_lanes<P>(i, o, opt, role);
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void fdgrad_refmode_lanes_kernel(long n, const float* __restrict__ vol,
+                                                                      float* __restrict__ dqdd, Options opt) {
+  const int role = threadIdx.x % kLanesPerKnotGroup;
+  const long groups = long(gridDim.x) * (blockDim.x / kLanesPerKnotGroup);
+  for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / kLanesPerKnotGroup; k < n; k += groups) {
+    InAoS i{vol + kVolPerKnot * k};
+    OutDqddAoS o{dqdd + kDqddPerKnot * k};
+    fdgrad_knot_given_qdd_minv_lanes<P>(i, o, opt, role);
+  }
+}
+
 }  // namespace rbd
 
 namespace {
 
-// grid for n knots: one lane per knot, capped so huge batches grid-stride
-unsigned grid_for(int64_t n) {
-  const int64_t blocks = (n + kBlock - 1) / kBlock;
+// grid for n knots at `lanes` lanes per knot, capped so huge batches grid-stride
+unsigned grid_for(int64_t n, int lanes) {
+  const int64_t knots_per_block = kBlock / lanes;
+  const int64_t blocks = (n + knots_per_block - 1) / knots_per_block;
   const int64_t cap = int64_t(1) << 20;
   return unsigned(blocks < cap ? blocks : cap);
 }
 
-template <template <class> class Launch, class... A>
-hipError_t by_precision(int prec, A&&... a) {
-  switch (prec) {
-    case RBD_PREC_FP32: return Launch<rbd::PrecFP32>::go(a...);
-    case RBD_PREC_FP64_PIVOT: return Launch<rbd::PrecFP64Pivot>::go(a...);
-    case RBD_PREC_FP64_COLUMNS: return Launch<rbd::PrecFP64Col>::go(a...);
-    default: return Launch<rbd::PrecFP64>::go(a...);
+enum class Layout { AoS, SoA, Ref };
+
+struct FlatCall {
+  Layout layout;
+  int lanes;              // 1 or rbd::kLanesPerKnotGroup
+  int64_t n;
+  const float* in;
+  int64_t ld_in;          // SoA only
+  float* out;
+  int64_t ld_out;         // SoA only
+  rbd::Options opt;
+  hipStream_t stream;
+};
+
+template <class P>
+hipError_t launch_flat(const FlatCall& c) {
+  const dim3 grid(grid_for(c.n, c.lanes)), block(kBlock);
+  const long n = long(c.n);
+  if (c.lanes == 1) {
+    switch (c.layout) {
+      case Layout::AoS:
+        hipLaunchKernelGGL(rbd::fdgrad_aos_kernel<P>, grid, block, 0, c.stream, n, c.in, c.out, c.opt);
+        break;
+      case Layout::SoA:
+        hipLaunchKernelGGL(rbd::fdgrad_soa_kernel<P>, grid, block, 0, c.stream, n, c.in, long(c.ld_in), c.out,
+                           long(c.ld_out), c.opt);
+        break;
+      case Layout::Ref:
+        hipLaunchKernelGGL(rbd::fdgrad_refmode_kernel<P>, grid, block, 0, c.stream, n, c.in, c.out, c.opt);
+        break;
+    }
+  } else {
+    switch (c.layout) {
+      case Layout::AoS:
+        hipLaunchKernelGGL(rbd::fdgrad_aos_lanes_kernel<P>, grid, block, 0, c.stream, n, c.in, c.out, c.opt);
+        break;
+      case Layout::SoA:
+        hipLaunchKernelGGL(rbd::fdgrad_soa_lanes_kernel<P>, grid, block, 0, c.stream, n, c.in, long(c.ld_in), c.out,
+                           long(c.ld_out), c.opt);
+        break;
+      case Layout::Ref:
+        hipLaunchKernelGGL(rbd::fdgrad_refmode_lanes_kernel<P>, grid, block, 0, c.stream, n, c.in, c.out, c.opt);
+        break;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_flat(int precision, const FlatCall& c) {
+  switch (precision) {
+    case RBD_PREC_FP32: return launch_flat<rbd::PrecFP32>(c);
+    case RBD_PREC_FP64_PIVOT: return launch_flat<rbd::PrecFP64Pivot>(c);
+    case RBD_PREC_FP64_COLUMNS: return launch_flat<rbd::PrecFP64Col>(c);
+    default: return launch_flat<rbd::PrecFP64>(c);
   }
 }
 
-template <class P>
-struct LaunchAoS {
-  static hipError_t go(int64_t n, const float* in, float* out, const rbd::Options& opt, hipStream_t s) {
-    hipLaunchKernelGGL(rbd::fdgrad_aos_kernel<P>, dim3(grid_for(n)), dim3(kBlock), 0, s, long(n), in, out, opt);
-    return hipGetLastError();
-  }
-};
-template <class P>
-struct LaunchSoA {
-  static hipError_t go(int64_t n, const float* in, int64_t ld_in, float* out, int64_t ld_out,
-                       const rbd::Options& opt, hipStream_t s) {
-    hipLaunchKernelGGL(rbd::fdgrad_soa_kernel<P>, dim3(grid_for(n)), dim3(kBlock), 0, s, long(n), in,
-                       long(ld_in), out, long(ld_out), opt);
-    return hipGetLastError();
-  }
-};
-template <class P>
-struct LaunchRef {
-  static hipError_t go(int64_t n, const float* vol, float* dqdd, const rbd::Options& opt, hipStream_t s) {
-    hipLaunchKernelGGL(rbd::fdgrad_refmode_kernel<P>, dim3(grid_for(n)), dim3(kBlock), 0, s, long(n), vol, dqdd,
-                       opt);
-    return hipGetLastError();
-  }
-};
+int lanes_for(const rbd_context* ctx, int64_t n) {
+  if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
+  return n <= ctx->group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
+}
 
 rbd_status check_ctx(rbd_context* ctx) {
   if (!ctx) return fail(RBD_ERR_INVALID_ARGUMENT, "null context");
@@ -189,6 +269,8 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
     return fail(RBD_ERR_INVALID_ARGUMENT, "unknown flag bits");
   for (int r : c.reserved)
     if (r != 0) return fail(RBD_ERR_INVALID_ARGUMENT, "reserved config fields must be zero");
+  if (c.lanes_per_knot != 0 && c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup)
+    return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 0, 1 or 16");
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev <= 0) return fail(RBD_ERR_NO_DEVICE, "no HIP device visible");
@@ -198,6 +280,7 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   ctx->device = c.device;
   ctx->flags = c.flags;
   ctx->precision = c.precision;
+  ctx->lanes_per_knot = c.lanes_per_knot;
   ctx->opt.damping = (c.flags & RBD_FLAG_VEL_DAMPING) ? 0.5f : 0.0f;
   ctx->opt.gravity = (c.flags & RBD_FLAG_NO_GRAVITY) ? 0.0f : float(rbd::kGravity);
   *out = ctx;
@@ -223,8 +306,9 @@ rbd_status rbd_fdgrad_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, f
   if (n == 0) return RBD_OK;
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (rbd_status s = use_device(ctx)) return s;
-  hipError_t e = by_precision<LaunchAoS>(ctx->precision, n, d_in, d_out, ctx->opt, (hipStream_t)stream);
-  return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_aos_kernel launch");
+  const FlatCall c{Layout::AoS, lanes_for(ctx, n), n, d_in, 0, d_out, 0, ctx->opt, (hipStream_t)stream};
+  hipError_t e = launch_flat(ctx->precision, c);
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
 }
 
 rbd_status rbd_fdgrad_traj_soa_f32(rbd_context* ctx, int64_t n, const float* d_in, int64_t ld_in, float* d_out,
@@ -235,9 +319,9 @@ rbd_status rbd_fdgrad_traj_soa_f32(rbd_context* ctx, int64_t n, const float* d_i
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (ld_in < n || ld_out < n) return fail(RBD_ERR_INVALID_ARGUMENT, "leading dimension smaller than knot count");
   if (rbd_status s = use_device(ctx)) return s;
-  hipError_t e = by_precision<LaunchSoA>(ctx->precision, n, d_in, ld_in, d_out, ld_out, ctx->opt,
-                                         (hipStream_t)stream);
-  return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_soa_kernel launch");
+  const FlatCall c{Layout::SoA, lanes_for(ctx, n), n, d_in, ld_in, d_out, ld_out, ctx->opt, (hipStream_t)stream};
+  hipError_t e = launch_flat(ctx->precision, c);
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
 }
 
 rbd_status rbd_dynamics_gradient_kernel_f32(rbd_context* ctx, int64_t n, float* d_dqdd, const float* d_mem_vol,
@@ -248,8 +332,9 @@ rbd_status rbd_dynamics_gradient_kernel_f32(rbd_context* ctx, int64_t n, float* 
   if (n == 0) return RBD_OK;
   if (!d_dqdd || !d_mem_vol) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (rbd_status s = use_device(ctx)) return s;
-  hipError_t e = by_precision<LaunchRef>(ctx->precision, n, d_mem_vol, d_dqdd, ctx->opt, (hipStream_t)stream);
-  return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_refmode_kernel launch");
+  const FlatCall c{Layout::Ref, lanes_for(ctx, n), n, d_mem_vol, 0, d_dqdd, 0, ctx->opt, (hipStream_t)stream};
+  hipError_t e = launch_flat(ctx->precision, c);
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
 }
 
 rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_in, float* h_out) {
@@ -273,20 +358,86 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
   float* d_out = ctx->d_scratch + size_t(n) * rbd::kInPerKnot;
   hipError_t e = hipMemcpy(d_in, h_in, in_bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
-  e = by_precision<LaunchAoS>(ctx->precision, n, d_in, d_out, ctx->opt, (hipStream_t)0);
-  if (e != hipSuccess) return hip_fail(e, "fdgrad_aos_kernel launch");
+  const FlatCall c{Layout::AoS, lanes_for(ctx, n), n, d_in, 0, d_out, 0, ctx->opt, (hipStream_t)0};
+  e = launch_flat(ctx->precision, c);
+  if (e != hipSuccess) return hip_fail(e, "fdgrad kernel launch");
   e = hipMemcpy(h_out, d_out, out_bytes, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy D2H");
   return RBD_OK;
 }
 
-const char* rbd_kernel_name(rbd_context* ctx, int reference_mode) {
+const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knots) {
   if (!ctx) return "";
-  static const char* aos[] = {"fdgrad_aos_kernel<PrecFP32>", "fdgrad_aos_kernel<PrecFP64Pivot>",
-                              "fdgrad_aos_kernel<PrecFP64Col>", "fdgrad_aos_kernel<PrecFP64>"};
-  static const char* ref[] = {"fdgrad_refmode_kernel<PrecFP32>", "fdgrad_refmode_kernel<PrecFP64Pivot>",
-                              "fdgrad_refmode_kernel<PrecFP64Col>", "fdgrad_refmode_kernel<PrecFP64>"};
-  return reference_mode ? ref[ctx->precision] : aos[ctx->precision];
+  static const char* names[2][2][4] = {
+      {{"fdgrad_aos_kernel<PrecFP32>", "fdgrad_aos_kernel<PrecFP64Pivot>", "fdgrad_aos_kernel<PrecFP64Col>",
+        "fdgrad_aos_kernel<PrecFP64>"},
+       {"fdgrad_aos_lanes_kernel<PrecFP32>", "fdgrad_aos_lanes_kernel<PrecFP64Pivot>",
+        "fdgrad_aos_lanes_kernel<PrecFP64Col>", "fdgrad_aos_lanes_kernel<PrecFP64>"}},
+      {{"fdgrad_refmode_kernel<PrecFP32>", "fdgrad_refmode_kernel<PrecFP64Pivot>",
+        "fdgrad_refmode_kernel<PrecFP64Col>", "fdgrad_refmode_kernel<PrecFP64>"},
+       {"fdgrad_refmode_lanes_kernel<PrecFP32>", "fdgrad_refmode_lanes_kernel<PrecFP64Pivot>",
+        "fdgrad_refmode_lanes_kernel<PrecFP64Col>", "fdgrad_refmode_lanes_kernel<PrecFP64>"}}};
+  const int grouped = lanes_for(ctx, n_knots) == rbd::kLanesPerKnotGroup ? 1 : 0;
+  return names[reference_mode ? 1 : 0][grouped][ctx->precision];
+}
+
+int64_t rbd_group_threshold(rbd_context* ctx) { return ctx ? ctx->group_threshold.load() : -1; }
+
+rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative threshold");
+  ctx->group_threshold.store(n_knots);
+  return RBD_OK;
+}
+
+rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
+                        int table_len) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (rbd_status s = use_device(ctx)) return s;
+  static const int64_t sizes[] = {64, 256, 1024, 2048, 4096, 8192, 16384, 32768, 65536};
+  constexpr int kSizes = sizeof(sizes) / sizeof(sizes[0]);
+  const int64_t nmax = sizes[kSizes - 1];
+  float* buf = nullptr;
+  hipError_t e = hipMalloc(&buf, size_t(nmax) * (rbd::kInPerKnot + rbd::kOutPerKnot) * sizeof(float));
+  if (e != hipSuccess) return fail(RBD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  float* in = buf;
+  float* out = buf + size_t(nmax) * rbd::kInPerKnot;
+  // any finite inputs do: the work per knot does not depend on the values
+  e = hipMemsetAsync(in, 0, size_t(nmax) * rbd::kInPerKnot * sizeof(float), (hipStream_t)stream);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  int64_t threshold = 0;
+  float best_us[2][kSizes] = {};
+  for (int si = 0; si < kSizes && e == hipSuccess; ++si) {
+    for (int li = 0; li < 2 && e == hipSuccess; ++li) {
+      const FlatCall c{Layout::AoS, li ? rbd::kLanesPerKnotGroup : 1, sizes[si], in, 0, out, 0, ctx->opt,
+                       (hipStream_t)stream};
+      float best = 1e30f;
+      for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
+        e = launch_flat(ctx->precision, c);                       // warm
+        if (e == hipSuccess) e = hipEventRecord(t0, (hipStream_t)stream);
+        for (int it = 0; it < 4 && e == hipSuccess; ++it) e = launch_flat(ctx->precision, c);
+        if (e == hipSuccess) e = hipEventRecord(t1, (hipStream_t)stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e == hipSuccess && ms * 250.0f < best) best = ms * 250.0f;   // us per launch
+      }
+      best_us[li][si] = best;
+    }
+    if (e == hipSuccess && best_us[1][si] < best_us[0][si]) threshold = sizes[si];
+  }
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return hip_fail(e, "autotune");
+  ctx->group_threshold.store(threshold);
+  if (threshold_out) *threshold_out = threshold;
+  if (table_us)
+    for (int i = 0; i < table_len && i < 3 * kSizes; ++i)
+      table_us[i] = (i % 3 == 0) ? float(sizes[i / 3]) : best_us[(i % 3) - 1][i / 3];
+  return RBD_OK;
 }
 
 }  // extern "C"

@@ -34,6 +34,7 @@ EXPORTED = [
     "rbd_config_default", "rbd_create", "rbd_destroy", "rbd_status_string", "rbd_last_error",
     "rbd_abi_version", "rbd_fdgrad_traj_f32", "rbd_fdgrad_traj_soa_f32",
     "rbd_dynamics_gradient_kernel_f32", "rbd_fdgrad_traj_host_f32", "rbd_kernel_name",
+    "rbd_autotune", "rbd_group_threshold", "rbd_set_group_threshold",
 ]
 
 
@@ -47,7 +48,7 @@ class HipError(RuntimeError):
 
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("flags", ctypes.c_uint), ("precision", ctypes.c_int),
-                ("reserved", ctypes.c_int * 5)]
+                ("lanes_per_knot", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
 
 
 _lib = None
@@ -74,8 +75,12 @@ def lib():
         L.rbd_fdgrad_traj_soa_f32.argtypes = [vp, i64, vp, i64, vp, i64, vp]
         L.rbd_dynamics_gradient_kernel_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_fdgrad_traj_host_f32.argtypes = [vp, i64, vp, vp]
-        L.rbd_kernel_name.argtypes = [vp, ctypes.c_int]
+        L.rbd_kernel_name.argtypes = [vp, ctypes.c_int, i64]
         L.rbd_kernel_name.restype = ctypes.c_char_p
+        L.rbd_autotune.argtypes = [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+        L.rbd_group_threshold.argtypes = [vp]
+        L.rbd_group_threshold.restype = i64
+        L.rbd_set_group_threshold.argtypes = [vp, i64]
         _lib = L
     return _lib
 
@@ -223,13 +228,14 @@ def _sh(stream):
 class Context:
     """rbd_context: device, model flags, precision mode."""
 
-    def __init__(self, device=0, flags=0, precision=PREC_FP32):
+    def __init__(self, device=0, flags=0, precision=PREC_FP32, lanes_per_knot=0):
         L = lib()
         cfg = Config()
         L.rbd_config_default(ctypes.byref(cfg))
         cfg.device = device
         cfg.flags = flags
         cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        cfg.lanes_per_knot = int(lanes_per_knot)
         self._h = ctypes.c_void_p()
         _check(L.rbd_create(ctypes.byref(cfg), ctypes.byref(self._h)), "rbd_create")
         self.flags = flags
@@ -264,8 +270,25 @@ class Context:
                                                       d_const.handle if d_const is not None else None,
                                                       _sh(stream)), "rbd_dynamics_gradient_kernel_f32")
 
-    def kernel_name(self, reference_mode=False):
-        return lib().rbd_kernel_name(self._h, int(bool(reference_mode))).decode()
+    def kernel_name(self, n_knots, reference_mode=False):
+        return lib().rbd_kernel_name(self._h, int(bool(reference_mode)), int(n_knots)).decode()
+
+    def autotune(self, stream=None):
+        """Time both knot-to-lane mappings on the device; sets and returns the
+        crossover plus the table {n: (us one-lane, us lane-group)}."""
+        thr = ctypes.c_int64()
+        tab = (ctypes.c_float * 27)()
+        _check(lib().rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 27), "rbd_autotune")
+        table = {int(tab[i]): (tab[i + 1], tab[i + 2]) for i in range(0, 27, 3)}
+        return thr.value, table
+
+    @property
+    def group_threshold(self):
+        return lib().rbd_group_threshold(self._h)
+
+    @group_threshold.setter
+    def group_threshold(self, n):
+        _check(lib().rbd_set_group_threshold(self._h, int(n)), "rbd_set_group_threshold")
 
     # --- host-array conveniences (upload, run, download) ---
     def fdgrad_host(self, x):

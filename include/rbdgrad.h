@@ -67,13 +67,16 @@ typedef enum rbd_precision {
 } rbd_precision;
 
 typedef struct rbd_config {
-  int device;        /* HIP device ordinal */
-  unsigned flags;    /* RBD_FLAG_* */
-  int precision;     /* rbd_precision */
-  int reserved[5];   /* must be zero */
+  int device;         /* HIP device ordinal */
+  unsigned flags;     /* RBD_FLAG_* */
+  int precision;      /* rbd_precision */
+  int lanes_per_knot; /* 0: chosen per call from the batch size; 1: one lane per
+                         knot (throughput); 16: one knot per 16-lane group (latency) */
+  int reserved[4];    /* must be zero */
 } rbd_config;
 
-/* rbd_config with device 0, flags 0 (gravity on, no damping), RBD_PREC_FP32. */
+/* rbd_config with device 0, flags 0 (gravity on, no damping), RBD_PREC_FP32,
+ * lanes_per_knot 0. */
 void rbd_config_default(rbd_config* cfg);
 
 /* Context: device, model options, precision.  The robot constants are compiled
@@ -120,9 +123,24 @@ rbd_status rbd_dynamics_gradient_kernel_f32(rbd_context* ctx, int64_t n_knots, f
  * Replaces: the H2D copy + kernel + D2H copy sequence of time_GPU.cu:313-316. */
 rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n_knots, const float* h_in, float* h_out);
 
-/* Name of the kernel the context dispatches to (full path, or the reference-
- * compatible mode when reference_mode != 0), as profilers list it. */
-const char* rbd_kernel_name(rbd_context* ctx, int reference_mode);
+/* Name of the kernel the context dispatches a call of n_knots to (full path,
+ * or the reference-compatible mode when reference_mode != 0), as profilers list
+ * it (template arguments abbreviated).  No reference counterpart. */
+const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knots);
+
+/* Autotuner for the knot-to-lane mapping (lanes_per_knot = 0): times both
+ * mappings on the device at batch sizes 64 ... 65536 and sets the largest size
+ * at which the 16-lane-group mapping is faster as the context's crossover.
+ * Allocates and frees a temporary device buffer; synchronises `stream`.  Not
+ * to be called while other threads launch on the same context.  table_us, if
+ * not NULL, receives up to table_len floats as (n, us one-lane, us lane-group)
+ * triples.  No reference counterpart (the reference fixes dim3(32,7) per knot,
+ * time_GPU.cu:10-19). */
+rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
+                        int table_len);
+/* Current crossover (knots) and its manual override. */
+int64_t rbd_group_threshold(rbd_context* ctx);
+rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -14,28 +14,36 @@ static Options options(int flags) {
   return o;
 }
 
+// lanes != 0: emulate the lane-group mapping, running every lane role of a
+// knot in turn (each role stores its own part of the output).
 template <class P>
-static void run(bool ref, int flags, long n, const float* in, float* out) {
+static void run(bool ref, int flags, long n, const float* in, float* out, int lanes) {
   const Options opt = options(flags);
   for (long k = 0; k < n; ++k) {
     if (ref) {
       InAoS i{in + kVolPerKnot * k};
       OutDqddAoS o{out + kDqddPerKnot * k};
-      fdgrad_knot_given_qdd_minv<P>(i, o, opt);
+      if (lanes)
+        for (int role = 0; role < kLanesPerKnotGroup; ++role) fdgrad_knot_given_qdd_minv_lanes<P>(i, o, opt, role);
+      else
+        fdgrad_knot_given_qdd_minv<P>(i, o, opt);
     } else {
       InAoS i{in + kInPerKnot * k};
       OutAoS o{out + kOutPerKnot * k};
-      fdgrad_knot<P>(i, o, opt);
+      if (lanes)
+        for (int role = 0; role < kLanesPerKnotGroup; ++role) fdgrad_knot_lanes<P>(i, o, opt, role);
+      else
+        fdgrad_knot<P>(i, o, opt);
     }
   }
 }
 
-static void by_precision(int prec, bool ref, int flags, long n, const float* in, float* out) {
+static void by_precision(int prec, bool ref, int flags, long n, const float* in, float* out, int lanes) {
   switch (prec) {
-    case 0: run<PrecFP32>(ref, flags, n, in, out); break;
-    case 1: run<PrecFP64Pivot>(ref, flags, n, in, out); break;
-    case 2: run<PrecFP64Col>(ref, flags, n, in, out); break;
-    case 3: run<PrecFP64>(ref, flags, n, in, out); break;
+    case 0: run<PrecFP32>(ref, flags, n, in, out, lanes); break;
+    case 1: run<PrecFP64Pivot>(ref, flags, n, in, out, lanes); break;
+    case 2: run<PrecFP64Col>(ref, flags, n, in, out, lanes); break;
+    case 3: run<PrecFP64>(ref, flags, n, in, out, lanes); break;
     default: break;
   }
 }
@@ -71,10 +79,10 @@ void host_model_transform(int k, double q, double* out) {
   }
 }
 
-void host_fdgrad_f32(long n, const float* in, float* out, int flags, int prec) {
-  by_precision(prec, false, flags, n, in, out);
+void host_fdgrad_f32(long n, const float* in, float* out, int flags, int prec, int lanes) {
+  by_precision(prec, false, flags, n, in, out, lanes);
 }
-void host_fdgrad_refmode_f32(long n, const float* vol, float* out, int flags, int prec) {
-  by_precision(prec, true, flags, n, vol, out);
+void host_fdgrad_refmode_f32(long n, const float* vol, float* out, int flags, int prec, int lanes) {
+  by_precision(prec, true, flags, n, vol, out, lanes);
 }
 }

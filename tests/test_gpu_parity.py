@@ -29,40 +29,70 @@ def say(*a):
 
 @pytest.fixture(scope="module")
 def contexts(rbd):
+    """(flags, precision) -> context with the mapping chosen per call;
+    (flags, precision, lanes) -> context with the mapping forced."""
     ctxs = {}
     for flags in (0, 1, 2, 3):
         for p in PRECISIONS:
             ctxs[(flags, p)] = rbd.Context(flags=flags, precision=p)
+            for lanes in (1, 16):
+                ctxs[(flags, p, lanes)] = rbd.Context(flags=flags, precision=p, lanes_per_knot=lanes)
     yield ctxs
     for c in ctxs.values():
         c.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 16])
 @pytest.mark.parametrize("prec", PRECISIONS)
 @pytest.mark.parametrize("name", SETS)
-def test_golden_aos(contexts, name, prec):
+def test_golden_aos(contexts, name, prec, lanes):
     x, flags, ref = GOLDEN[name + "__in"], int(GOLDEN[name + "__flags"]), GOLDEN[name + "__out"]
-    say(f"golden {name} n={len(x)} flags={flags} prec={prec}")
-    out = contexts[(flags, prec)].run_aos(x)
+    say(f"golden {name} n={len(x)} flags={flags} prec={prec} lanes={lanes}")
+    out = contexts[(flags, prec, lanes)].run_aos(x)
     err = common.max_block_errors(out, ref)
     assert max(err.values()) < BOUND[prec], err
 
 
+@pytest.mark.parametrize("lanes", [1, 16])
 @pytest.mark.parametrize("prec", ["fp32", "fp64_columns"])
-def test_reference_mode_golden(contexts, prec):
+def test_reference_mode_golden(contexts, prec, lanes):
     vol, ref = GOLDEN["refmode_n128__in"], GOLDEN["refmode_n128__out"]
-    say(f"refmode n={len(vol)} prec={prec}")
-    out = contexts[(0, prec)].run_refmode(vol)
+    say(f"refmode n={len(vol)} prec={prec} lanes={lanes}")
+    out = contexts[(0, prec, lanes)].run_refmode(vol)
     err = common.dqdd_block_errors(out, ref)
     assert max(float(v.max()) for v in err.values()) < BOUND[prec]
 
 
+@pytest.mark.parametrize("lanes", [1, 16])
 @pytest.mark.parametrize("prec", PRECISIONS)
-def test_soa_equals_aos_bitwise(contexts, prec):
+def test_soa_equals_aos_bitwise(contexts, prec, lanes):
     x = common.knots(1000, seed=300)
-    say(f"soa vs aos n=1000 prec={prec}")
-    ctx = contexts[(0, prec)]
+    say(f"soa vs aos n=1000 prec={prec} lanes={lanes}")
+    ctx = contexts[(0, prec, lanes)]
     np.testing.assert_array_equal(ctx.run_soa(x, pad=24), ctx.run_aos(x))
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+def test_lane_group_matches_one_lane(contexts, prec):
+    """Both knot-to-lane mappings compute the same numbers (to FP32 rounding of
+    a different instruction schedule) and every output value is written."""
+    x = common.knots(300, seed=306)
+    a = contexts[(0, prec, 1)].run_aos(x)
+    b = contexts[(0, prec, 16)].run_aos(x)
+    assert np.all(np.isfinite(b))
+    err = common.max_block_errors(b, a.astype(np.float64))
+    assert max(err.values()) < 1e-5, err
+
+
+def test_autotune_sets_a_crossover(rbd):
+    ctx = rbd.Context(precision="fp32")
+    thr, table = ctx.autotune()
+    say("autotune crossover", thr, table)
+    assert ctx.group_threshold == thr
+    assert all(t1 > 0 and t16 > 0 for t1, t16 in table.values())
+    small = min(table)
+    assert table[small][1] < table[small][0]          # small batches: lane groups are faster
+    ctx.close()
 
 
 def test_host_pointer_call_equals_device_call(contexts):

@@ -20,18 +20,18 @@ def host():
     L = ctypes.CDLL(HOSTLIB)
     fp = ctypes.POINTER(ctypes.c_float)
     dp = ctypes.POINTER(ctypes.c_double)
-    L.host_fdgrad_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int]
-    L.host_fdgrad_refmode_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int]
+    L.host_fdgrad_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.host_fdgrad_refmode_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.host_model_inertia.argtypes = [ctypes.c_int, dp]
     L.host_model_transform.argtypes = [ctypes.c_int, ctypes.c_double, dp]
     return L
 
 
-def run(host, x, flags=0, prec=0):
+def run(host, x, flags=0, prec=0, lanes=0):
     x = np.ascontiguousarray(x, dtype=np.float32)
-    out = np.zeros((x.shape[0], 105), dtype=np.float32)
+    out = np.full((x.shape[0], 105), np.nan, dtype=np.float32)
     fp = ctypes.POINTER(ctypes.c_float)
-    host.host_fdgrad_f32(x.shape[0], x.ctypes.data_as(fp), out.ctypes.data_as(fp), flags, prec)
+    host.host_fdgrad_f32(x.shape[0], x.ctypes.data_as(fp), out.ctypes.data_as(fp), flags, prec, lanes)
     return out
 
 
@@ -73,10 +73,12 @@ def test_reference_mode(host, oracle):
     vol, ref = g["refmode_n128__in"], g["refmode_n128__out"]
     out = np.zeros((vol.shape[0], 98), dtype=np.float32)
     fp = ctypes.POINTER(ctypes.c_float)
-    host.host_fdgrad_refmode_f32(vol.shape[0], np.ascontiguousarray(vol).ctypes.data_as(fp),
-                                 out.ctypes.data_as(fp), 0, 0)
-    err = common.dqdd_block_errors(out, ref)
-    assert max(float(v.max()) for v in err.values()) < common.REL_TOL
+    for lanes in (0, 1):
+        out[:] = np.nan
+        host.host_fdgrad_refmode_f32(vol.shape[0], np.ascontiguousarray(vol).ctypes.data_as(fp),
+                                     out.ctypes.data_as(fp), 0, 0, lanes)
+        err = common.dqdd_block_errors(out, ref)
+        assert max(float(v.max()) for v in err.values()) < common.REL_TOL
 
 
 def test_golden_sets(host):
@@ -86,3 +88,17 @@ def test_golden_sets(host):
         x, flags, ref = g[name + "__in"], int(g[name + "__flags"]), g[name + "__out"]
         err = common.max_block_errors(run(host, x, flags, 2), ref)
         assert max(err.values()) < common.REL_TOL, (name, err)
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+@pytest.mark.parametrize("prec", [0, 2])
+def test_lane_group_mapping_matches_single_lane(host, oracle, flags, prec):
+    """The lane-group mapping writes every output value (none left NaN) and
+    agrees with the one-lane mapping and with the oracle."""
+    x = common.knots(200, seed=220 + flags)
+    one = run(host, x, flags, prec, lanes=0)
+    grp = run(host, x, flags, prec, lanes=1)
+    assert np.all(np.isfinite(grp))
+    np.testing.assert_allclose(grp, one, rtol=1e-5, atol=1e-6 * np.abs(one).max())
+    err = common.max_block_errors(grp, oracle.fdgrad(x.astype(np.float64), flags))
+    assert max(err.values()) < common.REL_TOL, err
