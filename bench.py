@@ -142,6 +142,45 @@ def mixed_trajectories(seed=7, count=64):
     return rng.integers(8, 257, size=count)
 
 
+def mixed_workload(ctx, stream, steps=500):
+    """Config 2: 64 trajectories with lengths U{8..256}, each in its own device
+    buffers.  (a) the plain per-call kernel, one launch per trajectory; (b) the
+    batched form: one persistent-kernel launch over the device work queue per
+    batch, replayed from a hipGraph."""
+    lens = mixed_trajectories()
+    n2 = int(lens.sum())
+    x2 = knots(n2, seed=102)
+    offs = np.concatenate([[0], np.cumsum(lens)])
+    bufs = [(rbdgrad.DeviceBuffer.from_array(x2[a:b]), rbdgrad.DeviceBuffer((b - a) * rbdgrad.OUT_PER_KNOT * 4),
+             int(b - a)) for a, b in zip(offs[:-1], offs[1:])]
+    res = {}
+
+    def per_call():
+        for d_in, d_out, n in bufs:
+            ctx.fdgrad(n, d_in, d_out, stream)
+    log(f"[c2] mixed 64 trajectories ({n2} knots): per-call kernel, 64 launches per batch")
+    ms = time_launches(per_call, steps // 5, 5, stream) / (steps // 5)
+    res["mixed_64_traj_per_call"] = {"knots_per_batch": n2, "launches_per_batch": 64, "ms_per_batch": ms,
+                                     "knots_per_s": n2 / (ms * 1e-3)}
+    log(f"[c2] {res['mixed_64_traj_per_call']}")
+    batch = rbdgrad.Batch(ctx, 64)
+    for use_graph in (False, True):
+        log(f"[c2] batched persistent kernel, graph={use_graph}: {steps} timed batches")
+        ms = time_launches(lambda: batch.run(bufs, use_graph=use_graph, stream=stream), steps, 20, stream) / steps
+        gbs, tfl = roofline(n2, ms)
+        out = np.concatenate([d_out.download((n, rbdgrad.OUT_PER_KNOT)) for _, d_out, n in bufs])
+        res["mixed_64_traj_batched" + ("_graph" if use_graph else "")] = {
+            "knots_per_batch": n2, "ms_per_batch": ms, "knots_per_s": n2 / (ms * 1e-3),
+            "lanes_per_knot": batch.last_lanes, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+            "fp32_tflops": tfl, "fp32_frac": tfl / FP32_PEAK_TFLOPS, "accuracy": accuracy(out, x2, sample=4096)}
+        log(f"[c2] {res['mixed_64_traj_batched' + ('_graph' if use_graph else '')]}")
+    batch.close()
+    for d_in, d_out, _ in bufs:
+        d_in.free()
+        d_out.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,20 +247,7 @@ def main():
                 log(f"[c4] {workloads[f'large_batch_2p20_{prec}_{layout}']}")
             if c4 is not ctx:
                 c4.close()
-        # ---- 64 trajectories of uneven length U{8..256}, one launch over all knots ----
-        lens = mixed_trajectories()
-        n2 = int(lens.sum())
-        x2 = knots(n2, seed=102)
-        steps2 = 500
-        log(f"[c2] mixed 64 trajectories ({n2} knots): one launch per batch, {steps2} timed")
-        ms2, out2 = run_flat(ctx, x2, steps2, 20, stream)
-        per2 = ms2 / steps2
-        gbs, tfl = roofline(n2, per2)
-        workloads["mixed_64_traj_flat"] = {
-            "knots_per_launch": n2, "ms_per_launch": per2, "knots_per_s": n2 / (per2 * 1e-3),
-            "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_tflops": tfl, "fp32_frac": tfl / FP32_PEAK_TFLOPS,
-            "accuracy": accuracy(out2, x2, sample=4096)}
-        log(f"[c2] {workloads['mixed_64_traj_flat']}")
+        workloads.update(mixed_workload(ctx, stream))
 
     cpu = None if args.no_cpu else cpu_baseline(x1, args.cpu_seconds)
 

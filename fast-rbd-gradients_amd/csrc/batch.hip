@@ -1,0 +1,205 @@
+// Batched uneven trajectories (rbd_batch_*): one persistent-kernel launch per
+// batch over a device work queue of knot chunks, optionally replayed from a
+// hipGraph.  Kernels in batch_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "batch_kernels.hpp"
+#include "context.hpp"
+
+using namespace rbdi;
+
+struct rbd_batch {
+  rbd_context* ctx = nullptr;
+  int max_traj = 0;
+  // device: [BatchHeader | TrajDesc[max_traj] | long prefix[max_traj + 1]]
+  char* d_mem = nullptr;
+  rbd::BatchHeader* d_hdr = nullptr;
+  rbd::TrajDesc* d_desc = nullptr;
+  long* d_prefix = nullptr;
+  size_t upload_bytes = 0;            // header + descriptors
+  // pinned host staging ring for the descriptor upload
+  static constexpr int kRing = 4;
+  char* h_stage[kRing] = {};
+  hipEvent_t staged[kRing] = {};
+  int ring = 0;
+  // persistent grid (waves) per mapping, and the captured graphs
+  unsigned grid[2] = {0, 0};          // [0]: one lane per knot, [1]: lane groups
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  int64_t last_total_knots = 0;
+  int last_lanes = 0;
+};
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+template <class P>
+hipError_t launch_batch_p(rbd_batch* b, int lanes, hipStream_t s) {
+  hipLaunchKernelGGL(rbd::batch_plan_kernel, dim3(1), dim3(rbd::kPlanThreads), 0, s, b->d_desc, b->d_hdr,
+                     b->d_prefix);
+  const unsigned grid = b->grid[lanes == 1 ? 0 : 1];
+  if (lanes == 1)
+    hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, 1>), dim3(grid), dim3(kBlock), 0, s, b->d_desc, b->d_hdr,
+                       b->d_prefix, b->ctx->opt);
+  else
+    hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, rbd::kLanesPerKnotGroup>), dim3(grid), dim3(kBlock), 0, s,
+                       b->d_desc, b->d_hdr, b->d_prefix, b->ctx->opt);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(rbd_batch* b, int lanes, hipStream_t s) {
+  switch (b->ctx->precision) {
+    case RBD_PREC_FP32: return launch_batch_p<rbd::PrecFP32>(b, lanes, s);
+    case RBD_PREC_FP64_PIVOT: return launch_batch_p<rbd::PrecFP64Pivot>(b, lanes, s);
+    case RBD_PREC_FP64_COLUMNS: return launch_batch_p<rbd::PrecFP64Col>(b, lanes, s);
+    default: return launch_batch_p<rbd::PrecFP64>(b, lanes, s);
+  }
+}
+
+// capture plan + persistent kernels for one mapping into a graph (once)
+rbd_status ensure_graph(rbd_batch* b, int lanes) {
+  const int gi = lanes == 1 ? 0 : 1;
+  if (b->exec[gi]) return RBD_OK;
+  hipStream_t cs = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamCreate (capture)");
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    hipError_t le = launch_batch(b, lanes, cs);
+    e = hipStreamEndCapture(cs, &b->graph[gi]);
+    if (le != hipSuccess) e = le;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[gi], b->graph[gi], nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess) {
+    if (b->graph[gi]) (void)hipGraphDestroy(b->graph[gi]);
+    b->graph[gi] = nullptr;
+    b->exec[gi] = nullptr;
+    return hip_fail(e, "batch graph capture");
+  }
+  return RBD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** out) {
+  if (!out) return fail(RBD_ERR_INVALID_ARGUMENT, "null output pointer");
+  *out = nullptr;
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (max_trajectories <= 0 || max_trajectories > (1 << 24))
+    return fail(RBD_ERR_INVALID_ARGUMENT, "max_trajectories must be in [1, 2^24]");
+  if (rbd_status s = use_device(ctx)) return s;
+  rbd_batch* b = new (std::nothrow) rbd_batch();
+  if (!b) return fail(RBD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  b->ctx = ctx;
+  b->max_traj = max_trajectories;
+  const size_t hdr = align_up(sizeof(rbd::BatchHeader), 256);
+  const size_t desc = align_up(sizeof(rbd::TrajDesc) * size_t(max_trajectories), 256);
+  const size_t pre = sizeof(long) * (size_t(max_trajectories) + 1);
+  b->upload_bytes = hdr + sizeof(rbd::TrajDesc) * size_t(max_trajectories);
+  hipError_t e = hipMalloc(&b->d_mem, hdr + desc + pre);
+  if (e != hipSuccess) { delete b; return fail(RBD_ERR_OUT_OF_MEMORY, "hipMalloc (batch)"); }
+  b->d_hdr = reinterpret_cast<rbd::BatchHeader*>(b->d_mem);
+  b->d_desc = reinterpret_cast<rbd::TrajDesc*>(b->d_mem + hdr);
+  b->d_prefix = reinterpret_cast<long*>(b->d_mem + hdr + desc);
+  for (int i = 0; i < rbd_batch::kRing && e == hipSuccess; ++i) {
+    e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stage[i]), b->upload_bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    rbd_batch_destroy(b);
+    return hip_fail(e, "batch staging");
+  }
+  // persistent grid: one resident wave per SIMD (the kernels run at one wave
+  // per SIMD: 256+ registers per lane)
+  b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4);
+  *out = b;
+  return RBD_OK;
+}
+
+rbd_status rbd_batch_destroy(rbd_batch* b) {
+  if (!b) return RBD_OK;
+  (void)use_device(b->ctx);
+  for (int gi = 0; gi < 2; ++gi) {
+    if (b->exec[gi]) (void)hipGraphExecDestroy(b->exec[gi]);
+    if (b->graph[gi]) (void)hipGraphDestroy(b->graph[gi]);
+  }
+  for (int i = 0; i < rbd_batch::kRing; ++i) {
+    if (b->staged[i]) { (void)hipEventSynchronize(b->staged[i]); (void)hipEventDestroy(b->staged[i]); }
+    if (b->h_stage[i]) (void)hipHostFree(b->h_stage[i]);
+  }
+  if (b->d_mem) (void)hipFree(b->d_mem);
+  delete b;
+  return RBD_OK;
+}
+
+rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves) {
+  if (!b) return fail(RBD_ERR_INVALID_ARGUMENT, "null batch");
+  if (lanes_per_knot != 1 && lanes_per_knot != rbd::kLanesPerKnotGroup)
+    return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 1 or 16");
+  if (waves == 0 || waves > (1u << 20)) return fail(RBD_ERR_INVALID_ARGUMENT, "waves must be in [1, 2^20]");
+  const int gi = lanes_per_knot == 1 ? 0 : 1;
+  if (b->grid[gi] != waves) {
+    b->grid[gi] = waves;
+    if (b->exec[gi]) { (void)hipGraphExecDestroy(b->exec[gi]); b->exec[gi] = nullptr; }
+    if (b->graph[gi]) { (void)hipGraphDestroy(b->graph[gi]); b->graph[gi] = nullptr; }
+  }
+  return RBD_OK;
+}
+
+rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph,
+                                  rbd_stream_t stream) {
+  if (!b) return fail(RBD_ERR_INVALID_ARGUMENT, "null batch");
+  if (n_traj < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative trajectory count");
+  if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
+  if (n_traj > 0 && !trajs) return fail(RBD_ERR_INVALID_ARGUMENT, "null trajectory list");
+  int64_t total = 0;
+  for (int t = 0; t < n_traj; ++t) {
+    if (trajs[t].n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative trajectory length");
+    if (trajs[t].n_knots > 0 && (!trajs[t].d_in || !trajs[t].d_out))
+      return fail(RBD_ERR_INVALID_ARGUMENT, "null trajectory buffer");
+    total += trajs[t].n_knots;
+  }
+  b->last_total_knots = total;
+  if (total == 0) return RBD_OK;
+  if (rbd_status s = use_device(b->ctx)) return s;
+  const hipStream_t st = (hipStream_t)stream;
+  const int lanes = lanes_for(b->ctx, total);
+  b->last_lanes = lanes;
+  // stage header + descriptors in pinned memory, copy to the device on `stream`
+  const int r = b->ring;
+  b->ring = (r + 1) % rbd_batch::kRing;
+  hipError_t e = hipEventSynchronize(b->staged[r]);     // previous copy out of this slot is done
+  if (e != hipSuccess) return hip_fail(e, "staging wait");
+  const size_t hdr = align_up(sizeof(rbd::BatchHeader), 256);
+  rbd::BatchHeader* h = reinterpret_cast<rbd::BatchHeader*>(b->h_stage[r]);
+  std::memset(h, 0, sizeof(*h));
+  h->n_traj = n_traj;
+  h->chunk_knots = 64 / lanes;
+  rbd::TrajDesc* d = reinterpret_cast<rbd::TrajDesc*>(b->h_stage[r] + hdr);
+  for (int t = 0; t < n_traj; ++t) d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
+  e = hipMemcpyAsync(b->d_mem, b->h_stage[r], hdr + sizeof(rbd::TrajDesc) * size_t(n_traj), hipMemcpyHostToDevice,
+                     st);
+  if (e == hipSuccess) e = hipEventRecord(b->staged[r], st);
+  if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+  if (use_graph) {
+    if (rbd_status s = ensure_graph(b, lanes)) return s;
+    e = hipGraphLaunch(b->exec[lanes == 1 ? 0 : 1], st);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  } else {
+    e = launch_batch(b, lanes, st);
+    if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+  }
+  return RBD_OK;
+}
+
+int rbd_batch_last_lanes(const rbd_batch* b) { return b ? b->last_lanes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// Host-side internals shared by the library's translation units: the context,
+// error reporting and the flat-launch helper.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "../../include/rbdgrad.h"
+#include "fdgrad_knot.hpp"
+
+namespace rbdi {
+
+constexpr int kBlock = 64;   // one wave per workgroup
+
+// Below this many knots a call uses the 16-lane-group mapping (when the
+// context's lanes_per_knot is 0): the crossover measured on MI355X by
+// rbd_autotune (one-lane 22.6 us vs lane-group 16.1 us at 8192 knots,
+// 23.1 vs 29.2 us at 16384).
+constexpr int64_t kDefaultGroupThreshold = 8192;
+
+// Records msg as the calling thread's last error and returns s.
+rbd_status fail(rbd_status s, const std::string& msg);
+rbd_status hip_fail(hipError_t e, const char* what);
+
+enum class Layout { AoS, SoA, Ref };
+
+struct FlatCall {
+  Layout layout;
+  int lanes;              // 1 or rbd::kLanesPerKnotGroup
+  int64_t n;
+  const float* in;
+  int64_t ld_in;          // SoA only
+  float* out;
+  int64_t ld_out;         // SoA only
+  rbd::Options opt;
+  hipStream_t stream;
+};
+
+hipError_t launch_flat(int precision, const FlatCall& c);
+
+}  // namespace rbdi
+
+struct rbd_context {
+  int device = 0;
+  unsigned flags = 0;
+  int precision = RBD_PREC_FP32;
+  int lanes_per_knot = 0;                        // 0 = by batch size, 1, or 16
+  std::atomic<int64_t> group_threshold{rbdi::kDefaultGroupThreshold};
+  int compute_units = 0;                         // of the device
+  rbd::Options opt;
+  // device buffer of the host-pointer convenience call
+  std::mutex host_mu;
+  float* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace rbdi {
+
+inline int lanes_for(const rbd_context* ctx, int64_t n) {
+  if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
+  return n <= ctx->group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
+}
+
+inline rbd_status check_ctx(const rbd_context* ctx) {
+  if (!ctx) return fail(RBD_ERR_INVALID_ARGUMENT, "null context");
+  return RBD_OK;
+}
+
+// the context's device must be current for a launch on the caller's thread
+inline rbd_status use_device(const rbd_context* ctx) {
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (cur != ctx->device) {
+    e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  }
+  return RBD_OK;
+}
+
+}  // namespace rbdi

@@ -14,19 +14,18 @@
 // rbd_autotune or the built-in default).
 #include <hip/hip_runtime.h>
 
-#include <atomic>
 #include <cstdio>
 #include <cstring>
-#include <mutex>
 #include <new>
 #include <string>
 
-#include "../../include/rbdgrad.h"
-#include "fdgrad_knot.hpp"
+#include "context.hpp"
+
+namespace rbdi {
 
 namespace {
-
 thread_local std::string g_last_error;
+}
 
 rbd_status fail(rbd_status s, const std::string& msg) {
   g_last_error = msg;
@@ -36,27 +35,9 @@ rbd_status hip_fail(hipError_t e, const char* what) {
   return fail(RBD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-constexpr int kBlock = 64;   // one wave per workgroup
+}  // namespace rbdi
 
-}  // namespace
-
-// Below this many knots a call uses the 16-lane-group mapping (when the
-// context's lanes_per_knot is 0); measured crossover on MI355X, refined by
-// rbd_autotune.
-constexpr int64_t kDefaultGroupThreshold = 8192;
-
-struct rbd_context {
-  int device = 0;
-  unsigned flags = 0;
-  int precision = RBD_PREC_FP32;
-  int lanes_per_knot = 0;                        // 0 = by batch size, 1, or 16
-  std::atomic<int64_t> group_threshold{kDefaultGroupThreshold};
-  rbd::Options opt;
-  // device buffer of the host-pointer convenience call
-  std::mutex host_mu;
-  float* d_scratch = nullptr;
-  size_t scratch_bytes = 0;
-};
+using rbdi::kBlock;
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -137,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void fdgrad_refmode_lanes_kernel(long n, co
 
 }  // namespace rbd
 
-namespace {
+namespace rbdi {
 
 // grid for n knots at `lanes` lanes per knot, capped so huge batches grid-stride
 unsigned grid_for(int64_t n, int lanes) {
@@ -147,22 +128,8 @@ unsigned grid_for(int64_t n, int lanes) {
   return unsigned(blocks < cap ? blocks : cap);
 }
 
-enum class Layout { AoS, SoA, Ref };
-
-struct FlatCall {
-  Layout layout;
-  int lanes;              // 1 or rbd::kLanesPerKnotGroup
-  int64_t n;
-  const float* in;
-  int64_t ld_in;          // SoA only
-  float* out;
-  int64_t ld_out;         // SoA only
-  rbd::Options opt;
-  hipStream_t stream;
-};
-
 template <class P>
-hipError_t launch_flat(const FlatCall& c) {
+hipError_t launch_flat_p(const FlatCall& c) {
   const dim3 grid(grid_for(c.n, c.lanes)), block(kBlock);
   const long n = long(c.n);
   if (c.lanes == 1) {
@@ -197,36 +164,17 @@ hipError_t launch_flat(const FlatCall& c) {
 
 hipError_t launch_flat(int precision, const FlatCall& c) {
   switch (precision) {
-    case RBD_PREC_FP32: return launch_flat<rbd::PrecFP32>(c);
-    case RBD_PREC_FP64_PIVOT: return launch_flat<rbd::PrecFP64Pivot>(c);
-    case RBD_PREC_FP64_COLUMNS: return launch_flat<rbd::PrecFP64Col>(c);
-    default: return launch_flat<rbd::PrecFP64>(c);
+    case RBD_PREC_FP32: return launch_flat_p<rbd::PrecFP32>(c);
+    case RBD_PREC_FP64_PIVOT: return launch_flat_p<rbd::PrecFP64Pivot>(c);
+    case RBD_PREC_FP64_COLUMNS: return launch_flat_p<rbd::PrecFP64Col>(c);
+    default: return launch_flat_p<rbd::PrecFP64>(c);
   }
 }
 
-int lanes_for(const rbd_context* ctx, int64_t n) {
-  if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
-  return n <= ctx->group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
-}
+}  // namespace rbdi
 
-rbd_status check_ctx(rbd_context* ctx) {
-  if (!ctx) return fail(RBD_ERR_INVALID_ARGUMENT, "null context");
-  return RBD_OK;
-}
+using namespace rbdi;
 
-// the context's device must be current for a launch on the caller's thread
-rbd_status use_device(rbd_context* ctx) {
-  int cur = -1;
-  hipError_t e = hipGetDevice(&cur);
-  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-  if (cur != ctx->device) {
-    e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  }
-  return RBD_OK;
-}
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -256,7 +204,7 @@ const char* rbd_status_string(rbd_status s) {
   return "unknown status";
 }
 
-const char* rbd_last_error(void) { return g_last_error.c_str(); }
+const char* rbd_last_error(void) { return rbdi::g_last_error.c_str(); }
 
 rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   if (!out) return fail(RBD_ERR_INVALID_ARGUMENT, "null output pointer");
@@ -277,6 +225,9 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   if (c.device < 0 || c.device >= ndev) return fail(RBD_ERR_INVALID_ARGUMENT, "device ordinal out of range");
   rbd_context* ctx = new (std::nothrow) rbd_context();
   if (!ctx) return fail(RBD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  if (hipDeviceGetAttribute(&ctx->compute_units, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess ||
+      ctx->compute_units <= 0)
+    ctx->compute_units = 256;
   ctx->device = c.device;
   ctx->flags = c.flags;
   ctx->precision = c.precision;

@@ -35,6 +35,8 @@ EXPORTED = [
     "rbd_abi_version", "rbd_fdgrad_traj_f32", "rbd_fdgrad_traj_soa_f32",
     "rbd_dynamics_gradient_kernel_f32", "rbd_fdgrad_traj_host_f32", "rbd_kernel_name",
     "rbd_autotune", "rbd_group_threshold", "rbd_set_group_threshold",
+    "rbd_batch_create", "rbd_batch_destroy", "rbd_fdgrad_batched_f32", "rbd_batch_set_grid",
+    "rbd_batch_last_lanes",
 ]
 
 
@@ -44,6 +46,10 @@ class RbdError(RuntimeError):
 
 class HipError(RuntimeError):
     pass
+
+
+class Traj(ctypes.Structure):
+    _fields_ = [("d_in", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("n_knots", ctypes.c_int64)]
 
 
 class Config(ctypes.Structure):
@@ -81,6 +87,11 @@ def lib():
         L.rbd_group_threshold.argtypes = [vp]
         L.rbd_group_threshold.restype = i64
         L.rbd_set_group_threshold.argtypes = [vp, i64]
+        L.rbd_batch_create.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.rbd_batch_destroy.argtypes = [vp]
+        L.rbd_fdgrad_batched_f32.argtypes = [vp, ctypes.POINTER(Traj), ctypes.c_int, ctypes.c_int, vp]
+        L.rbd_batch_set_grid.argtypes = [vp, ctypes.c_int, ctypes.c_uint]
+        L.rbd_batch_last_lanes.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -327,3 +338,40 @@ class Context:
         d_out = DeviceBuffer(n * DQDD_PER_KNOT * 4)
         self.dynamics_gradient_kernel(n, d_out, d_vol)
         return d_out.download((n, DQDD_PER_KNOT))
+
+
+class Batch:
+    """rbd_batch: runs many trajectories of uneven length with one persistent-kernel launch."""
+
+    def __init__(self, ctx, max_trajectories):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        _check(lib().rbd_batch_create(ctx.handle, int(max_trajectories), ctypes.byref(self._h)),
+               "rbd_batch_create")
+
+    def run(self, trajs, use_graph=False, stream=None):
+        """trajs: list of (d_in DeviceBuffer, d_out DeviceBuffer, n_knots)."""
+        arr = (Traj * max(1, len(trajs)))()
+        for i, (di, do, n) in enumerate(trajs):
+            arr[i] = Traj(di.handle.value if di is not None else None, do.handle.value if do is not None else None,
+                          int(n))
+        _check(lib().rbd_fdgrad_batched_f32(self._h, arr, len(trajs), int(bool(use_graph)), _sh(stream)),
+               "rbd_fdgrad_batched_f32")
+
+    def set_grid(self, lanes_per_knot, waves):
+        _check(lib().rbd_batch_set_grid(self._h, int(lanes_per_knot), int(waves)), "rbd_batch_set_grid")
+
+    @property
+    def last_lanes(self):
+        return lib().rbd_batch_last_lanes(self._h)
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().rbd_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

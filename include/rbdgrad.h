@@ -142,6 +142,45 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
 int64_t rbd_group_threshold(rbd_context* ctx);
 rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots);
 
+/* ------------------------------------------------------------------------
+ * Batched trajectories of uneven length.
+ * A batch object owns a device work queue (trajectory descriptors, chunk
+ * prefix sums, a cursor) and pinned staging for the descriptors.  One call runs
+ * a whole batch with one persistent-kernel launch: waves pull knot chunks from
+ * the queue until it is empty.  Trajectory buffers stay caller-owned device
+ * memory (AoS, 21 in / 105 out per knot).
+ * Replaces: the reference's per-trajectory launch over one traj<T,N> at a time
+ * (grid = N knots, time_GPU.cu:314; CPU split of a trajectory into per-thread
+ * chunks, time_CPU.cpp:30-36), generalised to many trajectories per launch.
+ * ------------------------------------------------------------------------ */
+typedef struct rbd_batch rbd_batch;
+
+typedef struct rbd_traj {
+  const float* d_in;  /* n_knots x 21 */
+  float* d_out;       /* n_knots x 105 */
+  int64_t n_knots;
+} rbd_traj;
+
+rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** out);
+rbd_status rbd_batch_destroy(rbd_batch* b);
+
+/* Run the forward-dynamics gradient over n_traj trajectories on `stream`.
+ * The descriptors are copied to the device asynchronously on `stream`; the
+ * caller's trajs array may be reused on return.  The knot-to-lane mapping is
+ * chosen from the total knot count as for rbd_fdgrad_traj_f32.  use_graph != 0
+ * replays the plan and persistent kernels from a hipGraph captured on first
+ * use (one graph per mapping).  Returns RBD_ERR_CAPACITY if n_traj exceeds the
+ * batch's max_trajectories. */
+rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph,
+                                  rbd_stream_t stream);
+
+/* Persistent grid size (waves) per mapping (default: 4 x compute units, one
+ * resident wave per SIMD); set by the batch autotuner in bench/tests. */
+rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves);
+
+/* Lanes per knot used by the last rbd_fdgrad_batched_f32 call (1 or 16). */
+int rbd_batch_last_lanes(const rbd_batch* b);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

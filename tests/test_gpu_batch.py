@@ -1,0 +1,117 @@
+"""GPU tests of the batched uneven-trajectory path (rbd_batch_*): a persistent
+kernel draining a device work queue, with and without hipGraph replay."""
+import numpy as np
+import pytest
+
+import common
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_batch(rbd, lengths, seed):
+    xs, trajs = [], []
+    for i, n in enumerate(lengths):
+        x = common.knots(int(n), seed=seed * 1000 + i) if n > 0 else np.zeros((0, 21), np.float32)
+        d_in = rbd.DeviceBuffer.from_array(x) if n > 0 else rbd.DeviceBuffer(64)
+        d_out = rbd.DeviceBuffer(max(1, int(n)) * 105 * 4)
+        xs.append(x)
+        trajs.append((d_in, d_out, int(n)))
+    return xs, trajs
+
+
+def download(trajs):
+    return [d_out.download((n, 105)) for _, d_out, n in trajs]
+
+
+@pytest.fixture(scope="module")
+def ctx(rbd):
+    c = rbd.Context(precision="fp32")
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_uneven_batch_matches_per_trajectory_calls(rbd, ctx, oracle, use_graph):
+    lengths = np.random.default_rng(11).integers(8, 257, 64)        # the mixed-length workload
+    xs, trajs = make_batch(rbd, lengths, seed=1)
+    say(f"batch 64 trajectories, {lengths.sum()} knots, graph={use_graph}")
+    b = rbd.Batch(ctx, 256)
+    b.run(trajs, use_graph=use_graph)
+    rbd.hip().hipDeviceSynchronize()
+    assert b.last_lanes in (1, 16)
+    got = download(trajs)
+    flat_ctx = rbd.Context(precision="fp32", lanes_per_knot=b.last_lanes)
+    for x, g in zip(xs, got):
+        np.testing.assert_array_equal(g, flat_ctx.run_aos(x))
+    allx = np.concatenate(xs)
+    err = common.max_block_errors(np.concatenate(got), oracle.fdgrad(allx.astype(np.float64), threads=8))
+    assert max(err.values()) < common.REL_TOL, err
+    flat_ctx.close()
+    b.close()
+
+
+def test_graph_replays_with_changing_batches(rbd, ctx):
+    """Replay the captured graph several times with different trajectory sets,
+    with eager work and synchronisation between replays; check every replay."""
+    b = rbd.Batch(ctx, 128)
+    flat = rbd.Context(precision="fp32", lanes_per_knot=16)
+    rng = np.random.default_rng(12)
+    for rep in range(5):
+        lengths = rng.integers(1, 200, rng.integers(1, 100))
+        xs, trajs = make_batch(rbd, lengths, seed=100 + rep)
+        b.run(trajs, use_graph=True)
+        rbd.hip().hipDeviceSynchronize()
+        flat.run_aos(common.knots(33, seed=rep))                      # eager work in between
+        for x, g in zip(xs, download(trajs)):
+            np.testing.assert_array_equal(g, flat.run_aos(x))
+        say(f"replay {rep}: {len(lengths)} trajectories ok")
+    flat.close()
+    b.close()
+
+
+def test_large_batch_uses_one_lane_mapping(rbd, ctx, oracle):
+    lengths = [4000] * 8 + [777, 1, 0, 5000]
+    xs, trajs = make_batch(rbd, lengths, seed=3)
+    b = rbd.Batch(ctx, 16)
+    b.run(trajs, use_graph=True)
+    rbd.hip().hipDeviceSynchronize()
+    assert b.last_lanes == 1
+    got = download(trajs)
+    one = rbd.Context(precision="fp32", lanes_per_knot=1)
+    for x, g, n in zip(xs, got, lengths):
+        if n:
+            np.testing.assert_array_equal(g, one.run_aos(x))
+    one.close()
+    b.close()
+
+
+def test_small_grid_still_drains_the_queue(rbd, ctx):
+    """A persistent grid far smaller than the chunk count: every wave loops
+    until the queue is empty, no knot is skipped."""
+    lengths = [300, 5, 129, 64, 63]
+    xs, trajs = make_batch(rbd, lengths, seed=4)
+    b = rbd.Batch(ctx, 8)
+    b.set_grid(16, 3)
+    b.set_grid(1, 2)
+    b.run(trajs)
+    rbd.hip().hipDeviceSynchronize()
+    flat = rbd.Context(precision="fp32", lanes_per_knot=b.last_lanes)
+    for x, g in zip(xs, download(trajs)):
+        np.testing.assert_array_equal(g, flat.run_aos(x))
+    flat.close()
+    b.close()
+
+
+def test_batch_errors(rbd, ctx):
+    b = rbd.Batch(ctx, 2)
+    xs, trajs = make_batch(rbd, [5, 5, 5], seed=5)
+    with pytest.raises(rbd.RbdError, match="capacity"):
+        b.run(trajs)
+    b.run([])                                     # empty batch is a no-op
+    with pytest.raises(rbd.RbdError):
+        b.set_grid(4, 10)
+    b.close()
