@@ -181,6 +181,23 @@ def mixed_workload(ctx, stream, steps=500):
     return res
 
 
+def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000), duration_s=1.0):
+    """Config 3: Poisson request arrivals at each rate; each request one
+    trajectory with length uniform over {16, 32, 64, 128, 256}; device-resident
+    constants, admission by a fixed memory budget, continuous batching over
+    4 in-flight slots.  Latency = submit -> result in the caller's host buffer."""
+    pool = knots(1 << 16, seed=103)
+    res = {}
+    srv = rbdgrad.Server(ctx, slots=4, lanes_per_knot=16, memory_budget_bytes=256 << 20)
+    for rate in rates:
+        log(f"[c3] served: Poisson {rate} req/s for {duration_s} s")
+        st = srv.run_poisson(rate, duration_s, [16, 32, 64, 128, 256], pool, seed=rate)
+        res[f"served_{rate}_req_per_s"] = st
+        log(f"[c3] {st}")
+    srv.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +265,7 @@ def main():
             if c4 is not ctx:
                 c4.close()
         workloads.update(mixed_workload(ctx, stream))
+        workloads.update(served_workload(ctx))
 
     cpu = None if args.no_cpu else cpu_baseline(x1, args.cpu_seconds)
 

@@ -37,6 +37,9 @@ EXPORTED = [
     "rbd_autotune", "rbd_group_threshold", "rbd_set_group_threshold",
     "rbd_batch_create", "rbd_batch_destroy", "rbd_fdgrad_batched_f32", "rbd_batch_set_grid",
     "rbd_batch_last_lanes",
+    "rbd_server_config_default", "rbd_server_create", "rbd_server_destroy", "rbd_server_submit",
+    "rbd_server_poll", "rbd_server_wait", "rbd_server_get_stats", "rbd_server_reset_stats",
+    "rbd_server_run_poisson",
 ]
 
 
@@ -50,6 +53,24 @@ class HipError(RuntimeError):
 
 class Traj(ctypes.Structure):
     _fields_ = [("d_in", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("n_knots", ctypes.c_int64)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("lanes_per_knot", ctypes.c_int), ("memory_budget_bytes", ctypes.c_int64),
+                ("max_batch_trajectories", ctypes.c_int64), ("max_batch_knots", ctypes.c_int64),
+                ("batch_window_us", ctypes.c_double), ("use_graph", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests_completed", ctypes.c_int64), ("knots_completed", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("max_inflight_batches", ctypes.c_int64),
+                ("max_batch_knots", ctypes.c_int64), ("max_budget_bytes_used", ctypes.c_int64),
+                ("queued", ctypes.c_int64), ("p50_latency_us", ctypes.c_double), ("p99_latency_us", ctypes.c_double),
+                ("mean_latency_us", ctypes.c_double), ("max_latency_us", ctypes.c_double),
+                ("knots_per_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class Config(ctypes.Structure):
@@ -92,6 +113,16 @@ def lib():
         L.rbd_fdgrad_batched_f32.argtypes = [vp, ctypes.POINTER(Traj), ctypes.c_int, ctypes.c_int, vp]
         L.rbd_batch_set_grid.argtypes = [vp, ctypes.c_int, ctypes.c_uint]
         L.rbd_batch_last_lanes.argtypes = [vp]
+        L.rbd_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
+        L.rbd_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+        L.rbd_server_destroy.argtypes = [vp]
+        L.rbd_server_submit.argtypes = [vp, vp, vp, i64, ctypes.POINTER(ctypes.c_uint64)]
+        L.rbd_server_poll.argtypes = [vp, ctypes.c_uint64]
+        L.rbd_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_double]
+        L.rbd_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.rbd_server_reset_stats.argtypes = [vp]
+        L.rbd_server_run_poisson.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(i64), ctypes.c_int,
+                                             vp, i64, ctypes.c_uint64, ctypes.POINTER(ServerStats)]
         _lib = L
     return _lib
 
@@ -368,6 +399,67 @@ class Batch:
     def close(self):
         if self._h is not None and self._h.value:
             lib().rbd_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """rbd_server: continuous batching of gradient requests (one trajectory each)."""
+
+    def __init__(self, ctx, **cfg):
+        c = ServerConfig()
+        lib().rbd_server_config_default(ctypes.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.ctx = ctx
+        self.config = c
+        self._h = ctypes.c_void_p()
+        self._keep = {}                      # request id -> (input, output) arrays, alive until collected
+        _check(lib().rbd_server_create(ctx.handle, ctypes.byref(c), ctypes.byref(self._h)), "rbd_server_create")
+
+    def submit(self, x):
+        """Queue one trajectory x (n, 21); returns the request id."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+        out = np.empty((x.shape[0], OUT_PER_KNOT), dtype=np.float32)
+        rid = ctypes.c_uint64()
+        _check(lib().rbd_server_submit(self._h, x.ctypes.data, out.ctypes.data, x.shape[0], ctypes.byref(rid)),
+               "rbd_server_submit")
+        self._keep[rid.value] = (x, out)
+        return rid.value
+
+    def poll(self, rid):
+        return lib().rbd_server_poll(self._h, rid)
+
+    def wait(self, rid, timeout_s=60.0):
+        """Block until done; returns the (n, 105) result."""
+        _check(lib().rbd_server_wait(self._h, rid, timeout_s), "rbd_server_wait")
+        return self._keep.pop(rid)[1]
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().rbd_server_get_stats(self._h, ctypes.byref(st)), "rbd_server_get_stats")
+        return st.as_dict()
+
+    def reset_stats(self):
+        _check(lib().rbd_server_reset_stats(self._h), "rbd_server_reset_stats")
+
+    def run_poisson(self, rate, duration_s, lengths, pool, seed=0):
+        lens = (ctypes.c_int64 * len(lengths))(*[int(v) for v in lengths])
+        pool = np.ascontiguousarray(pool, dtype=np.float32)
+        st = ServerStats()
+        _check(lib().rbd_server_run_poisson(self._h, float(rate), float(duration_s), lens, len(lengths),
+                                            pool.ctypes.data, pool.shape[0], int(seed), ctypes.byref(st)),
+               "rbd_server_run_poisson")
+        return st.as_dict()
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().rbd_server_destroy(self._h)
         self._h = None
 
     def __del__(self):

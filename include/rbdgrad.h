@@ -181,6 +181,71 @@ rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves);
 /* Lanes per knot used by the last rbd_fdgrad_batched_f32 call (1 or 16). */
 int rbd_batch_last_lanes(const rbd_batch* b);
 
+/* ------------------------------------------------------------------------
+ * Serving: continuous batching of gradient requests.
+ * A request is one trajectory (host input n x 21, host output n x 105; both
+ * caller-owned and valid until the request completes).  A dispatcher thread
+ * admits requests in arrival order while the memory budget allows (n x 504
+ * bytes, rounded up to 256, of device memory plus the same of pinned staging
+ * per request), packs them into batches and runs each batch on a free slot
+ * (stream + batch work queue), so batches overlap on the device; a completion
+ * thread delivers results and records latency (submit -> result in h_out).
+ * All batches of one server use the same knot-to-lane mapping, so a result does
+ * not depend on what it was batched with.  No reference counterpart (the
+ * reference has no serving layer); the latency statistics follow its
+ * printStats percentiles (utils/experiment_helpers.h:41-100).
+ * ------------------------------------------------------------------------ */
+typedef struct rbd_server rbd_server;
+
+typedef struct rbd_server_config {
+  int slots;                      /* batches in flight (one stream each), 1..64 */
+  int lanes_per_knot;             /* 1 or 16, used for every batch */
+  int64_t memory_budget_bytes;    /* device memory for admitted requests (and as much pinned staging) */
+  int64_t max_batch_trajectories; /* per batch */
+  int64_t max_batch_knots;        /* per batch; also the longest admissible request */
+  double batch_window_us;         /* after the oldest queued request arrived, wait this long for more (0: none) */
+  int use_graph;                  /* replay each slot's batch kernels from a hipGraph */
+  int reserved[3];
+} rbd_server_config;
+
+typedef struct rbd_server_stats {
+  int64_t requests_completed;
+  int64_t knots_completed;
+  int64_t batches;
+  int64_t max_inflight_batches;
+  int64_t max_batch_knots;
+  int64_t max_budget_bytes_used;
+  int64_t queued;                 /* submitted, not yet admitted */
+  double p50_latency_us;
+  double p99_latency_us;
+  double mean_latency_us;
+  double max_latency_us;
+  double knots_per_s;             /* completed knots / (last completion - first submit) */
+} rbd_server_stats;
+
+/* slots 4, lanes_per_knot 16, 64 MiB budget, 1024 trajectories and 2^17 knots
+ * per batch, no batching window, no graphs. */
+void rbd_server_config_default(rbd_server_config* cfg);
+rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd_server** out);
+/* Finishes the batches on the device, fails the queued requests, joins the threads. */
+rbd_status rbd_server_destroy(rbd_server* s);
+/* Queue a request; RBD_ERR_CAPACITY if it can never fit the budget or max_batch_knots. */
+rbd_status rbd_server_submit(rbd_server* s, const float* h_in, float* h_out, int64_t n_knots, uint64_t* id);
+/* 1: done (result in h_out), 0: pending, -1: failed, -2: unknown id.  A finished
+ * request is forgotten once polled or waited for. */
+int rbd_server_poll(rbd_server* s, uint64_t id);
+/* Block until the request finishes (timeout_s < 0: no limit). */
+rbd_status rbd_server_wait(rbd_server* s, uint64_t id, double timeout_s);
+rbd_status rbd_server_get_stats(rbd_server* s, rbd_server_stats* out);
+rbd_status rbd_server_reset_stats(rbd_server* s);
+/* Load generator for the served benchmark: Poisson arrivals at rate_per_s for
+ * duration_s, each request one trajectory with length drawn uniformly from
+ * lengths[0..n_lengths), inputs taken from pool_in (pool_knots x 21); waits for
+ * all requests and returns the statistics of the run. */
+rbd_status rbd_server_run_poisson(rbd_server* s, double rate_per_s, double duration_s, const int64_t* lengths,
+                                  int n_lengths, const float* pool_in, int64_t pool_knots, uint64_t seed,
+                                  rbd_server_stats* out);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -58,18 +58,20 @@ def test_graph_replays_with_changing_batches(rbd, ctx):
     """Replay the captured graph several times with different trajectory sets,
     with eager work and synchronisation between replays; check every replay."""
     b = rbd.Batch(ctx, 128)
-    flat = rbd.Context(precision="fp32", lanes_per_knot=16)
+    flat = {lanes: rbd.Context(precision="fp32", lanes_per_knot=lanes) for lanes in (1, 16)}
     rng = np.random.default_rng(12)
-    for rep in range(5):
+    for rep in range(6):
         lengths = rng.integers(1, 200, rng.integers(1, 100))
         xs, trajs = make_batch(rbd, lengths, seed=100 + rep)
         b.run(trajs, use_graph=True)
         rbd.hip().hipDeviceSynchronize()
-        flat.run_aos(common.knots(33, seed=rep))                      # eager work in between
+        lanes = b.last_lanes                                          # the mapping this batch ran
+        flat[1].run_aos(common.knots(33, seed=rep))                   # eager work in between
         for x, g in zip(xs, download(trajs)):
-            np.testing.assert_array_equal(g, flat.run_aos(x))
-        say(f"replay {rep}: {len(lengths)} trajectories ok")
-    flat.close()
+            np.testing.assert_array_equal(g, flat[lanes].run_aos(x))
+        say(f"replay {rep}: {len(lengths)} trajectories, {sum(lengths)} knots, lanes {lanes} ok")
+    for c in flat.values():
+        c.close()
     b.close()
 
 

@@ -1,0 +1,88 @@
+"""GPU tests of the serving loop (rbd_server_*): continuous batching, admission
+by a memory budget, overlap of batches, and results independent of batching."""
+import numpy as np
+import pytest
+
+import common
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx(rbd):
+    c = rbd.Context(precision="fp32")
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 16])
+def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes):
+    srv = rbd.Server(ctx, lanes_per_knot=lanes, slots=4)
+    rng = np.random.default_rng(21)
+    probe = common.knots(128, seed=500)
+    alone = srv.wait(srv.submit(probe))
+    say("alone done")
+    # now the same request among 300 others of mixed length, submitted back to back
+    others = [common.knots(int(rng.choice([16, 32, 64, 128, 256])), seed=600 + i) for i in range(300)]
+    ids = [srv.submit(x) for x in others[:150]]
+    pid = srv.submit(probe)
+    ids += [srv.submit(x) for x in others[150:]]
+    mixed = srv.wait(pid)
+    outs = [srv.wait(i) for i in ids]
+    np.testing.assert_array_equal(mixed, alone)
+    st = srv.stats()
+    say("stats", st)
+    assert st["requests_completed"] == 302
+    assert st["max_inflight_batches"] >= 2          # batches overlapped on the device
+    for x, o in zip(others[::37], outs[::37]):
+        err = common.max_block_errors(o, oracle.fdgrad(x.astype(np.float64)))
+        assert max(err.values()) < common.REL_TOL, err
+    # the same probe through the plain per-call path with the same mapping: identical
+    flat = rbd.Context(precision="fp32", lanes_per_knot=lanes)
+    np.testing.assert_array_equal(flat.run_aos(probe), alone)
+    flat.close()
+    srv.close()
+
+
+def test_memory_budget_is_respected(rbd, ctx):
+    """A budget that holds only a few requests: everything still completes, in
+    several batches, and the pool never exceeds the budget."""
+    per_knot = 21 * 4 + 105 * 4
+    budget = 3 * 256 * per_knot + 4096
+    srv = rbd.Server(ctx, memory_budget_bytes=budget, slots=2)
+    xs = [common.knots(256, seed=700 + i) for i in range(20)]
+    ids = [srv.submit(x) for x in xs]
+    outs = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    say("stats", st)
+    assert st["requests_completed"] == 20
+    assert st["max_budget_bytes_used"] <= budget
+    assert st["batches"] >= 7
+    flat = rbd.Context(precision="fp32", lanes_per_knot=16)
+    for x, o in zip(xs, outs):
+        np.testing.assert_array_equal(o, flat.run_aos(x))
+    flat.close()
+    srv.close()
+
+
+def test_request_larger_than_budget_is_rejected(rbd, ctx):
+    srv = rbd.Server(ctx, memory_budget_bytes=1 << 16)
+    with pytest.raises(rbd.RbdError, match="capacity"):
+        srv.submit(common.knots(256, seed=1))
+    srv.close()
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_poisson_load(rbd, ctx, use_graph):
+    srv = rbd.Server(ctx, slots=4, use_graph=use_graph)
+    pool = common.knots(8192, seed=800)
+    st = srv.run_poisson(rate=20000, duration_s=0.5, lengths=[16, 32, 64, 128, 256], pool=pool, seed=1)
+    say("poisson 20k/s", st)
+    assert st["requests_completed"] > 5000
+    assert 0 < st["p50_latency_us"] <= st["p99_latency_us"] <= st["max_latency_us"]
+    assert st["knots_per_s"] > 0
+    srv.close()
