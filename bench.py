@@ -198,6 +198,34 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000), duration_s=1.
     return res
 
 
+def profile_one(args, stream):
+    """One workload, for rocprofv3 runs (kernel trace / counters)."""
+    if args.only == "copy":
+        # a plain device-to-device copy of known size, to calibrate FETCH_SIZE/WRITE_SIZE
+        nbytes = (1 << 20) * rbdgrad.OUT_PER_KNOT * 4
+        a, b = rbdgrad.DeviceBuffer(nbytes), rbdgrad.DeviceBuffer(nbytes)
+        a.fill_zero()
+        for _ in range(args.steps):
+            rbdgrad._hipcheck(rbdgrad.hip().hipMemcpy(b.handle, a.handle, nbytes, rbdgrad.D2D), "hipMemcpy D2D")
+        rbdgrad.hip().hipDeviceSynchronize()
+        log(f"copy: {args.steps} x {nbytes} bytes device to device")
+        return
+    if args.only == "c2":
+        ctx = rbdgrad.Context(device=0, precision=args.precision)
+        res = mixed_workload(ctx, stream, steps=args.steps)
+        log(json.dumps(res))
+        return
+    lanes = 1 if args.only in ("c1-one-lane", "c4-aos", "c4-soa") else 0
+    ctx = rbdgrad.Context(device=0, precision=args.precision, lanes_per_knot=lanes)
+    if args.only.startswith("c1"):
+        n, x, layout = 128, knots(128, seed=101), "aos"
+    else:
+        n, layout = 1 << 20, args.only[3:]
+        x = knots(n, seed=104, near_singular_frac=0.1)
+    ms, _ = run_flat(ctx, x, args.steps, min(args.warmup, 10), stream, layout)
+    log(f"{args.only}: {ctx.kernel_name(n)} n={n}: {ms / args.steps * 1e3:.3f} us per launch over {args.steps}")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +235,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time per thread count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the large-batch and mixed workloads")
+    ap.add_argument("--only", choices=["c1", "c1-one-lane", "c4-aos", "c4-soa", "c2", "copy"],
+                    help="run one workload only and print its timing (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -217,6 +247,9 @@ def main():
         sys.exit(1)
 
     stream = rbdgrad.Stream()
+    if args.only:
+        profile_one(args, stream)
+        return
     ctx = rbdgrad.Context(device=0, precision=args.precision)
     threshold, tune_table = ctx.autotune(stream)
     log(f"autotune: lane-group mapping up to {threshold} knots; table (n: us one-lane, us lane-group) "

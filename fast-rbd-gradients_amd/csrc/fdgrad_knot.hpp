@@ -663,6 +663,26 @@ RBD_HD RBD_INLINE void phase_a_given_qdd_minv(const In& in, const Options& opt, 
   make_phase_c(PC, tc, qd, v, a, f, Mi);
 }
 
+// Output accessors that also want dqdd/du = Minv (dense) declare kMinv = true.
+template <class Out, class = void> struct WantsMinv { static constexpr bool value = false; };
+template <class Out> struct WantsMinv<Out, typename std::enable_if<Out::kMinv>::type> {
+  static constexpr bool value = true;
+};
+
+// column c of the symmetric Minv (upper-stored) to out.minv
+template <class Out, typename S>
+RBD_HD RBD_INLINE void store_minv_column(Out& out, const S Mi[NQ][NQ], int c) {
+  static_for<0, NQ>([&](auto Rc) {
+    constexpr int r = decltype(Rc)::value;
+    S m = S(0);
+    static_for<0, NQ>([&](auto Cc) {
+      constexpr int cc = decltype(Cc)::value;
+      m = (cc == c) ? (r <= cc ? Mi[r][cc] : Mi[cc][r]) : m;
+    });
+    out.minv(r, c, float(m));
+  });
+}
+
 // Full path for one knot in one lane.  P: arithmetic precision per phase.
 template <class P = PrecFP32, class In, class Out>
 RBD_HD RBD_INLINE void fdgrad_knot(const In& in, Out& out, const Options& opt) {
@@ -670,7 +690,40 @@ RBD_HD RBD_INLINE void fdgrad_knot(const In& in, Out& out, const Options& opt) {
   typename P::Rnea qdd[NQ];
   phase_a<P>(in, opt, PC, qdd);
   for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
+  if constexpr (WantsMinv<Out>::value)
+    for (int r = 0; r < NQ; ++r)
+      for (int c = 0; c < NQ; ++c) out.minv(r, c, float(r <= c ? PC.Mi[r][c] : PC.Mi[c][r]));
   gradient_columns(PC, opt.damping, out);
+}
+
+// Forward dynamics only: qdd = FD(q, qd, u) and the dense Minv, one lane per
+// knot (forwardDynamics with DENSE_MINV, dynamicsGradient.h:685-706, :936-939,
+// with the bias RNEA at qdd = 0).
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void fd_knot(const In& in, Out& out, const Options& opt) {
+  using SM = typename P::Minv; using SP = typename P::Pivot; using SR = typename P::Rnea;
+  using ST = Wider<SM, SR>;
+  CS<SM> tm[NQ]; CS<SR> tr[NQ];
+  SR qd[NQ], u[NQ];
+  for (int k = 0; k < NQ; ++k) {
+    ST s, c;
+    sincos_s(ST(in.q(k)), &s, &c);
+    tm[k] = CS<SM>{SM(c), SM(s)}; tr[k] = CS<SR>{SR(c), SR(s)};
+    qd[k] = SR(in.qd(k));
+    u[k] = SR(in.u(k));
+  }
+  SM Mi[NQ][NQ];
+  compute_minv<SM, SP>(tm, Mi);
+  SR v[NQ][6], a[NQ][6], f[NQ][6];
+  const SR zero[NQ] = {SR(0), SR(0), SR(0), SR(0), SR(0), SR(0), SR(0)};
+  rnea_fwd(tr, qd, zero, SR(opt.gravity), v, a, f);
+  rnea_bwd(tr, f);
+  SR bias[NQ], qdd[NQ];
+  for (int k = 0; k < NQ; ++k) bias[k] = u[k] - f[k][2] - SR(opt.damping) * qd[k];
+  minv_times<false>(Mi, bias, qdd);
+  for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
+  for (int r = 0; r < NQ; ++r)
+    for (int c = 0; c < NQ; ++c) out.minv(r, c, float(r <= c ? Mi[r][c] : Mi[c][r]));
 }
 
 // Full path for one knot spread over kLanesPerKnotGroup lanes: every lane of
@@ -685,6 +738,8 @@ RBD_HD RBD_INLINE void fdgrad_knot_lanes(const In& in, Out& out, const Options& 
     for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
     for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
   }
+  if constexpr (WantsMinv<Out>::value)
+    if (role < NQ) store_minv_column(out, PC.Mi, role);
   gradient_column_lane(PC, opt.damping, role, out);
 }
 
@@ -732,6 +787,21 @@ struct OutAoS {
   RBD_HD void qdd(int i, float x) const { p[i] = x; }
   RBD_HD void dq(int r, int c, float x) const { p[NQ + NQ * c + r] = x; }
   RBD_HD void dqd(int r, int c, float x) const { p[NQ + NQ * NQ + NQ * c + r] = x; }
+};
+// full-path output plus dqdd/du = Minv (dense, column-major, 49 per knot)
+struct OutAoSMinv {
+  static constexpr bool kMinv = true;
+  float* p; float* m;
+  RBD_HD void qdd(int i, float x) const { p[i] = x; }
+  RBD_HD void dq(int r, int c, float x) const { p[NQ + NQ * c + r] = x; }
+  RBD_HD void dqd(int r, int c, float x) const { p[NQ + NQ * NQ + NQ * c + r] = x; }
+  RBD_HD void minv(int r, int c, float x) const { m[NQ * c + r] = x; }
+};
+// forward dynamics only: qdd (7 per knot) and Minv (49 per knot, dense column-major)
+struct OutFD {
+  float* qdd_p; float* minv_p;
+  RBD_HD void qdd(int i, float x) const { qdd_p[i] = x; }
+  RBD_HD void minv(int r, int c, float x) const { minv_p[NQ * c + r] = x; }
 };
 struct OutDqddAoS {   // 98 per knot, no qdd
   float* p;

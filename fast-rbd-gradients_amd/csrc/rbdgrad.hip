@@ -116,6 +116,33 @@ __global__ __launch_bounds__(kBlock) void fdgrad_refmode_lanes_kernel(long n, co
   }
 }
 
+// Full path plus dqdd/du = Minv (49 per knot, dense), both mappings.
+template <class P, int LANES>
+__global__ __launch_bounds__(kBlock) void fdgrad_minv_kernel(long n, const float* __restrict__ in,
+                                                             float* __restrict__ out, float* __restrict__ minv,
+                                                             Options opt) {
+  const int role = threadIdx.x % LANES;
+  const long groups = long(gridDim.x) * (blockDim.x / LANES);
+  for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
+    InAoS i{in + kInPerKnot * k};
+    OutAoSMinv o{out + kOutPerKnot * k, minv + NQ * NQ * k};
+    if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
+    else fdgrad_knot_lanes<P>(i, o, opt, role);
+  }
+}
+
+// Forward dynamics only (qdd and dense Minv), one lane per knot.
+template <class P>
+__global__ __launch_bounds__(kBlock) void fd_kernel(long n, const float* __restrict__ in, float* __restrict__ qdd,
+                                                    float* __restrict__ minv, Options opt) {
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
+    InAoS i{in + kInPerKnot * k};
+    OutFD o{qdd + NQ * k, minv + NQ * NQ * k};
+    fd_knot<P>(i, o, opt);
+  }
+}
+
 }  // namespace rbd
 
 namespace rbdi {
@@ -330,6 +357,61 @@ const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knot
         "fdgrad_refmode_lanes_kernel<PrecFP64Col>", "fdgrad_refmode_lanes_kernel<PrecFP64>"}}};
   const int grouped = lanes_for(ctx, n_knots) == rbd::kLanesPerKnotGroup ? 1 : 0;
   return names[reference_mode ? 1 : 0][grouped][ctx->precision];
+}
+
+rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, float* d_out, float* d_minv,
+                                   rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
+  if (n == 0) return RBD_OK;
+  if (!d_in || !d_out || !d_minv) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+  if (rbd_status s = use_device(ctx)) return s;
+  const int lanes = lanes_for(ctx, n);
+  const dim3 grid(grid_for(n, lanes)), block(kBlock);
+  const hipStream_t st = (hipStream_t)stream;
+  auto go = [&](auto prec_tag) {
+    using P = decltype(prec_tag);
+    if (lanes == 1)
+      hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, 1>), grid, block, 0, st, long(n), d_in, d_out, d_minv, ctx->opt);
+    else
+      hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, rbd::kLanesPerKnotGroup>), grid, block, 0, st, long(n), d_in,
+                         d_out, d_minv, ctx->opt);
+  };
+  switch (ctx->precision) {
+    case RBD_PREC_FP32: go(rbd::PrecFP32{}); break;
+    case RBD_PREC_FP64_PIVOT: go(rbd::PrecFP64Pivot{}); break;
+    case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP64Col{}); break;
+    default: go(rbd::PrecFP64{}); break;
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_minv_kernel launch");
+}
+
+rbd_status rbd_forward_dynamics_f32(rbd_context* ctx, int64_t n, const float* d_in, float* d_qdd, float* d_minv,
+                                    rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
+  if (n == 0) return RBD_OK;
+  if (!d_in || !d_qdd || !d_minv) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+  if (rbd_status s = use_device(ctx)) return s;
+  const dim3 grid(grid_for(n, 1)), block(kBlock);
+  const hipStream_t st = (hipStream_t)stream;
+  // forward dynamics has no derivative columns: FP64_COLUMNS computes it as FP32 does
+  switch (ctx->precision) {
+    case RBD_PREC_FP32:
+    case RBD_PREC_FP64_COLUMNS:
+      hipLaunchKernelGGL(rbd::fd_kernel<rbd::PrecFP32>, grid, block, 0, st, long(n), d_in, d_qdd, d_minv, ctx->opt);
+      break;
+    case RBD_PREC_FP64_PIVOT:
+      hipLaunchKernelGGL(rbd::fd_kernel<rbd::PrecFP64Pivot>, grid, block, 0, st, long(n), d_in, d_qdd, d_minv,
+                         ctx->opt);
+      break;
+    default:
+      hipLaunchKernelGGL(rbd::fd_kernel<rbd::PrecFP64>, grid, block, 0, st, long(n), d_in, d_qdd, d_minv, ctx->opt);
+      break;
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "fd_kernel launch");
 }
 
 int64_t rbd_group_threshold(rbd_context* ctx) { return ctx ? ctx->group_threshold.load() : -1; }

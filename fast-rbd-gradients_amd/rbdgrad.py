@@ -39,7 +39,7 @@ EXPORTED = [
     "rbd_batch_last_lanes",
     "rbd_server_config_default", "rbd_server_create", "rbd_server_destroy", "rbd_server_submit",
     "rbd_server_poll", "rbd_server_wait", "rbd_server_get_stats", "rbd_server_reset_stats",
-    "rbd_server_run_poisson",
+    "rbd_server_run_poisson", "rbd_fdgrad_minv_traj_f32", "rbd_forward_dynamics_f32",
 ]
 
 
@@ -102,6 +102,8 @@ def lib():
         L.rbd_fdgrad_traj_soa_f32.argtypes = [vp, i64, vp, i64, vp, i64, vp]
         L.rbd_dynamics_gradient_kernel_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_fdgrad_traj_host_f32.argtypes = [vp, i64, vp, vp]
+        L.rbd_fdgrad_minv_traj_f32.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.rbd_forward_dynamics_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_kernel_name.argtypes = [vp, ctypes.c_int, i64]
         L.rbd_kernel_name.restype = ctypes.c_char_p
         L.rbd_autotune.argtypes = [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_float), ctypes.c_int]
@@ -361,6 +363,28 @@ class Context:
         d_out = DeviceBuffer(OUT_PER_KNOT * ld * 4)
         self.fdgrad_soa(n, d_in, ld, d_out, ld)
         return d_out.download((OUT_PER_KNOT, ld))[:, :n].T.copy()
+
+    def run_with_minv(self, x):
+        """Full path plus dqdd/du = Minv: returns ((n, 105), (n, 7, 7) Minv [knot][row][col])."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+        n = x.shape[0]
+        d_in = DeviceBuffer.from_array(x)
+        d_out = DeviceBuffer(n * OUT_PER_KNOT * 4)
+        d_m = DeviceBuffer(n * 49 * 4)
+        _check(lib().rbd_fdgrad_minv_traj_f32(self._h, n, d_in.handle, d_out.handle, d_m.handle, None),
+               "rbd_fdgrad_minv_traj_f32")
+        return d_out.download((n, OUT_PER_KNOT)), d_m.download((n, 7, 7)).transpose(0, 2, 1).copy()
+
+    def forward_dynamics(self, x):
+        """qdd (n, 7) and Minv (n, 7, 7) [knot][row][col] from x (n, 21)."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+        n = x.shape[0]
+        d_in = DeviceBuffer.from_array(x)
+        d_q = DeviceBuffer(n * 7 * 4)
+        d_m = DeviceBuffer(n * 49 * 4)
+        _check(lib().rbd_forward_dynamics_f32(self._h, n, d_in.handle, d_q.handle, d_m.handle, None),
+               "rbd_forward_dynamics_f32")
+        return d_q.download((n, 7)), d_m.download((n, 7, 7)).transpose(0, 2, 1).copy()
 
     def run_refmode(self, vol):
         vol = np.ascontiguousarray(np.atleast_2d(vol), dtype=np.float32)

@@ -123,6 +123,24 @@ rbd_status rbd_dynamics_gradient_kernel_f32(rbd_context* ctx, int64_t n_knots, f
  * Replaces: the H2D copy + kernel + D2H copy sequence of time_GPU.cu:313-316. */
 rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n_knots, const float* h_in, float* h_out);
 
+/* Full path plus dqdd/du = Minv: as rbd_fdgrad_traj_f32, and d_minv receives
+ * the dense Minv of every knot (49 floats per knot, column-major).  The three
+ * 7x7 blocks dqdd/dq, dqdd/dqd, dqdd/du are the WAFR baseline's gradient output
+ * (time_GPU_WAFR.cu:18) and what iLQR/DDP callers consume.
+ * Replaces: forwardDynamicsGradient + knot.Minv (dynamicsGradient.h:920-922:
+ * "dqdd/dtau = Minv"). */
+rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, float* d_out,
+                                   float* d_minv, rbd_stream_t stream);
+
+/* Forward dynamics only: qdd (7 per knot) and the dense Minv (49 per knot,
+ * column-major) from q, qd, u (21 per knot), one lane per knot.  The bias RNEA
+ * runs with qdd = 0.
+ * Replaces: forwardDynamics<T,MPC,VD,DENSE_MINV=1>(knot) (dynamicsGradient.h:685-706,
+ * :936-939), which the reference runs on the CPU before every GPU timing
+ * (time_GPU.cu:134). */
+rbd_status rbd_forward_dynamics_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, float* d_qdd,
+                                    float* d_minv, rbd_stream_t stream);
+
 /* Name of the kernel the context dispatches a call of n_knots to (full path,
  * or the reference-compatible mode when reference_mode != 0), as profilers list
  * it (template arguments abbreviated).  No reference counterpart. */

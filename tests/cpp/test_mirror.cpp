@@ -1,0 +1,105 @@
+// C++ parity test through the reference-shaped interface (rbdgrad.hpp), laid
+// out like the reference's GPU harness (time_GPU.cu test(): random knots,
+// forwardDynamics with DENSE_MINV, pack COMPLETELY_FUSED, dynamicsGradientKernel,
+// copy back).  Checks, on the GPU:
+//   1. dynamicsGradientKernel on (q, qd, u, qdd, Minv) equals the full path's
+//      dqdd blocks (same knots) to FP32 rounding;
+//   2. forwardDynamics' qdd equals the full path's qdd bit for bit;
+//   3. the TEST_FOR_EQUIVALENCE inputs (all 0.1, time_GPU.cu:124-127) match
+//      the values the FP64 oracle gives for them (passed on the command line).
+// Exit code 0 on success; prints what it checked.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+#include "../../fast-rbd-gradients_amd/rbdgrad.hpp"
+
+using namespace rbdgrad;
+
+template <int N, bool VD>
+int run(std::mt19937& rng) {
+  std::uniform_real_distribution<float> uq(-3.14159265f, 3.14159265f), uqd(-2.f, 2.f), uu(-50.f, 50.f);
+  auto* t = new traj<float, N>;
+  for (int k = 0; k < N; ++k)
+    for (int i = 0; i < NUM_POS; ++i) {
+      t->knots[k].q[i] = uq(rng);
+      t->knots[k].qd[i] = uqd(rng);
+      t->knots[k].u[i] = uu(rng);
+    }
+  rbdErrchk((forwardDynamics<float, N, false, VD>(t)));
+  std::vector<float> vol(size_t(COMPLETELY_FUSED_VALS) * N), in(21 * size_t(N));
+  packCompletelyFused<float, N>(*t, vol.data());
+  for (int k = 0; k < N; ++k)
+    for (int i = 0; i < 21; ++i) in[21 * k + i] = vol[COMPLETELY_FUSED_VALS * k + i];
+  float *d_vol, *d_dqdd, *d_in, *d_out;
+  if (hipMalloc(&d_vol, vol.size() * 4) || hipMalloc(&d_dqdd, size_t(N) * DQDD_VALS * 4) ||
+      hipMalloc(&d_in, in.size() * 4) || hipMalloc(&d_out, size_t(N) * 105 * 4)) {
+    std::fprintf(stderr, "hipMalloc failed\n");
+    return 1;
+  }
+  (void)hipMemcpy(d_vol, vol.data(), vol.size() * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(d_in, in.data(), in.size() * 4, hipMemcpyHostToDevice);
+  rbdErrchk((dynamicsGradientKernel<float, N, false, VD>(d_dqdd, d_vol, nullptr)));
+  rbdErrchk((forwardDynamicsGradient<float, N, false, VD>(d_out, d_in)));
+  (void)hipDeviceSynchronize();
+  std::vector<float> dqdd(size_t(N) * DQDD_VALS), out(size_t(N) * 105);
+  (void)hipMemcpy(dqdd.data(), d_dqdd, dqdd.size() * 4, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(out.data(), d_out, out.size() * 4, hipMemcpyDeviceToHost);
+  double worst = 0;
+  int qdd_mismatch = 0;
+  for (int k = 0; k < N; ++k) {
+    for (int b = 0; b < 2; ++b) {
+      double d = 0, m = 0;
+      for (int i = 0; i < 49; ++i) {
+        const double a = dqdd[DQDD_VALS * k + 49 * b + i], r = out[105 * k + 7 + 49 * b + i];
+        d = std::fmax(d, std::fabs(a - r));
+        m = std::fmax(m, std::fabs(r));
+      }
+      worst = std::fmax(worst, d / std::fmax(m, 1e-6));
+    }
+    for (int i = 0; i < NUM_POS; ++i) qdd_mismatch += t->knots[k].qdd[i] != out[105 * k + i];
+  }
+  std::printf("N=%d VEL_DAMPING=%d: reference-mode vs full path max rel err %.3e, qdd mismatches %d\n", N, int(VD),
+              worst, qdd_mismatch);
+  (void)hipFree(d_vol); (void)hipFree(d_dqdd); (void)hipFree(d_in); (void)hipFree(d_out);
+  delete t;
+  return (worst < 1e-4 && qdd_mismatch == 0) ? 0 : 1;
+}
+
+int main(int argc, char** argv) {
+  std::mt19937 rng(1234);
+  int bad = 0;
+  bad += run<16, false>(rng);
+  bad += run<128, false>(rng);
+  bad += run<128, true>(rng);
+  // TEST_FOR_EQUIVALENCE inputs against expected values given on the command line:
+  // argv[1..] = 105 floats (qdd, dqdd/dq, dqdd/dqd) from the FP64 oracle
+  if (argc == 106) {
+    float in[21], *d_in, *d_out;
+    for (float& x : in) x = 0.1f;
+    (void)hipMalloc(&d_in, sizeof(in));
+    (void)hipMalloc(&d_out, 105 * 4);
+    (void)hipMemcpy(d_in, in, sizeof(in), hipMemcpyHostToDevice);
+    rbdErrchk((forwardDynamicsGradient<float, 1, false, false>(d_out, d_in)));
+    float out[105];
+    (void)hipMemcpy(out, d_out, sizeof(out), hipMemcpyDeviceToHost);
+    double worst = 0;
+    for (int b = 0; b < 3; ++b) {
+      const int lo = b == 0 ? 0 : 7 + 49 * (b - 1), hi = b == 0 ? 7 : lo + 49;
+      double d = 0, m = 0;
+      for (int i = lo; i < hi; ++i) {
+        const double r = std::atof(argv[1 + i]);
+        d = std::fmax(d, std::fabs(out[i] - r));
+        m = std::fmax(m, std::fabs(r));
+      }
+      worst = std::fmax(worst, d / std::fmax(m, 1e-6));
+    }
+    std::printf("TEST_FOR_EQUIVALENCE inputs (all 0.1): max rel err vs FP64 oracle %.3e\n", worst);
+    bad += worst < 1e-4 ? 0 : 1;
+    (void)hipFree(d_in); (void)hipFree(d_out);
+  }
+  std::printf(bad ? "FAIL\n" : "OK\n");
+  return bad ? 1 : 0;
+}

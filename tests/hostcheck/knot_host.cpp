@@ -79,6 +79,28 @@ void host_model_transform(int k, double q, double* out) {
   }
 }
 
+// forward dynamics only (qdd 7 + Minv 49 dense per knot), FP32
+void host_fd_f32(long n, const float* in, float* qdd, float* minv, int flags) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    InAoS i{in + kInPerKnot * k};
+    OutFD o{qdd + NQ * k, minv + NQ * NQ * k};
+    fd_knot<PrecFP32>(i, o, opt);
+  }
+}
+// full path plus Minv, one-lane (lanes = 0) or lane-group emulation (lanes = 1)
+void host_fdgrad_minv_f32(long n, const float* in, float* out, float* minv, int flags, int lanes) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    InAoS i{in + kInPerKnot * k};
+    OutAoSMinv o{out + kOutPerKnot * k, minv + NQ * NQ * k};
+    if (lanes)
+      for (int role = 0; role < kLanesPerKnotGroup; ++role) fdgrad_knot_lanes<PrecFP32>(i, o, opt, role);
+    else
+      fdgrad_knot<PrecFP32>(i, o, opt);
+  }
+}
+
 void host_fdgrad_f32(long n, const float* in, float* out, int flags, int prec, int lanes) {
   by_precision(prec, false, flags, n, in, out, lanes);
 }

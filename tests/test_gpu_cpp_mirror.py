@@ -1,0 +1,49 @@
+"""Runs the C++ parity program tests/cpp/test_mirror (built by
+__graft_entry__.build()), which calls the library through the reference-shaped
+C++ interface fast-rbd-gradients_amd/rbdgrad.hpp the way the reference's
+time_GPU.cu harness calls its kernel."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import common
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(common.ROOT, "tests", "cpp", "test_mirror")
+
+
+def test_cpp_mirror_program(rbd, oracle):
+    assert os.path.exists(BIN), "tests/cpp/test_mirror is not built (run __graft_entry__.build())"
+    ref = oracle.fdgrad(np.full((1, 21), np.float32(0.1)).astype(np.float64))[0]
+    res = subprocess.run([BIN] + [repr(float(v)) for v in ref], capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr, flush=True)
+    assert res.returncode == 0
+    assert "TEST_FOR_EQUIVALENCE" in res.stdout and res.stdout.strip().endswith("OK")
+
+
+def test_forward_dynamics_entry_point(rbd, oracle):
+    x = common.knots(333, seed=900)
+    ctx = rbd.Context(precision="fp32")
+    qdd, minv = ctx.forward_dynamics(x)
+    full = ctx.run_aos(x)
+    np.testing.assert_array_equal(qdd, full[:, :7])
+    for i in range(0, 333, 37):
+        q_ref, m_ref = oracle.fd(*(x[i, s].astype(np.float64) for s in (slice(0, 7), slice(7, 14), slice(14, 21))))
+        np.testing.assert_allclose(minv[i], m_ref, rtol=0, atol=2e-6 * np.abs(m_ref).max())
+        np.testing.assert_allclose(qdd[i], q_ref, rtol=0, atol=2e-6 * np.abs(q_ref).max())
+    ctx.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 16])
+def test_dqdd_du_output(rbd, lanes):
+    x = common.knots(200, seed=901)
+    ctx = rbd.Context(precision="fp32", lanes_per_knot=lanes)
+    out, minv = ctx.run_with_minv(x)
+    np.testing.assert_array_equal(out, ctx.run_aos(x))
+    _, minv_fd = ctx.forward_dynamics(x)
+    np.testing.assert_array_equal(minv, minv_fd)
+    np.testing.assert_array_equal(minv, minv.transpose(0, 2, 1))
+    ctx.close()

@@ -23,6 +23,8 @@ def host():
     L.host_fdgrad_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.host_fdgrad_refmode_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.host_model_inertia.argtypes = [ctypes.c_int, dp]
+    L.host_fd_f32.argtypes = [ctypes.c_long, fp, fp, fp, ctypes.c_int]
+    L.host_fdgrad_minv_f32.argtypes = [ctypes.c_long, fp, fp, fp, ctypes.c_int, ctypes.c_int]
     L.host_model_transform.argtypes = [ctypes.c_int, ctypes.c_double, dp]
     return L
 
@@ -102,3 +104,27 @@ def test_lane_group_mapping_matches_single_lane(host, oracle, flags, prec):
     np.testing.assert_allclose(grp, one, rtol=1e-5, atol=1e-6 * np.abs(one).max())
     err = common.max_block_errors(grp, oracle.fdgrad(x.astype(np.float64), flags))
     assert max(err.values()) < common.REL_TOL, err
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2])
+def test_forward_dynamics_and_minv_outputs(host, oracle, flags):
+    x = common.knots(100, seed=230 + flags)
+    fp = ctypes.POINTER(ctypes.c_float)
+    xc = np.ascontiguousarray(x)
+    qdd = np.zeros((100, 7), np.float32)
+    minv = np.zeros((100, 49), np.float32)
+    host.host_fd_f32(100, xc.ctypes.data_as(fp), qdd.ctypes.data_as(fp), minv.ctypes.data_as(fp), flags)
+    full = run(host, x, flags)
+    for lanes in (0, 1):
+        out2 = np.full((100, 105), np.nan, np.float32)
+        minv2 = np.full((100, 49), np.nan, np.float32)
+        host.host_fdgrad_minv_f32(100, xc.ctypes.data_as(fp), out2.ctypes.data_as(fp), minv2.ctypes.data_as(fp),
+                                  flags, lanes)
+        np.testing.assert_array_equal(minv2, minv)             # same Minv arithmetic
+        assert np.all(np.isfinite(out2))
+    np.testing.assert_array_equal(qdd, full[:, :7])
+    for i in range(0, 100, 9):
+        q_ref, m_ref = oracle.fd(x[i, :7].astype(np.float64), x[i, 7:14].astype(np.float64),
+                                 x[i, 14:].astype(np.float64), flags)
+        np.testing.assert_allclose(minv[i].reshape(7, 7).T, m_ref, rtol=0, atol=1e-5 * np.abs(m_ref).max())
+        np.testing.assert_allclose(qdd[i], q_ref, rtol=0, atol=1e-5 * np.abs(q_ref).max())
