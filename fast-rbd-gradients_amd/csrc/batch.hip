@@ -15,9 +15,10 @@ using namespace rbdi;
 struct rbd_batch {
   rbd_context* ctx = nullptr;
   int max_traj = 0;
-  // device: [BatchHeader | TrajDesc[max_traj] | long prefix[max_traj + 1]]
+  // device: [BatchHeader | TrajDesc[max_traj] | long prefix[max_traj + 1] | BatchQueue]
   char* d_mem = nullptr;
   rbd::BatchHeader* d_hdr = nullptr;
+  rbd::BatchQueue* d_queue = nullptr;
   rbd::TrajDesc* d_desc = nullptr;
   long* d_prefix = nullptr;
   size_t upload_bytes = 0;            // header + descriptors
@@ -26,10 +27,11 @@ struct rbd_batch {
   char* h_stage[kRing] = {};
   hipEvent_t staged[kRing] = {};
   int ring = 0;
-  // persistent grid (waves) per mapping, and the captured graphs
+  // persistent grid (waves) per mapping, and the captured graphs per
+  // (mapping, small batch): index 2 * (lanes == 16) + small
   unsigned grid[2] = {0, 0};          // [0]: one lane per knot, [1]: lane groups
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
   int64_t last_total_knots = 0;
   int last_lanes = 0;
 };
@@ -38,39 +40,48 @@ namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+template <class P, int LANES, bool SMALL>
+void launch_persistent(rbd_batch* b, hipStream_t s) {
+  const unsigned grid = b->grid[LANES == 1 ? 0 : 1];
+  hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, SMALL>), dim3(grid), dim3(kBlock), 0, s, b->d_desc,
+                     b->d_hdr, b->d_queue, b->d_prefix, b->ctx->opt);
+}
+
 template <class P>
-hipError_t launch_batch_p(rbd_batch* b, int lanes, hipStream_t s) {
-  hipLaunchKernelGGL(rbd::batch_plan_kernel, dim3(1), dim3(rbd::kPlanThreads), 0, s, b->d_desc, b->d_hdr,
-                     b->d_prefix);
-  const unsigned grid = b->grid[lanes == 1 ? 0 : 1];
-  if (lanes == 1)
-    hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, 1>), dim3(grid), dim3(kBlock), 0, s, b->d_desc, b->d_hdr,
-                       b->d_prefix, b->ctx->opt);
-  else
-    hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, rbd::kLanesPerKnotGroup>), dim3(grid), dim3(kBlock), 0, s,
-                       b->d_desc, b->d_hdr, b->d_prefix, b->ctx->opt);
+hipError_t launch_batch_p(rbd_batch* b, int lanes, bool small, hipStream_t s) {
+  if (!small)
+    hipLaunchKernelGGL(rbd::batch_plan_kernel, dim3(1), dim3(rbd::kPlanThreads), 0, s, b->d_desc, b->d_hdr,
+                       b->d_prefix);
+  if (lanes == 1) {
+    if (small) launch_persistent<P, 1, true>(b, s); else launch_persistent<P, 1, false>(b, s);
+  } else {
+    if (small) launch_persistent<P, rbd::kLanesPerKnotGroup, true>(b, s);
+    else launch_persistent<P, rbd::kLanesPerKnotGroup, false>(b, s);
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_batch(rbd_batch* b, int lanes, hipStream_t s) {
+hipError_t launch_batch(rbd_batch* b, int lanes, bool small, hipStream_t s) {
   switch (b->ctx->precision) {
-    case RBD_PREC_FP32: return launch_batch_p<rbd::PrecFP32>(b, lanes, s);
-    case RBD_PREC_FP64_PIVOT: return launch_batch_p<rbd::PrecFP64Pivot>(b, lanes, s);
-    case RBD_PREC_FP64_COLUMNS: return launch_batch_p<rbd::PrecFP64Col>(b, lanes, s);
-    default: return launch_batch_p<rbd::PrecFP64>(b, lanes, s);
+    case RBD_PREC_FP32: return launch_batch_p<rbd::PrecFP32>(b, lanes, small, s);
+    case RBD_PREC_FP64_PIVOT: return launch_batch_p<rbd::PrecFP64Pivot>(b, lanes, small, s);
+    case RBD_PREC_FP64_COLUMNS: return launch_batch_p<rbd::PrecFP64Col>(b, lanes, small, s);
+    default: return launch_batch_p<rbd::PrecFP64>(b, lanes, small, s);
   }
 }
 
-// capture plan + persistent kernels for one mapping into a graph (once)
-rbd_status ensure_graph(rbd_batch* b, int lanes) {
-  const int gi = lanes == 1 ? 0 : 1;
+int graph_index(int lanes, bool small) { return 2 * (lanes == 1 ? 0 : 1) + (small ? 1 : 0); }
+
+// capture the batch's kernels for one (mapping, small) pair into a graph (once)
+rbd_status ensure_graph(rbd_batch* b, int lanes, bool small) {
+  const int gi = graph_index(lanes, small);
   if (b->exec[gi]) return RBD_OK;
   hipStream_t cs = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate (capture)");
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t le = launch_batch(b, lanes, cs);
+    hipError_t le = launch_batch(b, lanes, small, cs);
     e = hipStreamEndCapture(cs, &b->graph[gi]);
     if (le != hipSuccess) e = le;
   }
@@ -102,13 +113,17 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
   b->max_traj = max_trajectories;
   const size_t hdr = align_up(sizeof(rbd::BatchHeader), 256);
   const size_t desc = align_up(sizeof(rbd::TrajDesc) * size_t(max_trajectories), 256);
-  const size_t pre = sizeof(long) * (size_t(max_trajectories) + 1);
+  const size_t pre = align_up(sizeof(long) * (size_t(max_trajectories) + 1), 256);
   b->upload_bytes = hdr + sizeof(rbd::TrajDesc) * size_t(max_trajectories);
-  hipError_t e = hipMalloc(&b->d_mem, hdr + desc + pre);
+  hipError_t e = hipMalloc(&b->d_mem, hdr + desc + pre + sizeof(rbd::BatchQueue));
   if (e != hipSuccess) { delete b; return fail(RBD_ERR_OUT_OF_MEMORY, "hipMalloc (batch)"); }
   b->d_hdr = reinterpret_cast<rbd::BatchHeader*>(b->d_mem);
   b->d_desc = reinterpret_cast<rbd::TrajDesc*>(b->d_mem + hdr);
   b->d_prefix = reinterpret_cast<long*>(b->d_mem + hdr + desc);
+  b->d_queue = reinterpret_cast<rbd::BatchQueue*>(b->d_mem + hdr + desc + pre);
+  // queue counters start at zero (set once, outside any captured graph)
+  e = hipMemset(b->d_queue, 0, sizeof(rbd::BatchQueue));
+  if (e != hipSuccess) { (void)hipFree(b->d_mem); delete b; return hip_fail(e, "hipMemset (batch queue)"); }
   for (int i = 0; i < rbd_batch::kRing && e == hipSuccess; ++i) {
     e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stage[i]), b->upload_bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
@@ -127,7 +142,7 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
 rbd_status rbd_batch_destroy(rbd_batch* b) {
   if (!b) return RBD_OK;
   (void)use_device(b->ctx);
-  for (int gi = 0; gi < 2; ++gi) {
+  for (int gi = 0; gi < 4; ++gi) {
     if (b->exec[gi]) (void)hipGraphExecDestroy(b->exec[gi]);
     if (b->graph[gi]) (void)hipGraphDestroy(b->graph[gi]);
   }
@@ -145,11 +160,14 @@ rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves) 
   if (lanes_per_knot != 1 && lanes_per_knot != rbd::kLanesPerKnotGroup)
     return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 1 or 16");
   if (waves == 0 || waves > (1u << 20)) return fail(RBD_ERR_INVALID_ARGUMENT, "waves must be in [1, 2^20]");
-  const int gi = lanes_per_knot == 1 ? 0 : 1;
-  if (b->grid[gi] != waves) {
-    b->grid[gi] = waves;
-    if (b->exec[gi]) { (void)hipGraphExecDestroy(b->exec[gi]); b->exec[gi] = nullptr; }
-    if (b->graph[gi]) { (void)hipGraphDestroy(b->graph[gi]); b->graph[gi] = nullptr; }
+  const int mi = lanes_per_knot == 1 ? 0 : 1;
+  if (b->grid[mi] != waves) {
+    b->grid[mi] = waves;
+    for (int small = 0; small < 2; ++small) {
+      const int gi = graph_index(lanes_per_knot, small != 0);
+      if (b->exec[gi]) { (void)hipGraphExecDestroy(b->exec[gi]); b->exec[gi] = nullptr; }
+      if (b->graph[gi]) { (void)hipGraphDestroy(b->graph[gi]); b->graph[gi] = nullptr; }
+    }
   }
   return RBD_OK;
 }
@@ -189,12 +207,13 @@ rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_tra
                      st);
   if (e == hipSuccess) e = hipEventRecord(b->staged[r], st);
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+  const bool small = n_traj <= rbd::kSmallBatch;
   if (use_graph) {
-    if (rbd_status s = ensure_graph(b, lanes)) return s;
-    e = hipGraphLaunch(b->exec[lanes == 1 ? 0 : 1], st);
+    if (rbd_status s = ensure_graph(b, lanes, small)) return s;
+    e = hipGraphLaunch(b->exec[graph_index(lanes, small)], st);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   } else {
-    e = launch_batch(b, lanes, st);
+    e = launch_batch(b, lanes, small, st);
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   }
   return RBD_OK;

@@ -4,11 +4,12 @@
 // A request is one trajectory: host input n x 21, host output n x 105, both
 // caller-owned and valid until the request completes.  The dispatcher thread
 // admits queued requests in arrival order while the fixed memory budget
-// allows (an admitted request holds n x 504 bytes, rounded up, of device memory
-// and the same of pinned staging until its result is delivered); it packs
-// admitted requests into a batch and runs the batch on a free in-flight slot
-// (own stream, own rbd_batch work queue): input -> pinned staging -> H2D
-// copies, one persistent-kernel launch, D2H copies, an event.  While batches
+// allows: a batch holds one contiguous range [inputs of all its requests |
+// outputs of all its requests] (n x 504 bytes per request, the two halves
+// rounded up to 256 bytes) of device memory, and the same range of pinned
+// staging, until its results are delivered.  It runs the batch on a free
+// in-flight slot (own stream, own rbd_batch work queue): inputs -> pinned
+// staging, one H2D copy, one persistent-kernel launch, one D2H copy, an event.  While batches
 // are on the device the dispatcher keeps admitting newly arrived requests into
 // the next free slot, so work from several batches overlaps on the GPU.  The
 // completion thread waits for slots in launch order, copies results to the
@@ -80,6 +81,11 @@ class RangeAllocator {
       }
     }
   }
+  size_t largest() const {                 // longest free range
+    size_t m = 0;
+    for (const auto& f : free_) m = std::max(m, f.second);
+    return m;
+  }
   size_t used() const { return used_; }
   size_t size() const { return size_; }
   size_t free_ranges() const { return free_.size(); }
@@ -95,7 +101,7 @@ struct Request {
   int64_t n = 0;
   const float* h_in = nullptr;   // caller's buffers, valid until completion
   float* h_out = nullptr;
-  size_t off = SIZE_MAX;         // offset of [in | out] in the device pool and in the pinned pool
+  size_t in_off = 0, out_off = 0;  // offsets of its input and output in the pools
   size_t in_bytes = 0, out_bytes = 0, bytes = 0;
   Clock::time_point t_submit;
   int status = 0;                // 0 pending, 1 done, -1 failed
@@ -106,6 +112,8 @@ struct Slot {
   hipEvent_t done = nullptr;
   rbd_batch* batch = nullptr;
   std::vector<std::shared_ptr<Request>> reqs;
+  size_t off = 0, bytes = 0;            // the batch's range in the pools
+  size_t in_bytes = 0, out_bytes = 0;   // packed inputs, packed outputs
   bool busy = false;
 };
 
@@ -170,50 +178,61 @@ void dispatch_loop(rbd_server* s) {
     for (int i = 0; i < int(s->slots.size()); ++i)
       if (!s->slots[i].busy) { slot = i; break; }
     if (slot < 0) continue;
-    // admit in arrival order while the budget and the batch limits allow
+    // admit in arrival order while the budget and the batch limits allow; the
+    // batch takes one range, so it must fit the longest free range
     std::vector<std::shared_ptr<Request>> admitted;
     int64_t knots = 0;
-    while (!s->queue.empty() && int64_t(admitted.size()) < s->cfg.max_batch_trajectories) {
-      const std::shared_ptr<Request>& r = s->queue.front();
+    size_t in_bytes = 0, out_bytes = 0;
+    const size_t room = s->alloc->largest();
+    for (const auto& r : s->queue) {
+      if (int64_t(admitted.size()) >= s->cfg.max_batch_trajectories) break;
       if (knots + r->n > s->cfg.max_batch_knots) break;
-      const size_t off = s->alloc->alloc(r->bytes);
-      if (off == SIZE_MAX) break;                          // budget full: wait for returns
-      r->off = off;
+      if (round_up(in_bytes + r->in_bytes) + round_up(out_bytes + r->out_bytes) > room) break;
+      r->in_off = in_bytes;                                 // relative to the batch, fixed up below
+      r->out_off = out_bytes;
+      in_bytes += r->in_bytes;
+      out_bytes += r->out_bytes;
       knots += r->n;
       admitted.push_back(r);
-      s->queue.pop_front();
     }
-    if (admitted.empty()) {
+    size_t off = SIZE_MAX;
+    if (!admitted.empty()) off = s->alloc->alloc(round_up(in_bytes) + round_up(out_bytes));
+    if (off == SIZE_MAX) {
       // the head request does not fit yet: sleep until memory comes back
       const uint64_t gen = s->mem_gen;
       s->cv_dispatch.wait(lk, [s, gen] { return s->stop || s->mem_gen != gen; });
       continue;
+    }
+    s->queue.erase(s->queue.begin(), s->queue.begin() + std::ptrdiff_t(admitted.size()));
+    for (auto& r : admitted) {
+      r->in_off += off;
+      r->out_off += off + round_up(in_bytes);
     }
     s->budget_used_max = std::max(s->budget_used_max, s->alloc->used());
     s->admitted_knots_max = std::max(s->admitted_knots_max, knots);
     Slot& sl = s->slots[size_t(slot)];
     sl.busy = true;
     sl.reqs = std::move(admitted);
+    sl.off = off;
+    sl.bytes = round_up(in_bytes) + round_up(out_bytes);
+    sl.in_bytes = in_bytes;
+    sl.out_bytes = out_bytes;
     lk.unlock();
-    // stage inputs, H2D, one batched launch, D2H, event
+    // stage inputs, one H2D copy, one batched launch, one D2H copy, event
     std::vector<rbd_traj> trajs;
     trajs.reserve(sl.reqs.size());
-    hipError_t e = hipSuccess;
     for (auto& r : sl.reqs) {
-      std::memcpy(s->h_pool + r->off, r->h_in, r->in_bytes);
-      char* d = s->d_pool + r->off;
-      if (e == hipSuccess) e = hipMemcpyAsync(d, s->h_pool + r->off, r->in_bytes, hipMemcpyHostToDevice, sl.stream);
-      trajs.push_back(rbd_traj{reinterpret_cast<const float*>(d), reinterpret_cast<float*>(d + round_up(r->in_bytes)),
-                               r->n});
+      std::memcpy(s->h_pool + r->in_off, r->h_in, r->in_bytes);
+      trajs.push_back(rbd_traj{reinterpret_cast<const float*>(s->d_pool + r->in_off),
+                               reinterpret_cast<float*>(s->d_pool + r->out_off), r->n});
     }
+    const size_t out0 = off + round_up(in_bytes);
+    hipError_t e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, in_bytes, hipMemcpyHostToDevice, sl.stream);
     rbd_status st = RBD_OK;
     if (e == hipSuccess)
       st = rbd_fdgrad_batched_f32(sl.batch, trajs.data(), int(trajs.size()), s->cfg.use_graph, sl.stream);
-    for (auto& r : sl.reqs) {
-      const size_t o = r->off + round_up(r->in_bytes);
-      if (e == hipSuccess && st == RBD_OK)
-        e = hipMemcpyAsync(s->h_pool + o, s->d_pool + o, r->out_bytes, hipMemcpyDeviceToHost, sl.stream);
-    }
+    if (e == hipSuccess && st == RBD_OK)
+      e = hipMemcpyAsync(s->h_pool + out0, s->d_pool + out0, out_bytes, hipMemcpyDeviceToHost, sl.stream);
     if (e == hipSuccess && st == RBD_OK) e = hipEventRecord(sl.done, sl.stream);
     lk.lock();
     if (e != hipSuccess || st != RBD_OK) s->async_error = st != RBD_OK ? st : RBD_ERR_HIP;
@@ -238,14 +257,13 @@ void complete_loop(rbd_server* s) {
     lk.unlock();
     const hipError_t e = failed_launch ? hipErrorUnknown : hipEventSynchronize(sl.done);
     if (e == hipSuccess)
-      for (auto& r : sl.reqs)
-        std::memcpy(r->h_out, s->h_pool + r->off + round_up(r->in_bytes), r->out_bytes);
+      for (auto& r : sl.reqs) std::memcpy(r->h_out, s->h_pool + r->out_off, r->out_bytes);
     const auto now = Clock::now();
     lk.lock();
     s->launched.pop_front();
+    s->alloc->release(sl.off, sl.bytes);
     for (auto& r : sl.reqs) {
       r->status = (e == hipSuccess) ? 1 : -1;
-      s->alloc->release(r->off, r->bytes);
       if (e == hipSuccess) {
         s->latencies_us.push_back(std::chrono::duration<double, std::micro>(now - r->t_submit).count());
         s->knots_done += r->n;

@@ -4,7 +4,9 @@
 // copy back).  Checks, on the GPU:
 //   1. dynamicsGradientKernel on (q, qd, u, qdd, Minv) equals the full path's
 //      dqdd blocks (same knots) to FP32 rounding;
-//   2. forwardDynamics' qdd equals the full path's qdd bit for bit;
+//   2. forwardDynamics' qdd equals the full path's qdd to FP32 rounding (the
+//      two are separate kernels, so the compiler may contract different
+//      products into FMAs);
 //   3. the TEST_FOR_EQUIVALENCE inputs (all 0.1, time_GPU.cu:124-127) match
 //      the values the FP64 oracle gives for them (passed on the command line).
 // Exit code 0 on success; prints what it checked.
@@ -47,8 +49,7 @@ int run(std::mt19937& rng) {
   std::vector<float> dqdd(size_t(N) * DQDD_VALS), out(size_t(N) * 105);
   (void)hipMemcpy(dqdd.data(), d_dqdd, dqdd.size() * 4, hipMemcpyDeviceToHost);
   (void)hipMemcpy(out.data(), d_out, out.size() * 4, hipMemcpyDeviceToHost);
-  double worst = 0;
-  int qdd_mismatch = 0;
+  double worst = 0, qdd_worst = 0;
   for (int k = 0; k < N; ++k) {
     for (int b = 0; b < 2; ++b) {
       double d = 0, m = 0;
@@ -59,13 +60,18 @@ int run(std::mt19937& rng) {
       }
       worst = std::fmax(worst, d / std::fmax(m, 1e-6));
     }
-    for (int i = 0; i < NUM_POS; ++i) qdd_mismatch += t->knots[k].qdd[i] != out[105 * k + i];
+    double d = 0, m = 0;
+    for (int i = 0; i < NUM_POS; ++i) {
+      d = std::fmax(d, std::fabs(double(t->knots[k].qdd[i]) - out[105 * k + i]));
+      m = std::fmax(m, std::fabs(double(out[105 * k + i])));
+    }
+    qdd_worst = std::fmax(qdd_worst, d / std::fmax(m, 1e-6));
   }
-  std::printf("N=%d VEL_DAMPING=%d: reference-mode vs full path max rel err %.3e, qdd mismatches %d\n", N, int(VD),
-              worst, qdd_mismatch);
+  std::printf("N=%d VEL_DAMPING=%d: reference-mode vs full path max rel err %.3e, forwardDynamics qdd max rel err %.3e\n",
+              N, int(VD), worst, qdd_worst);
   (void)hipFree(d_vol); (void)hipFree(d_dqdd); (void)hipFree(d_in); (void)hipFree(d_out);
   delete t;
-  return (worst < 1e-4 && qdd_mismatch == 0) ? 0 : 1;
+  return (worst < 1e-4 && qdd_worst < 1e-5) ? 0 : 1;
 }
 
 int main(int argc, char** argv) {

@@ -108,6 +108,34 @@ def test_small_grid_still_drains_the_queue(rbd, ctx):
     b.close()
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_alternating_small_and_large_trajectory_counts(rbd, ctx, use_graph):
+    """Batches of up to 64 trajectories find a chunk's trajectory inside the
+    wave, larger ones through the plan kernel's prefix sum; the queue cursor is
+    returned to zero by each launch, so back-to-back launches of both kinds on
+    one batch object, without synchronisation between them, all cover every
+    knot (zero-length trajectories included)."""
+    b = rbd.Batch(ctx, 256)
+    rng = np.random.default_rng(21)
+    runs = []
+    for rep, n_traj in enumerate([200, 10, 64, 65, 3, 256]):
+        lengths = rng.integers(0, 90, n_traj)
+        lengths[0] = max(lengths[0], 1)
+        xs, trajs = make_batch(rbd, lengths, seed=300 + rep)
+        b.run(trajs, use_graph=use_graph)
+        runs.append((xs, trajs, b.last_lanes, lengths))
+    rbd.hip().hipDeviceSynchronize()
+    flat = {lanes: rbd.Context(precision="fp32", lanes_per_knot=lanes) for lanes in (1, 16)}
+    for xs, trajs, lanes, lengths in runs:
+        for x, g, n in zip(xs, download(trajs), lengths):
+            if n:
+                np.testing.assert_array_equal(g, flat[lanes].run_aos(x))
+        say(f"{len(lengths)} trajectories, {int(sum(lengths))} knots, lanes {lanes} ok")
+    for c in flat.values():
+        c.close()
+    b.close()
+
+
 def test_batch_errors(rbd, ctx):
     b = rbd.Batch(ctx, 2)
     xs, trajs = make_batch(rbd, [5, 5, 5], seed=5)
