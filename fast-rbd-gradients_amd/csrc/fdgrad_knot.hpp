@@ -364,6 +364,17 @@ RBD_HD RBD_INLINE void rnea_bwd(const CS<S>* t, S f[NQ][6]) {
   });
 }
 
+// External forces (RNEA_fp's s_fext, dynamicsGradient.h:654-657): f_k -= fext_k
+// with fext_k link k's spatial force in its own frame at fx[6k .. 6k+5].  The
+// input accessor's fx is null (a compile-time constant in the plain kernels,
+// so this folds away) unless the call passes forces.
+template <class In, typename S>
+RBD_HD RBD_INLINE void subtract_fext(const In& in, S f[NQ][6]) {
+  if (in.fx)
+    for (int k = 0; k < NQ; ++k)
+      for (int i = 0; i < 6; ++i) f[k][i] -= S(in.fx[6 * k + i]);
+}
+
 // ---------------------------------------------------------------------------
 // One derivative column.  Forward sweep (compute_vafdu, dynamicsGradient.h:786-829)
 // from link c: at k == c the seed is dv = S, da = v_c x S for d/dqd, and
@@ -614,6 +625,7 @@ RBD_HD RBD_INLINE void phase_a(const In& in, const Options& opt, PhaseC<typename
   SR v[NQ][6], a[NQ][6], f[NQ][6];
   const SR zero[NQ] = {SR(0), SR(0), SR(0), SR(0), SR(0), SR(0), SR(0)};
   rnea_fwd(tr, qd, zero, SR(opt.gravity), v, a, f);
+  subtract_fext(in, f);
   rnea_bwd(tr, f);
   SR bias[NQ];
   for (int k = 0; k < NQ; ++k) bias[k] = u[k] - f[k][2] - SR(opt.damping) * qd[k];
@@ -659,6 +671,7 @@ RBD_HD RBD_INLINE void phase_a_given_qdd_minv(const In& in, const Options& opt, 
     for (int c = r; c < NQ; ++c) Mi[r][c] = SC(in.minv(r, c));
   SR v[NQ][6], a[NQ][6], f[NQ][6];
   rnea_fwd(tr, qd, qdd, SR(opt.gravity), v, a, f);
+  subtract_fext(in, f);
   rnea_bwd(tr, f);
   make_phase_c(PC, tc, qd, v, a, f, Mi);
 }
@@ -717,6 +730,7 @@ RBD_HD RBD_INLINE void fd_knot(const In& in, Out& out, const Options& opt) {
   SR v[NQ][6], a[NQ][6], f[NQ][6];
   const SR zero[NQ] = {SR(0), SR(0), SR(0), SR(0), SR(0), SR(0), SR(0)};
   rnea_fwd(tr, qd, zero, SR(opt.gravity), v, a, f);
+  subtract_fext(in, f);
   rnea_bwd(tr, f);
   SR bias[NQ], qdd[NQ];
   for (int k = 0; k < NQ; ++k) bias[k] = u[k] - f[k][2] - SR(opt.damping) * qd[k];
@@ -772,10 +786,12 @@ constexpr int kInPerKnot = 21;     // q, qd, u
 constexpr int kOutPerKnot = 105;   // qdd, dqdd/dq (col-major), dqdd/dqd
 constexpr int kVolPerKnot = 77;    // q, qd, u, qdd, Minv dense (COMPLETELY_FUSED_VALS, .cuh:1145)
 constexpr int kDqddPerKnot = 98;   // dqdd/dq, dqdd/dqd
+constexpr int kFextPerSet = 42;    // external forces: 7 links x (moment, force) in the link frame
 
 // array-of-structures: one knot's values contiguous (the reference's knot<T> order)
 struct InAoS {
   const float* p;
+  const float* fx = nullptr;   // external link forces (42 floats) or null
   RBD_HD float q(int i) const { return p[i]; }
   RBD_HD float qd(int i) const { return p[NQ + i]; }
   RBD_HD float u(int i) const { return p[2 * NQ + i]; }
@@ -812,6 +828,7 @@ struct OutDqddAoS {   // 98 per knot, no qdd
 // structure-of-arrays: value i of knot k at p[i * ld + k]
 struct InSoA {
   const float* p; long ld;
+  const float* fx = nullptr;   // external link forces (42 floats) or null
   RBD_HD float q(int i) const { return p[i * ld]; }
   RBD_HD float qd(int i) const { return p[(NQ + i) * ld]; }
   RBD_HD float u(int i) const { return p[(2 * NQ + i) * ld]; }

@@ -131,13 +131,32 @@ __global__ __launch_bounds__(kBlock) void fdgrad_minv_kernel(long n, const float
   }
 }
 
-// Forward dynamics only (qdd and dense Minv), one lane per knot.
-template <class P>
-__global__ __launch_bounds__(kBlock) void fd_kernel(long n, const float* __restrict__ in, float* __restrict__ qdd,
-                                                    float* __restrict__ minv, Options opt) {
+// Full path with external link forces: knot k reads its 42 forces at
+// fext + k * fext_stride (stride 0: one set for every knot).  Both mappings.
+template <class P, int LANES>
+__global__ __launch_bounds__(kBlock) void fdgrad_fext_kernel(long n, const float* __restrict__ in,
+                                                             const float* __restrict__ fext, long fext_stride,
+                                                             float* __restrict__ out, Options opt) {
+  const int role = threadIdx.x % LANES;
+  const long groups = long(gridDim.x) * (blockDim.x / LANES);
+  for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
+    InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
+    OutAoS o{out + kOutPerKnot * k};
+    if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
+    else fdgrad_knot_lanes<P>(i, o, opt, role);
+  }
+}
+
+// Forward dynamics only (qdd and dense Minv), one lane per knot; FEXT: with
+// external link forces as above.
+template <class P, bool FEXT>
+__global__ __launch_bounds__(kBlock) void fd_kernel(long n, const float* __restrict__ in,
+                                                    const float* __restrict__ fext, long fext_stride,
+                                                    float* __restrict__ qdd, float* __restrict__ minv, Options opt) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
     InAoS i{in + kInPerKnot * k};
+    if constexpr (FEXT) i.fx = fext + fext_stride * k;
     OutFD o{qdd + NQ * k, minv + NQ * NQ * k};
     fd_knot<P>(i, o, opt);
   }
@@ -387,8 +406,17 @@ rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n, const float* d_
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_minv_kernel launch");
 }
 
-rbd_status rbd_forward_dynamics_f32(rbd_context* ctx, int64_t n, const float* d_in, float* d_qdd, float* d_minv,
-                                    rbd_stream_t stream) {
+namespace {
+
+rbd_status check_fext(const float* d_fext, int64_t fext_stride) {
+  if (!d_fext) return fail(RBD_ERR_INVALID_ARGUMENT, "null external-force buffer");
+  if (fext_stride != 0 && fext_stride < rbd::kFextPerSet)
+    return fail(RBD_ERR_INVALID_ARGUMENT, "fext_stride must be 0 or at least 42");
+  return RBD_OK;
+}
+
+rbd_status forward_dynamics(rbd_context* ctx, int64_t n, const float* d_in, const float* d_fext,
+                            int64_t fext_stride, float* d_qdd, float* d_minv, rbd_stream_t stream) {
   if (rbd_status s = check_ctx(ctx)) return s;
   if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
   if (n == 0) return RBD_OK;
@@ -396,22 +424,67 @@ rbd_status rbd_forward_dynamics_f32(rbd_context* ctx, int64_t n, const float* d_
   if (rbd_status s = use_device(ctx)) return s;
   const dim3 grid(grid_for(n, 1)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
+  auto go = [&](auto prec_tag) {
+    using P = decltype(prec_tag);
+    if (d_fext)
+      hipLaunchKernelGGL((rbd::fd_kernel<P, true>), grid, block, 0, st, long(n), d_in, d_fext, long(fext_stride),
+                         d_qdd, d_minv, ctx->opt);
+    else
+      hipLaunchKernelGGL((rbd::fd_kernel<P, false>), grid, block, 0, st, long(n), d_in, nullptr, 0L, d_qdd, d_minv,
+                         ctx->opt);
+  };
   // forward dynamics has no derivative columns: FP64_COLUMNS computes it as FP32 does
   switch (ctx->precision) {
     case RBD_PREC_FP32:
-    case RBD_PREC_FP64_COLUMNS:
-      hipLaunchKernelGGL(rbd::fd_kernel<rbd::PrecFP32>, grid, block, 0, st, long(n), d_in, d_qdd, d_minv, ctx->opt);
-      break;
-    case RBD_PREC_FP64_PIVOT:
-      hipLaunchKernelGGL(rbd::fd_kernel<rbd::PrecFP64Pivot>, grid, block, 0, st, long(n), d_in, d_qdd, d_minv,
-                         ctx->opt);
-      break;
-    default:
-      hipLaunchKernelGGL(rbd::fd_kernel<rbd::PrecFP64>, grid, block, 0, st, long(n), d_in, d_qdd, d_minv, ctx->opt);
-      break;
+    case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP32{}); break;
+    case RBD_PREC_FP64_PIVOT: go(rbd::PrecFP64Pivot{}); break;
+    default: go(rbd::PrecFP64{}); break;
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fd_kernel launch");
+}
+
+}  // namespace
+
+rbd_status rbd_forward_dynamics_f32(rbd_context* ctx, int64_t n, const float* d_in, float* d_qdd, float* d_minv,
+                                    rbd_stream_t stream) {
+  return forward_dynamics(ctx, n, d_in, nullptr, 0, d_qdd, d_minv, stream);
+}
+
+rbd_status rbd_forward_dynamics_fext_f32(rbd_context* ctx, int64_t n, const float* d_in, const float* d_fext,
+                                         int64_t fext_stride, float* d_qdd, float* d_minv, rbd_stream_t stream) {
+  if (rbd_status s = check_fext(d_fext, fext_stride)) return s;
+  return forward_dynamics(ctx, n, d_in, d_fext, fext_stride, d_qdd, d_minv, stream);
+}
+
+rbd_status rbd_fdgrad_traj_fext_f32(rbd_context* ctx, int64_t n, const float* d_in, const float* d_fext,
+                                    int64_t fext_stride, float* d_out, rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
+  if (n == 0) return RBD_OK;
+  if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+  if (rbd_status s = check_fext(d_fext, fext_stride)) return s;
+  if (rbd_status s = use_device(ctx)) return s;
+  const int lanes = lanes_for(ctx, n);
+  const dim3 grid(grid_for(n, lanes)), block(kBlock);
+  const hipStream_t st = (hipStream_t)stream;
+  auto go = [&](auto prec_tag) {
+    using P = decltype(prec_tag);
+    if (lanes == 1)
+      hipLaunchKernelGGL((rbd::fdgrad_fext_kernel<P, 1>), grid, block, 0, st, long(n), d_in, d_fext,
+                         long(fext_stride), d_out, ctx->opt);
+    else
+      hipLaunchKernelGGL((rbd::fdgrad_fext_kernel<P, rbd::kLanesPerKnotGroup>), grid, block, 0, st, long(n), d_in,
+                         d_fext, long(fext_stride), d_out, ctx->opt);
+  };
+  switch (ctx->precision) {
+    case RBD_PREC_FP32: go(rbd::PrecFP32{}); break;
+    case RBD_PREC_FP64_PIVOT: go(rbd::PrecFP64Pivot{}); break;
+    case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP64Col{}); break;
+    default: go(rbd::PrecFP64{}); break;
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_fext_kernel launch");
 }
 
 int64_t rbd_group_threshold(rbd_context* ctx) { return ctx ? ctx->group_threshold.load() : -1; }

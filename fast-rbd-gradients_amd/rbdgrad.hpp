@@ -91,14 +91,30 @@ inline rbd_status forwardDynamicsGradient(T* d_out, const T* d_in, hipStream_t s
   return rbd_fdgrad_traj_f32(context<MPC_MODE, VEL_DAMPING>(), KNOT_POINTS, d_in, d_out, stream);
 }
 
+// The same with external link forces (the reference's s_fext argument,
+// dynamicsGradient.h:916-934): d_fext holds 42 floats per set (link k's
+// [moment | force] in its own frame at 6k), one set for every knot
+// (fext_stride = 0) or one per knot (fext_stride = FEXT_VALS).
+constexpr int FEXT_VALS = 6 * NUM_POS;
+template <typename T, int KNOT_POINTS, bool MPC_MODE = false, bool VEL_DAMPING = true>
+inline rbd_status forwardDynamicsGradient(T* d_out, const T* d_in, const T* d_fext, int64_t fext_stride = 0,
+                                          hipStream_t stream = 0) {
+  static_assert(std::is_same<T, float>::value, "librbdgrad provides the FP32 interface");
+  return rbd_fdgrad_traj_fext_f32(context<MPC_MODE, VEL_DAMPING>(), KNOT_POINTS, d_in, d_fext, fext_stride, d_out,
+                                  stream);
+}
+
 // forwardDynamics for every knot of a host trajectory, computed on the GPU:
 // fills knot.qdd and knot.Minv (dense, DENSE_MINV = 1) from q, qd, u.  Blocking.
+// h_fext (host, FEXT_VALS floats, may be null): external link forces applied
+// at every knot (forwardDynamics(knot, s_fext), dynamicsGradient.h:936-939).
 template <typename T, int KNOT_POINTS, bool MPC_MODE = false, bool VEL_DAMPING = true>
-inline rbd_status forwardDynamics(traj<T, KNOT_POINTS>* h_traj) {
+inline rbd_status forwardDynamics(traj<T, KNOT_POINTS>* h_traj, const T* h_fext = nullptr) {
   static_assert(std::is_same<T, float>::value, "librbdgrad provides the FP32 interface");
   constexpr size_t n = KNOT_POINTS;
   float* d = nullptr;
-  if (hipMalloc(&d, n * (21 + 7 + 49) * sizeof(float)) != hipSuccess) return RBD_ERR_OUT_OF_MEMORY;
+  if (hipMalloc(&d, (n * (21 + 7 + 49) + FEXT_VALS) * sizeof(float)) != hipSuccess) return RBD_ERR_OUT_OF_MEMORY;
+  float* d_fext = d + n * (21 + 7 + 49);
   float* h = nullptr;
   if (hipHostMalloc(reinterpret_cast<void**>(&h), n * (21 + 7 + 49) * sizeof(float), hipHostMallocDefault) !=
       hipSuccess) {
@@ -113,8 +129,13 @@ inline rbd_status forwardDynamics(traj<T, KNOT_POINTS>* h_traj) {
     }
   rbd_status st = RBD_OK;
   if (hipMemcpy(d, h, n * 21 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) st = RBD_ERR_HIP;
+  if (st == RBD_OK && h_fext &&
+      hipMemcpy(d_fext, h_fext, FEXT_VALS * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    st = RBD_ERR_HIP;
   if (st == RBD_OK)
-    st = rbd_forward_dynamics_f32(context<MPC_MODE, VEL_DAMPING>(), int64_t(n), d, d + 21 * n, d + 28 * n, 0);
+    st = h_fext ? rbd_forward_dynamics_fext_f32(context<MPC_MODE, VEL_DAMPING>(), int64_t(n), d, d_fext, 0,
+                                                d + 21 * n, d + 28 * n, 0)
+                : rbd_forward_dynamics_f32(context<MPC_MODE, VEL_DAMPING>(), int64_t(n), d, d + 21 * n, d + 28 * n, 0);
   if (st == RBD_OK &&
       hipMemcpy(h + 21 * n, d + 21 * n, n * 56 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
     st = RBD_ERR_HIP;

@@ -40,6 +40,7 @@ EXPORTED = [
     "rbd_server_config_default", "rbd_server_create", "rbd_server_destroy", "rbd_server_submit",
     "rbd_server_poll", "rbd_server_wait", "rbd_server_get_stats", "rbd_server_reset_stats",
     "rbd_server_run_poisson", "rbd_fdgrad_minv_traj_f32", "rbd_forward_dynamics_f32",
+    "rbd_fdgrad_traj_fext_f32", "rbd_forward_dynamics_fext_f32",
 ]
 
 
@@ -104,6 +105,8 @@ def lib():
         L.rbd_fdgrad_traj_host_f32.argtypes = [vp, i64, vp, vp]
         L.rbd_fdgrad_minv_traj_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_forward_dynamics_f32.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.rbd_fdgrad_traj_fext_f32.argtypes = [vp, i64, vp, vp, i64, vp, vp]
+        L.rbd_forward_dynamics_fext_f32.argtypes = [vp, i64, vp, vp, i64, vp, vp, vp]
         L.rbd_kernel_name.argtypes = [vp, ctypes.c_int, i64]
         L.rbd_kernel_name.restype = ctypes.c_char_p
         L.rbd_autotune.argtypes = [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_float), ctypes.c_int]
@@ -375,16 +378,33 @@ class Context:
                "rbd_fdgrad_minv_traj_f32")
         return d_out.download((n, OUT_PER_KNOT)), d_m.download((n, 7, 7)).transpose(0, 2, 1).copy()
 
-    def forward_dynamics(self, x):
-        """qdd (n, 7) and Minv (n, 7, 7) [knot][row][col] from x (n, 21)."""
+    def forward_dynamics(self, x, fext=None):
+        """qdd (n, 7) and Minv (n, 7, 7) [knot][row][col] from x (n, 21);
+        fext: external link forces, (42,) for every knot or (n, 42)."""
         x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
         n = x.shape[0]
         d_in = DeviceBuffer.from_array(x)
         d_q = DeviceBuffer(n * 7 * 4)
         d_m = DeviceBuffer(n * 49 * 4)
-        _check(lib().rbd_forward_dynamics_f32(self._h, n, d_in.handle, d_q.handle, d_m.handle, None),
-               "rbd_forward_dynamics_f32")
+        if fext is None:
+            _check(lib().rbd_forward_dynamics_f32(self._h, n, d_in.handle, d_q.handle, d_m.handle, None),
+                   "rbd_forward_dynamics_f32")
+        else:
+            d_f, stride = _fext_buffer(fext, n)
+            _check(lib().rbd_forward_dynamics_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_q.handle,
+                                                       d_m.handle, None), "rbd_forward_dynamics_fext_f32")
         return d_q.download((n, 7)), d_m.download((n, 7, 7)).transpose(0, 2, 1).copy()
+
+    def run_fext(self, x, fext):
+        """Full path with external link forces: x (n, 21), fext (42,) or (n, 42) -> (n, 105)."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+        n = x.shape[0]
+        d_in = DeviceBuffer.from_array(x)
+        d_out = DeviceBuffer(n * OUT_PER_KNOT * 4)
+        d_f, stride = _fext_buffer(fext, n)
+        _check(lib().rbd_fdgrad_traj_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_out.handle, None),
+               "rbd_fdgrad_traj_fext_f32")
+        return d_out.download((n, OUT_PER_KNOT))
 
     def run_refmode(self, vol):
         vol = np.ascontiguousarray(np.atleast_2d(vol), dtype=np.float32)
@@ -393,6 +413,21 @@ class Context:
         d_out = DeviceBuffer(n * DQDD_PER_KNOT * 4)
         self.dynamics_gradient_kernel(n, d_out, d_vol)
         return d_out.download((n, DQDD_PER_KNOT))
+
+
+FEXT_PER_SET = 42
+
+
+def _fext_buffer(fext, n):
+    """Device copy of external forces and its stride (0: one set for all knots)."""
+    f = np.ascontiguousarray(fext, dtype=np.float32)
+    if f.ndim == 1:
+        if f.shape != (FEXT_PER_SET,):
+            raise ValueError("fext must have 42 values (7 links x 6)")
+        return DeviceBuffer.from_array(f), 0
+    if f.shape != (n, FEXT_PER_SET):
+        raise ValueError("per-knot fext must have shape (n, 42)")
+    return DeviceBuffer.from_array(f), FEXT_PER_SET
 
 
 class Batch:

@@ -141,6 +141,22 @@ rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n_knots, const flo
 rbd_status rbd_forward_dynamics_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, float* d_qdd,
                                     float* d_minv, rbd_stream_t stream);
 
+/* External forces.  d_fext holds, for each of the 7 links, the spatial force
+ * the environment applies to it, in the link's own frame: 6 floats
+ * [moment(3) | force(3)] at d_fext[6k .. 6k+5], 42 floats per set.  Knot j
+ * uses the set at d_fext + j * fext_stride: fext_stride = 0 applies one set to
+ * every knot (the reference's per-call s_fext), fext_stride >= 42 one set per
+ * knot.  The forces enter the bias, qdd and the gradient (they are constant in
+ * the link frames, so they move with the links).  Device memory, valid until
+ * the call completes on `stream`.
+ * Replaces: forwardDynamicsGradient(dqdd, knot, T, I, s_fext) and
+ * forwardDynamics(knot, s_fext) (dynamicsGradient.h:916-939), whose RNEA_fp
+ * subtracts s_fext from each link's force (:654-657). */
+rbd_status rbd_fdgrad_traj_fext_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, const float* d_fext,
+                                    int64_t fext_stride, float* d_out, rbd_stream_t stream);
+rbd_status rbd_forward_dynamics_fext_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, const float* d_fext,
+                                         int64_t fext_stride, float* d_qdd, float* d_minv, rbd_stream_t stream);
+
 /* Name of the kernel the context dispatches a call of n_knots to (full path,
  * or the reference-compatible mode when reference_mode != 0), as profilers list
  * it (template arguments abbreviated).  No reference counterpart. */

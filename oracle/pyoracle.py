@@ -44,6 +44,10 @@ def lib():
         L.oracle_fdgrad_refmode_f64.argtypes = [ctypes.c_long, dp, dp, ctypes.c_int]
         L.oracle_fdgrad_refmode_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int]
         L.oracle_fd_complex_step.argtypes = [dp, ctypes.c_double, dp, ctypes.c_int]
+        L.oracle_fd_complex_step_fext.argtypes = [dp, dp, ctypes.c_double, dp, ctypes.c_int]
+        L.oracle_fdgrad_fext_f64.argtypes = [ctypes.c_long, dp, dp, ctypes.c_long, dp, ctypes.c_int]
+        L.oracle_rnea_fext.argtypes = [dp, dp, dp, dp, dp, ctypes.c_int]
+        L.oracle_fd_fext.argtypes = [dp, dp, dp, dp, dp, dp, ctypes.c_int]
         L.oracle_inertias.argtypes = [dp]
         L.oracle_transforms.argtypes = [dp, dp]
         L.oracle_rnea.argtypes = [dp, dp, dp, dp, ctypes.c_int]
@@ -72,10 +76,30 @@ def _c64(a, shape_last):
     return a
 
 
-def fdgrad(x, flags=0, threads=1):
-    """FP64 full path. x: (n, 21) [q|qd|u] -> (n, 105) [qdd|dqdd/dq|dqdd/dqd]."""
+FEXT_PER_SET = 42   # 7 links x spatial force (moment, force) in the link frame
+
+
+def _fext(fext, n):
+    """fext: None, (42,) for every knot (stride 0) or (n, 42) per knot (stride 42)."""
+    if fext is None:
+        return None, 0
+    f = np.ascontiguousarray(fext, dtype=np.float64)
+    if f.ndim == 1:
+        assert f.shape == (FEXT_PER_SET,)
+        return f, 0
+    assert f.shape == (n, FEXT_PER_SET)
+    return f, FEXT_PER_SET
+
+
+def fdgrad(x, flags=0, threads=1, fext=None):
+    """FP64 full path. x: (n, 21) [q|qd|u] -> (n, 105) [qdd|dqdd/dq|dqdd/dqd].
+    fext: external link forces, (42,) for all knots or (n, 42)."""
     x = _c64(np.atleast_2d(x), IN_PER_KNOT)
     out = np.empty((x.shape[0], OUT_PER_KNOT))
+    if fext is not None:
+        f, stride = _fext(fext, x.shape[0])
+        lib().oracle_fdgrad_fext_f64(x.shape[0], _dp(x), _dp(f), stride, _dp(out), flags)
+        return out
     if threads > 1:
         lib().oracle_fdgrad_f64_mt(threads, x.shape[0], _dp(x), _dp(out), flags)
     else:
@@ -99,11 +123,15 @@ def fdgrad_refmode(vol, flags=0):
     return out
 
 
-def fd_complex_step(x, h=1e-20, flags=0):
+def fd_complex_step(x, h=1e-20, flags=0, fext=None):
     """[qdd | dqdd/dq | dqdd/dqd] of one knot by complex-step differentiation of FD."""
     x = _c64(x, IN_PER_KNOT)
     out = np.empty(OUT_PER_KNOT)
-    lib().oracle_fd_complex_step(_dp(x), h, _dp(out), flags)
+    if fext is None:
+        lib().oracle_fd_complex_step(_dp(x), h, _dp(out), flags)
+    else:
+        f = _c64(fext, FEXT_PER_SET)
+        lib().oracle_fd_complex_step_fext(_dp(x), _dp(f), h, _dp(out), flags)
     return out
 
 
@@ -120,19 +148,27 @@ def transforms(q):
     return out.reshape(7, 6, 6).transpose(0, 2, 1)
 
 
-def rnea(q, qd, qdd, flags=0):
+def rnea(q, qd, qdd, flags=0, fext=None):
     q, qd, qdd = (_c64(a, NQ) for a in (q, qd, qdd))
     tau = np.empty(NQ)
-    lib().oracle_rnea(_dp(q), _dp(qd), _dp(qdd), _dp(tau), flags)
+    if fext is None:
+        lib().oracle_rnea(_dp(q), _dp(qd), _dp(qdd), _dp(tau), flags)
+    else:
+        f = _c64(fext, FEXT_PER_SET)
+        lib().oracle_rnea_fext(_dp(q), _dp(qd), _dp(qdd), _dp(f), _dp(tau), flags)
     return tau
 
 
-def fd(q, qd, u, flags=0):
+def fd(q, qd, u, flags=0, fext=None):
     """Returns (qdd, Minv dense [row][col])."""
     q, qd, u = (_c64(a, NQ) for a in (q, qd, u))
     qdd = np.empty(NQ)
     minv = np.empty(NQ * NQ)
-    lib().oracle_fd(_dp(q), _dp(qd), _dp(u), _dp(qdd), _dp(minv), flags)
+    if fext is None:
+        lib().oracle_fd(_dp(q), _dp(qd), _dp(u), _dp(qdd), _dp(minv), flags)
+    else:
+        f = _c64(fext, FEXT_PER_SET)
+        lib().oracle_fd_fext(_dp(q), _dp(qd), _dp(u), _dp(f), _dp(qdd), _dp(minv), flags)
     return qdd, minv.reshape(NQ, NQ).T
 
 

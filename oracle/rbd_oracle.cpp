@@ -63,6 +63,27 @@ void oracle_fd(const double* q, const double* qd, const double* u, double* qdd, 
   fd_knot<double>(q, qd, u, qdd, Minv, flags);
 }
 
+// The same with external forces fext (42 values: link k's spatial force in its
+// own frame at fext[6k .. 6k+5]; may be null).
+void oracle_rnea_fext(const double* q, const double* qd, const double* qdd, const double* fext, double* tau,
+                      int flags) {
+  rnea_knot<double>(q, qd, qdd, tau, flags, fext);
+}
+void oracle_fd_fext(const double* q, const double* qd, const double* u, const double* fext, double* qdd,
+                    double* Minv, int flags) {
+  fd_knot<double>(q, qd, u, qdd, Minv, flags, fext);
+}
+
+// Full path with external forces: knot k uses fext + k * fext_stride (stride 0:
+// one set of 42 for every knot, as the reference's per-call s_fext).
+void oracle_fdgrad_fext_f64(long n, const double* in, const double* fext, long fext_stride, double* out,
+                            int flags) {
+  for (long k = 0; k < n; ++k) {
+    const double* x = in + 21 * k;
+    fdgrad_knot<double>(x, x + 7, x + 14, out + 105 * k, flags, nullptr, fext + k * fext_stride);
+  }
+}
+
 // The articulated-inertia pivots D_k = S^T IA_k S of the Minv recursion.
 void oracle_pivots(const double* q, double* D) {
   Knot<double> K;
@@ -89,17 +110,20 @@ void oracle_fdgrad_refmode_f32(long n, const float* vol, float* dqdd, int flags)
 // Complex-step derivative of forward dynamics: for one knot, out = [qdd | dqdd/dq | dqdd/dqd]
 // with each derivative column Im(FD(x + i h e_j)) / h -- no subtractive cancellation,
 // so h = 1e-20 gives derivatives to machine precision.  Pins the analytic gradient.
-void oracle_fd_complex_step(const double* in, double h, double* out, int flags) {
+void oracle_fd_complex_step_fext(const double* in, const double* fext, double h, double* out, int flags) {
   using C = std::complex<double>;
   C q[NQ], qd[NQ], u[NQ], qdd[NQ];
   for (int j = 0; j < 2 * NQ + 1; ++j) {
     for (int i = 0; i < NQ; ++i) { q[i] = in[i]; qd[i] = in[7 + i]; u[i] = in[14 + i]; }
     if (j < NQ) q[j] += C(0, h);
     else if (j < 2 * NQ) qd[j - NQ] += C(0, h);
-    fd_knot<C>(q, qd, u, qdd, nullptr, flags);
+    fd_knot<C>(q, qd, u, qdd, nullptr, flags, fext);
     if (j == 2 * NQ) for (int i = 0; i < NQ; ++i) out[i] = qdd[i].real();
     else for (int i = 0; i < NQ; ++i) out[NQ + NQ * j + i] = qdd[i].imag() / h;
   }
+}
+void oracle_fd_complex_step(const double* in, double h, double* out, int flags) {
+  oracle_fd_complex_step_fext(in, nullptr, h, out, flags);
 }
 
 // Operation count of one knot point of the FP32 method.

@@ -281,8 +281,10 @@ void setup_model(Knot<T>& K, const T* q) {
 
 // RNEA forward pass (RNEA_fp, dynamicsGradient.h:619-659).  qdd may be null
 // (treated as zero -- the reference reads it unconditionally, :634,:647).
+// fext (may be null): external spatial force on each link in its own frame,
+// fext[6k + i], subtracted from f_k (dynamicsGradient.h:654-657).
 template <typename T>
-void rnea_forward(Knot<T>& K, const T* qd, const T* qdd, int flags) {
+void rnea_forward(Knot<T>& K, const T* qd, const T* qdd, int flags, const double* fext = nullptr) {
   for (int k = 0; k < NQ; ++k) {
     T* v = K.v[k]; T* a = K.a[k]; const T* X = K.X[k];
     if (k == 0) {
@@ -302,6 +304,8 @@ void rnea_forward(Knot<T>& K, const T* qd, const T* qdd, int flags) {
     mat6_mv<T, false>(Iv, K.I[k], v);
     crf_mv(K.f[k], v, Iv);
     mat6_mv<T, true>(K.f[k], K.I[k], a);
+    if (fext)
+      for (int i = 0; i < 6; ++i) K.f[k][i] = K.f[k][i] - T(fext[6 * k + i]);
   }
 }
 
@@ -522,11 +526,12 @@ void inverse_dynamics_gradient(const Knot<T>& K, const T* qd, T* cdq, T* cdqd, i
 // except that the bias RNEA uses qdd = 0 (the reference reads a stale knot.qdd).
 // out: [qdd(7) | dqdd/dq (49, col-major) | dqdd/dqd (49)].
 template <typename T>
-void fdgrad_knot(const T* q, const T* qd, const T* u, T* out, int flags, Knot<T>* keep = nullptr) {
+void fdgrad_knot(const T* q, const T* qd, const T* u, T* out, int flags, Knot<T>* keep = nullptr,
+                 const double* fext = nullptr) {
   Knot<T> local;
   Knot<T>& K = keep ? *keep : local;
   setup_model(K, q);
-  rnea_forward(K, qd, (const T*)nullptr, flags);
+  rnea_forward(K, qd, (const T*)nullptr, flags, fext);
   T cbias[NQ];
   rnea_backward(K, cbias);
   T bias[NQ];
@@ -537,7 +542,7 @@ void fdgrad_knot(const T* q, const T* qd, const T* u, T* out, int flags, Knot<T>
   compute_minv(K);
   T* qdd = out;
   symv_upper<false>(qdd, K.Minv, bias);
-  rnea_forward(K, qd, qdd, flags);
+  rnea_forward(K, qd, qdd, flags, fext);
   rnea_backward(K, (T*)nullptr);
   T cdu[2 * NQ * NQ];
   inverse_dynamics_gradient(K, qd, cdu, cdu + NQ * NQ, flags);
@@ -548,11 +553,11 @@ void fdgrad_knot(const T* q, const T* qd, const T* u, T* out, int flags, Knot<T>
 // configuration; time_GPU.cu:141-158 COMPLETELY_FUSED layout): per knot
 // vol = [q | qd | u | qdd | Minv dense 7x7], out = [dqdd/dq | dqdd/dqd].
 template <typename T>
-void fdgrad_knot_given_qdd_minv(const T* vol, T* out, int flags) {
+void fdgrad_knot_given_qdd_minv(const T* vol, T* out, int flags, const double* fext = nullptr) {
   Knot<T> K;
   const T* q = vol; const T* qd = vol + NQ; const T* qdd = vol + 3 * NQ; const T* Md = vol + 4 * NQ;
   setup_model(K, q);
-  rnea_forward(K, qd, qdd, flags);
+  rnea_forward(K, qd, qdd, flags, fext);
   rnea_backward(K, (T*)nullptr);
   T cdu[2 * NQ * NQ];
   inverse_dynamics_gradient(K, qd, cdu, cdu + NQ * NQ, flags);
@@ -562,10 +567,10 @@ void fdgrad_knot_given_qdd_minv(const T* vol, T* out, int flags) {
 
 // Forward dynamics only: qdd and dense Minv (forwardDynamics, dynamicsGradient.h:685-706).
 template <typename T>
-void fd_knot(const T* q, const T* qd, const T* u, T* qdd, T* Minv_dense, int flags) {
+void fd_knot(const T* q, const T* qd, const T* u, T* qdd, T* Minv_dense, int flags, const double* fext = nullptr) {
   Knot<T> K;
   setup_model(K, q);
-  rnea_forward(K, qd, (const T*)nullptr, flags);
+  rnea_forward(K, qd, (const T*)nullptr, flags, fext);
   T cbias[NQ];
   rnea_backward(K, cbias);
   T bias[NQ];
@@ -582,10 +587,10 @@ void fd_knot(const T* q, const T* qd, const T* u, T* qdd, T* Minv_dense, int fla
 
 // Inverse dynamics tau = RNEA(q, qd, qdd) (+0.5 qd with damping).
 template <typename T>
-void rnea_knot(const T* q, const T* qd, const T* qdd, T* tau, int flags) {
+void rnea_knot(const T* q, const T* qd, const T* qdd, T* tau, int flags, const double* fext = nullptr) {
   Knot<T> K;
   setup_model(K, q);
-  rnea_forward(K, qd, qdd, flags);
+  rnea_forward(K, qd, qdd, flags, fext);
   rnea_backward(K, tau);
   if (flags & kVelDamping)
     for (int k = 0; k < NQ; ++k) tau[k] = tau[k] + T(0.5) * qd[k];

@@ -136,6 +136,15 @@ class UrdfModel:
                 fs[k - 1] = fs[k - 1] + Xs[k].T @ fs[k]
         return tau
 
+    def link_velocities(self, q, qd):
+        """Spatial velocity [omega; v] of each link in its own frame, base at rest."""
+        S = np.array([0, 0, 1.0, 0, 0, 0])
+        v, out = np.zeros(6), []
+        for k in range(NQ):
+            v = self.transform(k, q[k]) @ v + S * qd[k]
+            out.append(v)
+        return np.array(out)
+
     def com_heights(self, q):
         """World z of each link's COM (base frame z up)."""
         R = np.eye(3)

@@ -7,7 +7,10 @@
 //   2. forwardDynamics' qdd equals the full path's qdd to FP32 rounding (the
 //      two are separate kernels, so the compiler may contract different
 //      products into FMAs);
-//   3. the TEST_FOR_EQUIVALENCE inputs (all 0.1, time_GPU.cu:124-127) match
+//   3. with external link forces, forwardDynamics(traj, fext) and
+//      forwardDynamicsGradient(out, in, fext) agree on qdd, and the forces
+//      change it;
+//   4. the TEST_FOR_EQUIVALENCE inputs (all 0.1, time_GPU.cu:124-127) match
 //      the values the FP64 oracle gives for them (passed on the command line).
 // Exit code 0 on success; prints what it checked.
 #include <cmath>
@@ -74,12 +77,63 @@ int run(std::mt19937& rng) {
   return (worst < 1e-4 && qdd_worst < 1e-5) ? 0 : 1;
 }
 
+int run_fext(std::mt19937& rng) {
+  constexpr int N = 64;
+  std::uniform_real_distribution<float> uq(-3.14159265f, 3.14159265f), uqd(-2.f, 2.f), uu(-50.f, 50.f),
+      uf(-20.f, 20.f);
+  auto* t = new traj<float, N>;
+  for (int k = 0; k < N; ++k)
+    for (int i = 0; i < NUM_POS; ++i) {
+      t->knots[k].q[i] = uq(rng);
+      t->knots[k].qd[i] = uqd(rng);
+      t->knots[k].u[i] = uu(rng);
+    }
+  float fext[FEXT_VALS];
+  for (float& f : fext) f = uf(rng);
+  std::vector<float> in(21 * size_t(N));
+  for (int k = 0; k < N; ++k)
+    for (int i = 0; i < NUM_POS; ++i) {
+      in[21 * k + i] = t->knots[k].q[i];
+      in[21 * k + 7 + i] = t->knots[k].qd[i];
+      in[21 * k + 14 + i] = t->knots[k].u[i];
+    }
+  rbdErrchk((forwardDynamics<float, N>(t, fext)));
+  float *d_in, *d_out, *d_f;
+  if (hipMalloc(&d_in, in.size() * 4) || hipMalloc(&d_out, size_t(N) * 105 * 4) || hipMalloc(&d_f, sizeof(fext))) {
+    std::fprintf(stderr, "hipMalloc failed\n");
+    return 1;
+  }
+  (void)hipMemcpy(d_in, in.data(), in.size() * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(d_f, fext, sizeof(fext), hipMemcpyHostToDevice);
+  std::vector<float> with(size_t(N) * 105), without(size_t(N) * 105);
+  rbdErrchk((forwardDynamicsGradient<float, N>(d_out, d_in, d_f, 0)));
+  (void)hipMemcpy(with.data(), d_out, with.size() * 4, hipMemcpyDeviceToHost);
+  rbdErrchk((forwardDynamicsGradient<float, N>(d_out, d_in)));
+  (void)hipMemcpy(without.data(), d_out, without.size() * 4, hipMemcpyDeviceToHost);
+  double worst = 0, moved = 0;
+  for (int k = 0; k < N; ++k) {
+    double d = 0, m = 0;
+    for (int i = 0; i < NUM_POS; ++i) {
+      d = std::fmax(d, std::fabs(double(t->knots[k].qdd[i]) - with[105 * k + i]));
+      m = std::fmax(m, std::fabs(double(with[105 * k + i])));
+      moved = std::fmax(moved, std::fabs(double(with[105 * k + i]) - without[105 * k + i]));
+    }
+    worst = std::fmax(worst, d / std::fmax(m, 1e-6));
+  }
+  std::printf("external forces: forwardDynamics vs forwardDynamicsGradient qdd max rel err %.3e, "
+              "change from the forces %.3e\n", worst, moved);
+  (void)hipFree(d_in); (void)hipFree(d_out); (void)hipFree(d_f);
+  delete t;
+  return (worst < 1e-5 && moved > 1e-2) ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
   std::mt19937 rng(1234);
   int bad = 0;
   bad += run<16, false>(rng);
   bad += run<128, false>(rng);
   bad += run<128, true>(rng);
+  bad += run_fext(rng);
   // TEST_FOR_EQUIVALENCE inputs against expected values given on the command line:
   // argv[1..] = 105 floats (qdd, dqdd/dq, dqdd/dqd) from the FP64 oracle
   if (argc == 106) {

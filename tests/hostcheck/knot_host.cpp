@@ -101,6 +101,30 @@ void host_fdgrad_minv_f32(long n, const float* in, float* out, float* minv, int 
   }
 }
 
+// full path and forward dynamics with external link forces (knot k uses
+// fext + k * fext_stride), FP32, one-lane (lanes = 0) or lane-group emulation
+void host_fdgrad_fext_f32(long n, const float* in, const float* fext, long fext_stride, float* out, int flags,
+                          int lanes) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
+    OutAoS o{out + kOutPerKnot * k};
+    if (lanes)
+      for (int role = 0; role < kLanesPerKnotGroup; ++role) fdgrad_knot_lanes<PrecFP32>(i, o, opt, role);
+    else
+      fdgrad_knot<PrecFP32>(i, o, opt);
+  }
+}
+void host_fd_fext_f32(long n, const float* in, const float* fext, long fext_stride, float* qdd, float* minv,
+                      int flags) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
+    OutFD o{qdd + NQ * k, minv + NQ * NQ * k};
+    fd_knot<PrecFP32>(i, o, opt);
+  }
+}
+
 void host_fdgrad_f32(long n, const float* in, float* out, int flags, int prec, int lanes) {
   by_precision(prec, false, flags, n, in, out, lanes);
 }

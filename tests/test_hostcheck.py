@@ -26,6 +26,8 @@ def host():
     L.host_fd_f32.argtypes = [ctypes.c_long, fp, fp, fp, ctypes.c_int]
     L.host_fdgrad_minv_f32.argtypes = [ctypes.c_long, fp, fp, fp, ctypes.c_int, ctypes.c_int]
     L.host_model_transform.argtypes = [ctypes.c_int, ctypes.c_double, dp]
+    L.host_fdgrad_fext_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_long, fp, ctypes.c_int, ctypes.c_int]
+    L.host_fd_fext_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_long, fp, fp, ctypes.c_int]
     return L
 
 
@@ -128,3 +130,33 @@ def test_forward_dynamics_and_minv_outputs(host, oracle, flags):
                                  x[i, 14:].astype(np.float64), flags)
         np.testing.assert_allclose(minv[i].reshape(7, 7).T, m_ref, rtol=0, atol=1e-5 * np.abs(m_ref).max())
         np.testing.assert_allclose(qdd[i], q_ref, rtol=0, atol=1e-5 * np.abs(q_ref).max())
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2])
+@pytest.mark.parametrize("per_knot", [False, True])
+def test_external_forces(host, oracle, flags, per_knot):
+    n = 120
+    x = np.ascontiguousarray(common.knots(n, seed=260 + flags, near_singular_frac=0.1))
+    rng = np.random.default_rng(261)
+    fext = rng.uniform(-20, 20, (n, 42) if per_knot else 42).astype(np.float32)
+    stride = 42 if per_knot else 0
+    fp = ctypes.POINTER(ctypes.c_float)
+    ref = oracle.fdgrad(x.astype(np.float64), flags, fext=fext.astype(np.float64))
+    for lanes in (0, 1):
+        out = np.full((n, 105), np.nan, np.float32)
+        host.host_fdgrad_fext_f32(n, x.ctypes.data_as(fp), fext.ctypes.data_as(fp), stride, out.ctypes.data_as(fp),
+                                  flags, lanes)
+        err = common.max_block_errors(out, ref)
+        assert max(err.values()) < common.REL_TOL, (lanes, err)
+    qdd = np.zeros((n, 7), np.float32)
+    minv = np.zeros((n, 49), np.float32)
+    host.host_fd_fext_f32(n, x.ctypes.data_as(fp), fext.ctypes.data_as(fp), stride, qdd.ctypes.data_as(fp),
+                          minv.ctypes.data_as(fp), flags)
+    for i in range(0, n, 13):
+        f = fext[i] if per_knot else fext
+        q_ref, _ = oracle.fd(*(x[i, s].astype(np.float64) for s in (slice(0, 7), slice(7, 14), slice(14, 21))),
+                             flags, fext=f.astype(np.float64))
+        np.testing.assert_allclose(qdd[i], q_ref, rtol=0, atol=1e-5 * np.abs(q_ref).max())
+    # the forces matter: the result without them is far off
+    plain = run(host, x, flags)
+    assert np.abs(plain - out).max() > 1e-2

@@ -169,6 +169,53 @@ def test_multithreaded_oracle_matches_serial(oracle):
     np.testing.assert_array_equal(oracle.fdgrad(x, threads=4), oracle.fdgrad(x))
 
 
+@pytest.mark.parametrize("flags", [0, 2])
+def test_external_forces_do_virtual_power(oracle, model, flags):
+    """A force fext_k on link k (its own frame) changes the joint torques by
+    -J^T fext: the power it delivers, sum_k fext_k . v_k, equals -tau_ext . qd
+    for every qd; with qd = e_j that gives tau_ext[j] = -sum_k fext_k . v_k(e_j),
+    with the link velocities v_k from the textbook model."""
+    rng = np.random.default_rng(41)
+    for _ in range(5):
+        q, qd, qdd = rng.uniform(-3, 3, 7), rng.uniform(-2, 2, 7), rng.uniform(-5, 5, 7)
+        fext = rng.uniform(-20, 20, 42)
+        d_tau = oracle.rnea(q, qd, qdd, flags, fext=fext) - oracle.rnea(q, qd, qdd, flags)
+        want = np.array([-sum(fext[6 * k:6 * k + 6] @ v for k, v in enumerate(model.link_velocities(q, e)))
+                         for e in np.eye(7)])
+        np.testing.assert_allclose(d_tau, want, rtol=0, atol=1e-11 * np.abs(want).max())
+
+
+def test_forward_dynamics_with_external_forces_inverts_rnea(oracle):
+    rng = np.random.default_rng(42)
+    for flags in (0, 1, 2):
+        q, qd, u = rng.uniform(-3, 3, 7), rng.uniform(-2, 2, 7), rng.uniform(-50, 50, 7)
+        fext = rng.uniform(-20, 20, 42)
+        qdd, _ = oracle.fd(q, qd, u, flags, fext=fext)
+        tau = oracle.rnea(q, qd, qdd, flags, fext=fext)
+        np.testing.assert_allclose(tau, u, rtol=0, atol=1e-10 * np.abs(u).max())
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2])
+def test_external_force_gradient_matches_complex_step(oracle, flags):
+    x = common.knots(12, seed=43, near_singular_frac=0.25).astype(np.float64)
+    fext = np.random.default_rng(44).uniform(-20, 20, 42)
+    out = oracle.fdgrad(x, flags, fext=fext)
+    for i in range(len(x)):
+        cs = oracle.fd_complex_step(x[i], flags=flags, fext=fext)
+        err = common.max_block_errors(out[i], cs)
+        assert max(err.values()) < 1e-11, err
+    # fext changes the result (the test above is not vacuous)
+    assert np.abs(out - oracle.fdgrad(x, flags)).max() > 1e-3
+
+
+def test_per_knot_external_forces(oracle):
+    x = common.knots(6, seed=45).astype(np.float64)
+    fext = np.random.default_rng(46).uniform(-20, 20, (6, 42))
+    out = oracle.fdgrad(x, fext=fext)
+    for i in range(6):
+        np.testing.assert_array_equal(out[i], oracle.fdgrad(x[i:i + 1], fext=fext[i])[0])
+
+
 def test_op_count_is_frozen(oracle):
     """The algorithmic FLOP count per knot used for the roofline (bench.py)."""
     c = oracle.op_count(0)
