@@ -545,8 +545,9 @@ RBD_HD RBD_INLINE void make_phase_c(PhaseC<SC>& P, const CS<SC>* t, const SR* qd
 }
 
 // All derivative columns in one lane (one lane per knot) and the stores of
-// dqdd/dq, dqdd/dqd.
-template <typename SC, class Out>
+// dqdd/dq, dqdd/dqd.  MINV = false stores the inverse-dynamics gradient
+// dtau/dq, dtau/dqd itself (the reference's cdu) instead of -Minv times it.
+template <bool MINV = true, typename SC, class Out>
 RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out& out) {
   // d/dq column 0 is identically zero (rotation about the gravity axis)
   for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
@@ -555,11 +556,13 @@ RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out&
     SC tau[NQ], col[NQ];
     dtau_column<false>(P, c, tau);
     add_on_diagonal(tau, c, SC(damping));
-    minv_times<true>(P.Mi, tau, col);
+    if constexpr (MINV) minv_times<true>(P.Mi, tau, col);
+    else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
     for (int r = 0; r < NQ; ++r) out.dqd(r, c, float(col[r]));
     if (c > 0) {
       dtau_column<true>(P, c, tau);
-      minv_times<true>(P.Mi, tau, col);
+      if constexpr (MINV) minv_times<true>(P.Mi, tau, col);
+      else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
       for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
     }
   }
@@ -571,15 +574,17 @@ RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out&
 constexpr int kLanesPerKnotGroup = 16;
 constexpr int kRoleQdd = 13;
 
-// One derivative column for lane role `role` of a knot group and its store.
-template <typename SC, class Out>
+// One derivative column for lane role `role` of a knot group and its store
+// (MINV as in gradient_columns).
+template <bool MINV = true, typename SC, class Out>
 RBD_HD RBD_INLINE void gradient_column_lane(const PhaseC<SC>& P, float damping, int role, Out& out) {
   const bool dq = role >= NQ;
   const int c = dq ? role - (NQ - 1) : role;
   SC tau[NQ], col[NQ];
   dtau_column_uniform(P, c, dq, tau);
   add_on_diagonal(tau, dq ? -1 : c, SC(damping));
-  minv_times<true>(P.Mi, tau, col);
+  if constexpr (MINV) minv_times<true>(P.Mi, tau, col);
+  else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
   if (role < NQ) {
     for (int r = 0; r < NQ; ++r) out.dqd(r, c, float(col[r]));
   } else if (role < 2 * NQ - 1) {
@@ -652,8 +657,9 @@ RBD_HD RBD_INLINE void phase_a(const In& in, const Options& opt, PhaseC<typename
   make_phase_c(PC, tc, qd, v, a, f, Mi);
 }
 
-// Steps 1 and 5 of the reference-compatible mode (qdd and Minv given).
-template <class P, class In>
+// Steps 1 and 5 of the reference-compatible modes: qdd given, and with
+// WITH_MINV the dense Minv too (otherwise PC.Mi is left zero and unused).
+template <class P, bool WITH_MINV = true, class In>
 RBD_HD RBD_INLINE void phase_a_given_qdd_minv(const In& in, const Options& opt, PhaseC<typename P::Col>& PC) {
   using SR = typename P::Rnea; using SC = typename P::Col;
   using ST = Wider<SR, SC>;
@@ -668,7 +674,10 @@ RBD_HD RBD_INLINE void phase_a_given_qdd_minv(const In& in, const Options& opt, 
   }
   SC Mi[NQ][NQ];
   for (int r = 0; r < NQ; ++r)
-    for (int c = r; c < NQ; ++c) Mi[r][c] = SC(in.minv(r, c));
+    for (int c = r; c < NQ; ++c) {
+      if constexpr (WITH_MINV) Mi[r][c] = SC(in.minv(r, c));
+      else Mi[r][c] = SC(0);
+    }
   SR v[NQ][6], a[NQ][6], f[NQ][6];
   rnea_fwd(tr, qd, qdd, SR(opt.gravity), v, a, f);
   subtract_fext(in, f);
@@ -779,6 +788,97 @@ RBD_HD RBD_INLINE void fdgrad_knot_given_qdd_minv_lanes(const In& in, Out& out, 
   gradient_column_lane(PC, opt.damping, role, out);
 }
 
+// Inverse-dynamics gradient cdu = [dtau/dq | dtau/dqd] (98 per knot, the
+// reference's "fused" GPU stage, dynamicsGradientKernel_fused,
+// helpers_GPU/dynamicsGradient.cuh:1223-1255): q, qd, qdd given (u unused),
+// RNEA at qdd, then the derivative sweeps; no Minv.
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void id_gradient_knot(const In& in, Out& out, const Options& opt) {
+  PhaseC<typename P::Col> PC;
+  phase_a_given_qdd_minv<P, false>(in, opt, PC);
+  gradient_columns<false>(PC, opt.damping, out);
+}
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void id_gradient_knot_lanes(const In& in, Out& out, const Options& opt, int role) {
+  PhaseC<typename P::Col> PC;
+  phase_a_given_qdd_minv<P, false>(in, opt, PC);
+  if (role == kRoleQdd)
+    for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
+  gradient_column_lane<false>(PC, opt.damping, role, out);
+}
+
+// The same from precomputed v, a, f (the reference's "split" GPU stage,
+// dynamicsGradientKernel_split, .cuh:1191-1221): in.v/a/f(k, i) are RNEA's
+// link velocities, accelerations and forces, f after the backward pass.
+template <class P, class In>
+RBD_HD RBD_INLINE void phase_split(const In& in, PhaseC<typename P::Col>& PC) {
+  using SC = typename P::Col;
+  CS<SC> tc[NQ];
+  SC qd[NQ], v[NQ][6], a[NQ][6], f[NQ][6], Mi[NQ][NQ];
+  for (int k = 0; k < NQ; ++k) {
+    SC s, c;
+    sincos_s(SC(in.q(k)), &s, &c);
+    tc[k] = CS<SC>{c, s};
+    qd[k] = SC(in.qd(k));
+    for (int i = 0; i < 6; ++i) { v[k][i] = SC(in.v(k, i)); a[k][i] = SC(in.a(k, i)); f[k][i] = SC(in.f(k, i)); }
+    for (int c2 = 0; c2 < NQ; ++c2) Mi[k][c2] = SC(0);
+  }
+  make_phase_c(PC, tc, qd, v, a, f, Mi);
+}
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void id_gradient_split_knot(const In& in, Out& out, const Options& opt) {
+  PhaseC<typename P::Col> PC;
+  phase_split<P>(in, PC);
+  gradient_columns<false>(PC, opt.damping, out);
+}
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void id_gradient_split_knot_lanes(const In& in, Out& out, const Options& opt, int role) {
+  PhaseC<typename P::Col> PC;
+  phase_split<P>(in, PC);
+  if (role == kRoleQdd)
+    for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
+  gradient_column_lane<false>(PC, opt.damping, role, out);
+}
+
+// RNEA's v, a, f at (q, qd, qdd), f after the backward pass: what the
+// reference's CPU "start" stage writes into the split layout
+// (forwardDynamicsGradientSetup_forGPU, .cuh:1148-1157, with QDD_MINV_PASSED_IN).
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void vaf_knot(const In& in, Out& out, const Options& opt) {
+  using SR = typename P::Rnea;
+  CS<SR> tr[NQ];
+  SR qd[NQ], qdd[NQ];
+  for (int k = 0; k < NQ; ++k) {
+    SR s, c;
+    sincos_s(SR(in.q(k)), &s, &c);
+    tr[k] = CS<SR>{c, s};
+    qd[k] = SR(in.qd(k));
+    qdd[k] = SR(in.qdd(k));
+  }
+  SR v[NQ][6], a[NQ][6], f[NQ][6];
+  rnea_fwd(tr, qd, qdd, SR(opt.gravity), v, a, f);
+  subtract_fext(in, f);
+  rnea_bwd(tr, f);
+  for (int k = 0; k < NQ; ++k)
+    for (int i = 0; i < 6; ++i) out.vaf(k, i, float(v[k][i]), float(a[k][i]), float(f[k][i]));
+}
+
+// The reference's CPU "finish" stage (forwardDynamicsGradientFinish_forGPU,
+// .cuh:1168-1179; matMultSym, helpers_CPU/math.h:16-25): dqdd = -Minv cdu
+// with the upper triangle of Minv (49, column-major) and cdu 98 per knot.
+template <typename S = float, class In, class Out>
+RBD_HD RBD_INLINE void finish_knot(const In& in, Out& out) {
+  S Mi[NQ][NQ];
+  for (int r = 0; r < NQ; ++r)
+    for (int c = r; c < NQ; ++c) Mi[r][c] = S(in.minv(r, c));
+  for (int c = 0; c < 2 * NQ; ++c) {
+    S x[NQ], y[NQ];
+    for (int r = 0; r < NQ; ++r) x[r] = S(in.cdu(r, c));
+    minv_times<true>(Mi, x, y);
+    for (int r = 0; r < NQ; ++r) out.dqdd(r, c, float(y[r]));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Memory layouts
 // ---------------------------------------------------------------------------
@@ -787,6 +887,8 @@ constexpr int kOutPerKnot = 105;   // qdd, dqdd/dq (col-major), dqdd/dqd
 constexpr int kVolPerKnot = 77;    // q, qd, u, qdd, Minv dense (COMPLETELY_FUSED_VALS, .cuh:1145)
 constexpr int kDqddPerKnot = 98;   // dqdd/dq, dqdd/dqd
 constexpr int kFextPerSet = 42;    // external forces: 7 links x (moment, force) in the link frame
+constexpr int kFusedPerKnot = 28;  // q, qd, u, qdd (FUSED_VALS, .cuh:1144)
+constexpr int kSplitPerKnot = 154; // q, qd, u, qdd, v (42), a (42), f (42) (SPLIT_VALS, .cuh:1143)
 
 // array-of-structures: one knot's values contiguous (the reference's knot<T> order)
 struct InAoS {
@@ -826,6 +928,32 @@ struct OutDqddAoS {   // 98 per knot, no qdd
   RBD_HD void dqd(int r, int c, float x) const { p[NQ * NQ + NQ * c + r] = x; }
 };
 // structure-of-arrays: value i of knot k at p[i * ld + k]
+// split layout: [q | qd | u | qdd | v | a | f], link k's 6-vectors at 6k
+struct InSplit {
+  const float* p;
+  const float* fx = nullptr;
+  RBD_HD float q(int i) const { return p[i]; }
+  RBD_HD float qd(int i) const { return p[NQ + i]; }
+  RBD_HD float v(int k, int i) const { return p[4 * NQ + 6 * k + i]; }
+  RBD_HD float a(int k, int i) const { return p[10 * NQ + 6 * k + i]; }
+  RBD_HD float f(int k, int i) const { return p[16 * NQ + 6 * k + i]; }
+};
+struct OutVaf {   // v, a, f into the split layout
+  float* p;
+  RBD_HD void vaf(int k, int i, float v, float a, float f) const {
+    p[4 * NQ + 6 * k + i] = v; p[10 * NQ + 6 * k + i] = a; p[16 * NQ + 6 * k + i] = f;
+  }
+};
+// finish stage: Minv (49, column-major, upper triangle read) and cdu (98) in, dqdd (98) out
+struct InFinish {
+  const float* m; const float* c;
+  RBD_HD float minv(int r, int col) const { return m[r + NQ * col]; }
+  RBD_HD float cdu(int r, int col) const { return c[r + NQ * col]; }
+};
+struct OutFinish {
+  float* p;
+  RBD_HD void dqdd(int r, int col, float x) const { p[r + NQ * col] = x; }
+};
 struct InSoA {
   const float* p; long ld;
   const float* fx = nullptr;   // external link forces (42 floats) or null

@@ -162,6 +162,51 @@ __global__ __launch_bounds__(kBlock) void fd_kernel(long n, const float* __restr
   }
 }
 
+// Inverse-dynamics gradient (cdu, 98 per knot) from q, qd, qdd (SPLIT = false:
+// 28 per knot, the fused layout) or from q, qd and precomputed v, a, f
+// (SPLIT = true: 154 per knot, the split layout).  Both mappings.
+template <class P, int LANES, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void id_gradient_kernel(long n, const float* __restrict__ vol,
+                                                             float* __restrict__ cdu, Options opt) {
+  const int role = threadIdx.x % LANES;
+  const long groups = long(gridDim.x) * (blockDim.x / LANES);
+  for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
+    OutDqddAoS o{cdu + kDqddPerKnot * k};
+    if constexpr (SPLIT) {
+      InSplit i{vol + kSplitPerKnot * k};
+      if constexpr (LANES == 1) id_gradient_split_knot<P>(i, o, opt);
+      else id_gradient_split_knot_lanes<P>(i, o, opt, role);
+    } else {
+      InAoS i{vol + kFusedPerKnot * k};
+      if constexpr (LANES == 1) id_gradient_knot<P>(i, o, opt);
+      else id_gradient_knot_lanes<P>(i, o, opt, role);
+    }
+  }
+}
+
+// v, a, f of every knot of a split-layout buffer, written in place after its
+// q, qd, u, qdd.  One lane per knot.
+template <class P>
+__global__ __launch_bounds__(kBlock) void vaf_kernel(long n, float* vol, Options opt) {
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
+    InAoS i{vol + kSplitPerKnot * k};
+    OutVaf o{vol + kSplitPerKnot * k};
+    vaf_knot<P>(i, o, opt);
+  }
+}
+
+// dqdd = -Minv cdu per knot.  One lane per knot.
+__global__ __launch_bounds__(kBlock) void finish_kernel(long n, const float* __restrict__ minv,
+                                                        const float* __restrict__ cdu, float* __restrict__ dqdd) {
+  const long stride = long(gridDim.x) * blockDim.x;
+  for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
+    InFinish i{minv + NQ * NQ * k, cdu + kDqddPerKnot * k};
+    OutFinish o{dqdd + kDqddPerKnot * k};
+    finish_knot(i, o);
+  }
+}
+
 }  // namespace rbd
 
 namespace rbdi {
@@ -485,6 +530,83 @@ rbd_status rbd_fdgrad_traj_fext_f32(rbd_context* ctx, int64_t n, const float* d_
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_fext_kernel launch");
+}
+
+extern "C++" {
+namespace {
+
+template <bool SPLIT>
+rbd_status id_gradient(rbd_context* ctx, int64_t n, float* d_cdu, const float* d_vol, rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
+  if (n == 0) return RBD_OK;
+  if (!d_cdu || !d_vol) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+  if (rbd_status s = use_device(ctx)) return s;
+  const int lanes = lanes_for(ctx, n);
+  const dim3 grid(grid_for(n, lanes)), block(kBlock);
+  const hipStream_t st = (hipStream_t)stream;
+  auto go = [&](auto prec_tag) {
+    using P = decltype(prec_tag);
+    if (lanes == 1)
+      hipLaunchKernelGGL((rbd::id_gradient_kernel<P, 1, SPLIT>), grid, block, 0, st, long(n), d_vol, d_cdu, ctx->opt);
+    else
+      hipLaunchKernelGGL((rbd::id_gradient_kernel<P, rbd::kLanesPerKnotGroup, SPLIT>), grid, block, 0, st, long(n),
+                         d_vol, d_cdu, ctx->opt);
+  };
+  switch (ctx->precision) {
+    case RBD_PREC_FP32: go(rbd::PrecFP32{}); break;
+    case RBD_PREC_FP64_PIVOT: go(rbd::PrecFP64Pivot{}); break;   // no pivots here: same as FP32
+    case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP64Col{}); break;
+    default: go(rbd::PrecFP64{}); break;
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "id_gradient_kernel launch");
+}
+
+}  // namespace
+}  // extern "C++"
+
+rbd_status rbd_dynamics_gradient_kernel_fused_f32(rbd_context* ctx, int64_t n, float* d_cdu,
+                                                  const float* d_mem_vol_fused, const float* d_mem_const,
+                                                  rbd_stream_t stream) {
+  (void)d_mem_const;  // the constants are compiled into the kernel
+  return id_gradient<false>(ctx, n, d_cdu, d_mem_vol_fused, stream);
+}
+
+rbd_status rbd_dynamics_gradient_kernel_split_f32(rbd_context* ctx, int64_t n, float* d_cdu,
+                                                  const float* d_mem_vol_split, const float* d_mem_const,
+                                                  rbd_stream_t stream) {
+  (void)d_mem_const;
+  return id_gradient<true>(ctx, n, d_cdu, d_mem_vol_split, stream);
+}
+
+rbd_status rbd_gradient_setup_split_f32(rbd_context* ctx, int64_t n, float* d_mem_vol_split, rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
+  if (n == 0) return RBD_OK;
+  if (!d_mem_vol_split) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+  if (rbd_status s = use_device(ctx)) return s;
+  const dim3 grid(grid_for(n, 1)), block(kBlock);
+  const hipStream_t st = (hipStream_t)stream;
+  if (ctx->precision == RBD_PREC_FP64)
+    hipLaunchKernelGGL(rbd::vaf_kernel<rbd::PrecFP64>, grid, block, 0, st, long(n), d_mem_vol_split, ctx->opt);
+  else
+    hipLaunchKernelGGL(rbd::vaf_kernel<rbd::PrecFP32>, grid, block, 0, st, long(n), d_mem_vol_split, ctx->opt);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "vaf_kernel launch");
+}
+
+rbd_status rbd_gradient_finish_f32(rbd_context* ctx, int64_t n, float* d_dqdd, const float* d_cdu,
+                                   const float* d_minv, rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
+  if (n == 0) return RBD_OK;
+  if (!d_dqdd || !d_cdu || !d_minv) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+  if (rbd_status s = use_device(ctx)) return s;
+  const dim3 grid(grid_for(n, 1)), block(kBlock);
+  hipLaunchKernelGGL(rbd::finish_kernel, grid, block, 0, (hipStream_t)stream, long(n), d_minv, d_cdu, d_dqdd);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "finish_kernel launch");
 }
 
 int64_t rbd_group_threshold(rbd_context* ctx) { return ctx ? ctx->group_threshold.load() : -1; }

@@ -5,6 +5,9 @@
 //   knot<T>, traj<T,N>                 helpers_CPU/dynamicsGradient.h:18-30
 //   COMPLETELY_FUSED_VALS, CONST_VALS  helpers_GPU/dynamicsGradient.cuh:1142-1146
 //   dynamicsGradientKernel             helpers_GPU/dynamicsGradient.cuh:1257-1291
+//   dynamicsGradientKernel_fused/_split helpers_GPU/dynamicsGradient.cuh:1191-1255
+//   dynamicsGradientReusableThreaded_start/_finish (as device stages)
+//                                      helpers_GPU/dynamicsGradient.cuh:1148-1188
 //   forwardDynamics                    helpers_CPU/dynamicsGradient.h:685-706, :936-939
 //   gpuErrchk-style abort on error     helpers_GPU/dynamicsGradient.cuh:28-35
 // Template options keep the reference's names and defaults (MPC_MODE = false,
@@ -28,7 +31,7 @@ constexpr int SPLIT_VALS = 4 * NUM_POS + 18 * NUM_POS;                  // 154
 constexpr int FUSED_VALS = 4 * NUM_POS;                                 // 28
 constexpr int COMPLETELY_FUSED_VALS = 4 * NUM_POS + NUM_POS * NUM_POS;  // 77
 constexpr int CONST_VALS = 72 * NUM_POS;                                // 504
-constexpr int DQDD_VALS = 2 * NUM_POS * NUM_POS;                        // 98 per knot
+constexpr int DQDD_VALS = 2 * NUM_POS * NUM_POS;                        // 98 per knot (also cdu)
 
 template <typename T>
 struct knot {
@@ -79,6 +82,38 @@ inline rbd_status dynamicsGradientKernel(T* d_dqdd, T* d_mem_vol_completely_fuse
   static_assert(std::is_same<T, float>::value, "librbdgrad provides the FP32 interface");
   return rbd_dynamics_gradient_kernel_f32(context<MPC_MODE, VEL_DAMPING>(), KNOT_POINTS, d_dqdd,
                                           d_mem_vol_completely_fused, d_mem_const, stream);
+}
+
+// dynamicsGradientKernel_fused<T,KNOT_POINTS,MPC_MODE,VEL_DAMPING>(d_cdu,
+// d_mem_vol_fused, d_mem_const): q, qd, u, qdd (28 per knot) -> cdu (98 per knot).
+template <typename T, int KNOT_POINTS, bool MPC_MODE = false, bool VEL_DAMPING = true>
+inline rbd_status dynamicsGradientKernel_fused(T* d_cdu, T* d_mem_vol_fused, T* d_mem_const, hipStream_t stream = 0) {
+  static_assert(std::is_same<T, float>::value, "librbdgrad provides the FP32 interface");
+  return rbd_dynamics_gradient_kernel_fused_f32(context<MPC_MODE, VEL_DAMPING>(), KNOT_POINTS, d_cdu,
+                                                d_mem_vol_fused, d_mem_const, stream);
+}
+
+// dynamicsGradientKernel_split<T,KNOT_POINTS,VEL_DAMPING>(d_cdu, d_mem_vol_split,
+// d_mem_const): q, qd, u, qdd, v, a, f (154 per knot) -> cdu.
+template <typename T, int KNOT_POINTS, bool VEL_DAMPING = true>
+inline rbd_status dynamicsGradientKernel_split(T* d_cdu, T* d_mem_vol_split, T* d_mem_const, hipStream_t stream = 0) {
+  static_assert(std::is_same<T, float>::value, "librbdgrad provides the FP32 interface");
+  return rbd_dynamics_gradient_kernel_split_f32(context<false, VEL_DAMPING>(), KNOT_POINTS, d_cdu, d_mem_vol_split,
+                                                d_mem_const, stream);
+}
+
+// The split pipeline's CPU stages, run on the device: v, a, f into a split
+// buffer (the reference's _start), and dqdd = -Minv cdu (its _finish; Minv 49
+// per knot, column-major, upper triangle read).
+template <typename T, int KNOT_POINTS, bool MPC_MODE = false, bool VEL_DAMPING = true>
+inline rbd_status dynamicsGradientStart(T* d_mem_vol_split, hipStream_t stream = 0) {
+  static_assert(std::is_same<T, float>::value, "librbdgrad provides the FP32 interface");
+  return rbd_gradient_setup_split_f32(context<MPC_MODE, VEL_DAMPING>(), KNOT_POINTS, d_mem_vol_split, stream);
+}
+template <typename T, int KNOT_POINTS>
+inline rbd_status dynamicsGradientFinish(T* d_dqdd, const T* d_cdu, const T* d_Minv, hipStream_t stream = 0) {
+  static_assert(std::is_same<T, float>::value, "librbdgrad provides the FP32 interface");
+  return rbd_gradient_finish_f32(context<false, true>(), KNOT_POINTS, d_dqdd, d_cdu, d_Minv, stream);
 }
 
 // The full path for one trajectory on the device: 21 in (q, qd, u) and 105 out

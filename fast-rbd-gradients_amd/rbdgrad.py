@@ -41,6 +41,8 @@ EXPORTED = [
     "rbd_server_poll", "rbd_server_wait", "rbd_server_get_stats", "rbd_server_reset_stats",
     "rbd_server_run_poisson", "rbd_fdgrad_minv_traj_f32", "rbd_forward_dynamics_f32",
     "rbd_fdgrad_traj_fext_f32", "rbd_forward_dynamics_fext_f32",
+    "rbd_dynamics_gradient_kernel_fused_f32", "rbd_dynamics_gradient_kernel_split_f32",
+    "rbd_gradient_setup_split_f32", "rbd_gradient_finish_f32",
 ]
 
 
@@ -106,6 +108,10 @@ def lib():
         L.rbd_fdgrad_minv_traj_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_forward_dynamics_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_fdgrad_traj_fext_f32.argtypes = [vp, i64, vp, vp, i64, vp, vp]
+        L.rbd_dynamics_gradient_kernel_fused_f32.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.rbd_dynamics_gradient_kernel_split_f32.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.rbd_gradient_setup_split_f32.argtypes = [vp, i64, vp, vp]
+        L.rbd_gradient_finish_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_forward_dynamics_fext_f32.argtypes = [vp, i64, vp, vp, i64, vp, vp, vp]
         L.rbd_kernel_name.argtypes = [vp, ctypes.c_int, i64]
         L.rbd_kernel_name.restype = ctypes.c_char_p
@@ -405,6 +411,47 @@ class Context:
         _check(lib().rbd_fdgrad_traj_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_out.handle, None),
                "rbd_fdgrad_traj_fext_f32")
         return d_out.download((n, OUT_PER_KNOT))
+
+    def id_gradient(self, vol):
+        """Inverse-dynamics gradient (n, 98) from the fused layout (n, 28) [q|qd|u|qdd]."""
+        vol = np.ascontiguousarray(np.atleast_2d(vol), dtype=np.float32)
+        n = vol.shape[0]
+        d_vol = DeviceBuffer.from_array(vol)
+        d_cdu = DeviceBuffer(n * DQDD_PER_KNOT * 4)
+        _check(lib().rbd_dynamics_gradient_kernel_fused_f32(self._h, n, d_cdu.handle, d_vol.handle, None, None),
+               "rbd_dynamics_gradient_kernel_fused_f32")
+        return d_cdu.download((n, DQDD_PER_KNOT))
+
+    def id_gradient_split(self, vol):
+        """Inverse-dynamics gradient (n, 98) from the split layout (n, 154)."""
+        vol = np.ascontiguousarray(np.atleast_2d(vol), dtype=np.float32)
+        n = vol.shape[0]
+        d_vol = DeviceBuffer.from_array(vol)
+        d_cdu = DeviceBuffer(n * DQDD_PER_KNOT * 4)
+        _check(lib().rbd_dynamics_gradient_kernel_split_f32(self._h, n, d_cdu.handle, d_vol.handle, None, None),
+               "rbd_dynamics_gradient_kernel_split_f32")
+        return d_cdu.download((n, DQDD_PER_KNOT))
+
+    def gradient_setup_split(self, vol28):
+        """Split layout (n, 154) with v, a, f computed on the device from (n, 28)."""
+        vol28 = np.ascontiguousarray(np.atleast_2d(vol28), dtype=np.float32)
+        n = vol28.shape[0]
+        vol = np.zeros((n, 154), np.float32)
+        vol[:, :28] = vol28
+        d_vol = DeviceBuffer.from_array(vol)
+        _check(lib().rbd_gradient_setup_split_f32(self._h, n, d_vol.handle, None), "rbd_gradient_setup_split_f32")
+        return d_vol.download((n, 154))
+
+    def gradient_finish(self, minv, cdu):
+        """-Minv cdu: minv (n, 49) column-major (upper triangle read), cdu (n, 98) -> (n, 98)."""
+        minv = np.ascontiguousarray(np.atleast_2d(minv), dtype=np.float32)
+        cdu = np.ascontiguousarray(np.atleast_2d(cdu), dtype=np.float32)
+        n = cdu.shape[0]
+        d_m, d_c = DeviceBuffer.from_array(minv), DeviceBuffer.from_array(cdu)
+        d_out = DeviceBuffer(n * DQDD_PER_KNOT * 4)
+        _check(lib().rbd_gradient_finish_f32(self._h, n, d_out.handle, d_c.handle, d_m.handle, None),
+               "rbd_gradient_finish_f32")
+        return d_out.download((n, DQDD_PER_KNOT))
 
     def run_refmode(self, vol):
         vol = np.ascontiguousarray(np.atleast_2d(vol), dtype=np.float32)

@@ -157,6 +157,44 @@ rbd_status rbd_fdgrad_traj_fext_f32(rbd_context* ctx, int64_t n_knots, const flo
 rbd_status rbd_forward_dynamics_fext_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, const float* d_fext,
                                          int64_t fext_stride, float* d_qdd, float* d_minv, rbd_stream_t stream);
 
+/* The reference's staged GPU variants (time_GPU.cu:227-333), for callers
+ * that keep their pipeline.  cdu is the inverse-dynamics gradient
+ * [dtau/dq | dtau/dqd], 98 floats per knot, each 7x7 column-major
+ * (cdu[7c + k] = dtau_k/dx_c; with VEL_DAMPING +0.5 on the dtau/dqd diagonal).
+ * d_mem_const is accepted and ignored (the model constants are compiled in).
+ *
+ * Fused: q, qd, u, qdd (28 floats per knot; u unused) -> cdu.
+ * Replaces: dynamicsGradientKernel_fused<T,N,MPC,VD><<<N,(32,7)>>>(d_cdu,
+ * d_mem_vol_fused, d_mem_const) (helpers_GPU/dynamicsGradient.cuh:1223-1255). */
+rbd_status rbd_dynamics_gradient_kernel_fused_f32(rbd_context* ctx, int64_t n_knots, float* d_cdu,
+                                                  const float* d_mem_vol_fused, const float* d_mem_const,
+                                                  rbd_stream_t stream);
+
+/* Split: q, qd, u, qdd, v, a, f (154 floats per knot: the 28 above, then
+ * link k's v, a, f 6-vectors at 28 + 6k, 70 + 6k, 112 + 6k; f after RNEA's
+ * backward pass) -> cdu.  The context's gravity flag does not apply (it is
+ * already in a).
+ * Replaces: dynamicsGradientKernel_split<T,N,VD><<<N,(32,7)>>>(d_cdu,
+ * d_mem_vol_split, d_mem_const) (.cuh:1191-1221). */
+rbd_status rbd_dynamics_gradient_kernel_split_f32(rbd_context* ctx, int64_t n_knots, float* d_cdu,
+                                                  const float* d_mem_vol_split, const float* d_mem_const,
+                                                  rbd_stream_t stream);
+
+/* Fill v, a, f of a split-layout buffer (154 floats per knot) from its q, qd,
+ * qdd, in place, on the device.
+ * Replaces: dynamicsGradientReusableThreaded_start (.cuh:1159-1166, the CPU
+ * stage before the split kernel; forwardDynamicsGradientSetup_forGPU :1148-1157). */
+rbd_status rbd_gradient_setup_split_f32(rbd_context* ctx, int64_t n_knots, float* d_mem_vol_split,
+                                        rbd_stream_t stream);
+
+/* dqdd = -Minv cdu per knot: Minv 49 floats per knot (column-major, the upper
+ * triangle is read), cdu 98 -> dqdd 98 ([dqdd/dq | dqdd/dqd]), on the device.
+ * Replaces: dynamicsGradientReusableThreaded_finish (.cuh:1181-1188, the CPU
+ * stage after the split and fused kernels; forwardDynamicsGradientFinish_forGPU
+ * :1168-1179). */
+rbd_status rbd_gradient_finish_f32(rbd_context* ctx, int64_t n_knots, float* d_dqdd, const float* d_cdu,
+                                   const float* d_minv, rbd_stream_t stream);
+
 /* Name of the kernel the context dispatches a call of n_knots to (full path,
  * or the reference-compatible mode when reference_mode != 0), as profilers list
  * it (template arguments abbreviated).  No reference counterpart. */

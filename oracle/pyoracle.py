@@ -44,6 +44,11 @@ def lib():
         L.oracle_fdgrad_refmode_f64.argtypes = [ctypes.c_long, dp, dp, ctypes.c_int]
         L.oracle_fdgrad_refmode_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int]
         L.oracle_fd_complex_step.argtypes = [dp, ctypes.c_double, dp, ctypes.c_int]
+        L.oracle_id_gradient_f64.argtypes = [ctypes.c_long, dp, dp, ctypes.c_int]
+        L.oracle_vaf_f64.argtypes = [ctypes.c_long, dp, ctypes.c_int]
+        L.oracle_id_gradient_split_f64.argtypes = [ctypes.c_long, dp, dp, ctypes.c_int]
+        L.oracle_finish_f64.argtypes = [ctypes.c_long, dp, dp, dp]
+        L.oracle_rnea_complex_step.argtypes = [dp, ctypes.c_double, dp, ctypes.c_int]
         L.oracle_fd_complex_step_fext.argtypes = [dp, dp, ctypes.c_double, dp, ctypes.c_int]
         L.oracle_fdgrad_fext_f64.argtypes = [ctypes.c_long, dp, dp, ctypes.c_long, dp, ctypes.c_int]
         L.oracle_rnea_fext.argtypes = [dp, dp, dp, dp, dp, ctypes.c_int]
@@ -132,6 +137,51 @@ def fd_complex_step(x, h=1e-20, flags=0, fext=None):
     else:
         f = _c64(fext, FEXT_PER_SET)
         lib().oracle_fd_complex_step_fext(_dp(x), _dp(f), h, _dp(out), flags)
+    return out
+
+
+FUSED_PER_KNOT = 28    # q, qd, u, qdd
+SPLIT_PER_KNOT = 154   # q, qd, u, qdd, v (42), a (42), f (42)
+
+
+def id_gradient(vol, flags=0):
+    """Inverse-dynamics gradient cdu (n, 98) = [dtau/dq | dtau/dqd] from the
+    fused layout (n, 28) [q|qd|u|qdd]."""
+    vol = _c64(np.atleast_2d(vol), FUSED_PER_KNOT)
+    out = np.empty((vol.shape[0], DQDD_PER_KNOT))
+    lib().oracle_id_gradient_f64(vol.shape[0], _dp(vol), _dp(out), flags)
+    return out
+
+
+def vaf(vol, flags=0):
+    """Split layout (n, 154) with v, a, f filled from the first 28 columns."""
+    vol = np.array(np.atleast_2d(vol), dtype=np.float64)
+    assert vol.shape[-1] == SPLIT_PER_KNOT
+    lib().oracle_vaf_f64(vol.shape[0], _dp(vol), flags)
+    return vol
+
+
+def id_gradient_split(vol, flags=0):
+    vol = _c64(np.atleast_2d(vol), SPLIT_PER_KNOT)
+    out = np.empty((vol.shape[0], DQDD_PER_KNOT))
+    lib().oracle_id_gradient_split_f64(vol.shape[0], _dp(vol), _dp(out), flags)
+    return out
+
+
+def finish(minv, cdu):
+    """dqdd = -Minv cdu: minv (n, 49) column-major (upper triangle read), cdu (n, 98)."""
+    minv = _c64(np.atleast_2d(minv), NQ * NQ)
+    cdu = _c64(np.atleast_2d(cdu), DQDD_PER_KNOT)
+    out = np.empty_like(cdu)
+    lib().oracle_finish_f64(cdu.shape[0], _dp(minv), _dp(cdu), _dp(out))
+    return out
+
+
+def rnea_complex_step(vol, h=1e-20, flags=0):
+    """cdu of one knot (vol: 28 values) by complex-step differentiation of RNEA."""
+    vol = _c64(vol, FUSED_PER_KNOT)
+    out = np.empty(DQDD_PER_KNOT)
+    lib().oracle_rnea_complex_step(_dp(vol), h, _dp(out), flags)
     return out
 
 

@@ -126,6 +126,67 @@ void oracle_fd_complex_step(const double* in, double h, double* out, int flags) 
   oracle_fd_complex_step_fext(in, nullptr, h, out, flags);
 }
 
+// The reference's staged variants (time_GPU.cu:227-333).
+// Fused layout: vol28 = [q | qd | u | qdd]; split layout: vol154 = [q | qd | u | qdd |
+// v (6 per link) | a | f], f after RNEA's backward pass.  cdu = [dtau/dq | dtau/dqd],
+// each 7x7 column-major.
+void oracle_id_gradient_f64(long n, const double* vol, double* cdu, int flags) {
+  for (long k = 0; k < n; ++k) {
+    const double* x = vol + 28 * k;
+    Knot<double> K;
+    setup_model(K, x);
+    rnea_forward(K, x + 7, x + 21, flags);
+    rnea_backward(K, (double*)nullptr);
+    inverse_dynamics_gradient(K, x + 7, cdu + 98 * k, cdu + 98 * k + 49, flags);
+  }
+}
+void oracle_vaf_f64(long n, double* vol, int flags) {
+  for (long k = 0; k < n; ++k) {
+    double* x = vol + 154 * k;
+    Knot<double> K;
+    setup_model(K, x);
+    rnea_forward(K, x + 7, x + 21, flags);
+    rnea_backward(K, (double*)nullptr);
+    for (int j = 0; j < NQ; ++j)
+      for (int i = 0; i < 6; ++i) {
+        x[28 + 6 * j + i] = K.v[j][i];
+        x[70 + 6 * j + i] = K.a[j][i];
+        x[112 + 6 * j + i] = K.f[j][i];
+      }
+  }
+}
+void oracle_id_gradient_split_f64(long n, const double* vol, double* cdu, int flags) {
+  for (long k = 0; k < n; ++k) {
+    const double* x = vol + 154 * k;
+    Knot<double> K;
+    setup_model(K, x);
+    for (int j = 0; j < NQ; ++j)
+      for (int i = 0; i < 6; ++i) {
+        K.v[j][i] = x[28 + 6 * j + i];
+        K.a[j][i] = x[70 + 6 * j + i];
+        K.f[j][i] = x[112 + 6 * j + i];
+      }
+    inverse_dynamics_gradient(K, x + 7, cdu + 98 * k, cdu + 98 * k + 49, flags);
+  }
+}
+// dqdd = -Minv cdu (upper triangle of the 49-value column-major Minv).
+void oracle_finish_f64(long n, const double* minv, const double* cdu, double* dqdd) {
+  for (long k = 0; k < n; ++k)
+    for (int c = 0; c < 2 * NQ; ++c) symv_upper<true>(dqdd + 98 * k + NQ * c, minv + 49 * k, cdu + 98 * k + NQ * c);
+}
+// cdu of one knot (vol28) by complex-step differentiation of RNEA at fixed qdd.
+void oracle_rnea_complex_step(const double* vol, double h, double* cdu, int flags) {
+  using C = std::complex<double>;
+  C q[NQ], qd[NQ], qdd[NQ], tau[NQ];
+  for (int j = 0; j < 2 * NQ; ++j) {
+    for (int i = 0; i < NQ; ++i) { q[i] = vol[i]; qd[i] = vol[7 + i]; qdd[i] = vol[21 + i]; }
+    if (j < NQ) q[j] += C(0, h);
+    else qd[j - NQ] += C(0, h);
+    rnea_knot<C>(q, qd, qdd, tau, flags);
+    for (int i = 0; i < NQ; ++i) cdu[NQ * j + i] = tau[i].imag() / h;
+  }
+}
+
 // Operation count of one knot point of the FP32 method.
 // mode 0: full path (q, qd, u -> qdd, dqdd/dq, dqdd/dqd);
 // mode 1: reference-compatible mode (qdd and Minv given).

@@ -10,7 +10,9 @@
 //   3. with external link forces, forwardDynamics(traj, fext) and
 //      forwardDynamicsGradient(out, in, fext) agree on qdd, and the forces
 //      change it;
-//   4. the TEST_FOR_EQUIVALENCE inputs (all 0.1, time_GPU.cu:124-127) match
+//   4. the staged pipelines (fused kernel + finish; start + split kernel +
+//      finish) give the same dqdd as dynamicsGradientKernel;
+//   5. the TEST_FOR_EQUIVALENCE inputs (all 0.1, time_GPU.cu:124-127) match
 //      the values the FP64 oracle gives for them (passed on the command line).
 // Exit code 0 on success; prints what it checked.
 #include <cmath>
@@ -70,8 +72,48 @@ int run(std::mt19937& rng) {
     }
     qdd_worst = std::fmax(qdd_worst, d / std::fmax(m, 1e-6));
   }
-  std::printf("N=%d VEL_DAMPING=%d: reference-mode vs full path max rel err %.3e, forwardDynamics qdd max rel err %.3e\n",
-              N, int(VD), worst, qdd_worst);
+  // staged pipelines: fused kernel -> finish, and start -> split kernel -> finish
+  std::vector<float> fused(size_t(FUSED_VALS) * N), split(size_t(SPLIT_VALS) * N, 0.f), minv(size_t(N) * 49);
+  for (int k = 0; k < N; ++k) {
+    for (int i = 0; i < FUSED_VALS; ++i) fused[FUSED_VALS * k + i] = split[SPLIT_VALS * k + i] = vol[COMPLETELY_FUSED_VALS * k + i];
+    for (int i = 0; i < 49; ++i) minv[49 * k + i] = vol[COMPLETELY_FUSED_VALS * k + FUSED_VALS + i];
+  }
+  float *d_fused, *d_split, *d_cdu, *d_minv, *d_dqdd2;
+  if (hipMalloc(&d_fused, fused.size() * 4) || hipMalloc(&d_split, split.size() * 4) ||
+      hipMalloc(&d_cdu, size_t(N) * DQDD_VALS * 4) || hipMalloc(&d_minv, minv.size() * 4) ||
+      hipMalloc(&d_dqdd2, size_t(N) * DQDD_VALS * 4)) {
+    std::fprintf(stderr, "hipMalloc failed\n");
+    return 1;
+  }
+  (void)hipMemcpy(d_fused, fused.data(), fused.size() * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(d_split, split.data(), split.size() * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(d_minv, minv.data(), minv.size() * 4, hipMemcpyHostToDevice);
+  double staged_worst = 0;
+  for (int variant = 0; variant < 2; ++variant) {
+    if (variant == 0) {
+      rbdErrchk((dynamicsGradientKernel_fused<float, N, false, VD>(d_cdu, d_fused, nullptr)));
+    } else {
+      rbdErrchk((dynamicsGradientStart<float, N, false, VD>(d_split)));
+      rbdErrchk((dynamicsGradientKernel_split<float, N, VD>(d_cdu, d_split, nullptr)));
+    }
+    rbdErrchk((dynamicsGradientFinish<float, N>(d_dqdd2, d_cdu, d_minv)));
+    std::vector<float> got(size_t(N) * DQDD_VALS);
+    (void)hipMemcpy(got.data(), d_dqdd2, got.size() * 4, hipMemcpyDeviceToHost);
+    for (int k = 0; k < N; ++k)
+      for (int b = 0; b < 2; ++b) {
+        double d = 0, m = 0;
+        for (int i = 0; i < 49; ++i) {
+          const double a = got[DQDD_VALS * k + 49 * b + i], r = dqdd[DQDD_VALS * k + 49 * b + i];
+          d = std::fmax(d, std::fabs(a - r));
+          m = std::fmax(m, std::fabs(r));
+        }
+        staged_worst = std::fmax(staged_worst, d / std::fmax(m, 1e-6));
+      }
+  }
+  (void)hipFree(d_fused); (void)hipFree(d_split); (void)hipFree(d_cdu); (void)hipFree(d_minv); (void)hipFree(d_dqdd2);
+  std::printf("N=%d VEL_DAMPING=%d: reference-mode vs full path max rel err %.3e, forwardDynamics qdd max rel err %.3e, "
+              "staged pipelines vs reference mode %.3e\n", N, int(VD), worst, qdd_worst, staged_worst);
+  worst = std::fmax(worst, staged_worst);
   (void)hipFree(d_vol); (void)hipFree(d_dqdd); (void)hipFree(d_in); (void)hipFree(d_out);
   delete t;
   return (worst < 1e-4 && qdd_worst < 1e-5) ? 0 : 1;

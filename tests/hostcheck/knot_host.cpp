@@ -125,6 +125,44 @@ void host_fd_fext_f32(long n, const float* in, const float* fext, long fext_stri
   }
 }
 
+// the reference's staged variants: inverse-dynamics gradient from the fused
+// (split = 0, 28 per knot) or split (154 per knot) layout, the v/a/f stage and
+// the finish stage; FP32, one lane (lanes = 0) or lane-group emulation
+void host_id_gradient_f32(long n, const float* vol, float* cdu, int flags, int split, int lanes) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    OutDqddAoS o{cdu + kDqddPerKnot * k};
+    if (split) {
+      InSplit i{vol + kSplitPerKnot * k};
+      if (lanes)
+        for (int role = 0; role < kLanesPerKnotGroup; ++role) id_gradient_split_knot_lanes<PrecFP32>(i, o, opt, role);
+      else
+        id_gradient_split_knot<PrecFP32>(i, o, opt);
+    } else {
+      InAoS i{vol + kFusedPerKnot * k};
+      if (lanes)
+        for (int role = 0; role < kLanesPerKnotGroup; ++role) id_gradient_knot_lanes<PrecFP32>(i, o, opt, role);
+      else
+        id_gradient_knot<PrecFP32>(i, o, opt);
+    }
+  }
+}
+void host_vaf_f32(long n, float* vol, int flags) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    InAoS i{vol + kSplitPerKnot * k};
+    OutVaf o{vol + kSplitPerKnot * k};
+    vaf_knot<PrecFP32>(i, o, opt);
+  }
+}
+void host_finish_f32(long n, const float* minv, const float* cdu, float* dqdd) {
+  for (long k = 0; k < n; ++k) {
+    InFinish i{minv + NQ * NQ * k, cdu + kDqddPerKnot * k};
+    OutFinish o{dqdd + kDqddPerKnot * k};
+    finish_knot(i, o);
+  }
+}
+
 void host_fdgrad_f32(long n, const float* in, float* out, int flags, int prec, int lanes) {
   by_precision(prec, false, flags, n, in, out, lanes);
 }

@@ -28,6 +28,9 @@ def host():
     L.host_model_transform.argtypes = [ctypes.c_int, ctypes.c_double, dp]
     L.host_fdgrad_fext_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_long, fp, ctypes.c_int, ctypes.c_int]
     L.host_fd_fext_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_long, fp, fp, ctypes.c_int]
+    L.host_id_gradient_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.host_vaf_f32.argtypes = [ctypes.c_long, fp, ctypes.c_int]
+    L.host_finish_f32.argtypes = [ctypes.c_long, fp, fp, fp]
     return L
 
 
@@ -160,3 +163,37 @@ def test_external_forces(host, oracle, flags, per_knot):
     # the forces matter: the result without them is far off
     plain = run(host, x, flags)
     assert np.abs(plain - out).max() > 1e-2
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2])
+def test_staged_variants(host, oracle, flags):
+    """Fused and split inverse-dynamics gradient, the v/a/f stage and the
+    finish stage against the oracle's."""
+    n = 150
+    x = common.knots(n, seed=270 + flags, near_singular_frac=0.1).astype(np.float64)
+    qdd = oracle.fdgrad(x, flags)[:, :7]
+    vol = np.concatenate([x, qdd], axis=1)
+    ref_cdu = oracle.id_gradient(vol, flags)
+    vol32 = np.ascontiguousarray(vol, dtype=np.float32)
+    split32 = np.ascontiguousarray(np.concatenate([vol32, np.zeros((n, 126), np.float32)], axis=1))
+    host.host_vaf_f32(n, _p(split32), flags)
+    ref_split = oracle.vaf(np.concatenate([vol32.astype(np.float64), np.zeros((n, 126))], axis=1), flags)
+    np.testing.assert_allclose(split32, ref_split, rtol=0, atol=2e-5 * np.abs(ref_split).max())
+    for split, inp in ((0, vol32), (1, split32)):
+        for lanes in (0, 1):
+            cdu = np.full((n, 98), np.nan, np.float32)
+            host.host_id_gradient_f32(n, _p(inp), _p(cdu), flags, split, lanes)
+            err = {k: float(v.max()) for k, v in common.dqdd_block_errors(cdu, ref_cdu).items()}
+            assert max(err.values()) < common.REL_TOL, (split, lanes, err)
+    minv = np.stack([oracle.fd(v[:7], v[7:14], v[14:21], flags)[1].T.ravel() for v in x])
+    minv32 = np.ascontiguousarray(minv, dtype=np.float32)
+    cdu32 = np.ascontiguousarray(ref_cdu, dtype=np.float32)
+    dqdd = np.full((n, 98), np.nan, np.float32)
+    host.host_finish_f32(n, _p(minv32), _p(cdu32), _p(dqdd))
+    ref = oracle.finish(minv32.astype(np.float64), cdu32.astype(np.float64))
+    err = {k: float(v.max()) for k, v in common.dqdd_block_errors(dqdd, ref).items()}
+    assert max(err.values()) < 1e-6, err

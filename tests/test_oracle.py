@@ -216,6 +216,47 @@ def test_per_knot_external_forces(oracle):
         np.testing.assert_array_equal(out[i], oracle.fdgrad(x[i:i + 1], fext=fext[i])[0])
 
 
+def fused_inputs(n, seed, oracle, flags=0):
+    """(n, 28) [q|qd|u|qdd] with qdd = FD(q, qd, u), as the reference prepares them."""
+    x = common.knots(n, seed=seed, near_singular_frac=0.2).astype(np.float64)
+    qdd = oracle.fdgrad(x, flags)[:, :7]
+    return np.concatenate([x, qdd], axis=1)
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2])
+def test_id_gradient_matches_complex_step(oracle, flags):
+    vol = fused_inputs(20, 47, oracle, flags)
+    vol[::3, 21:] = np.random.default_rng(48).uniform(-30, 30, (7, 7))   # also off the dynamics manifold
+    cdu = oracle.id_gradient(vol, flags)
+    for i in range(len(vol)):
+        cs = oracle.rnea_complex_step(vol[i], flags=flags)
+        scale = np.abs(cs).max()
+        np.testing.assert_allclose(cdu[i], cs, rtol=0, atol=1e-12 * scale)
+
+
+def test_split_stage_matches_fused(oracle, model):
+    vol = fused_inputs(10, 49, oracle)
+    split = oracle.vaf(np.concatenate([vol, np.zeros((10, 126))], axis=1))
+    np.testing.assert_array_equal(oracle.id_gradient_split(split), oracle.id_gradient(vol))
+    # v is the textbook link velocity; f[k][2] after the backward pass is tau
+    for i in range(10):
+        q, qd, qdd = vol[i, :7], vol[i, 7:14], vol[i, 21:]
+        np.testing.assert_allclose(split[i, 28:70].reshape(7, 6), model.link_velocities(q, qd), rtol=0, atol=1e-12)
+        np.testing.assert_allclose(split[i, 112:154].reshape(7, 6)[:, 2], model.rnea(q, qd, qdd), rtol=0,
+                                   atol=1e-10 * np.abs(model.rnea(q, qd, qdd)).max())
+
+
+@pytest.mark.parametrize("flags", [0, 1])
+def test_finish_of_id_gradient_is_the_full_path(oracle, flags):
+    """dqdd = -Minv dtau/dx at qdd = FD(q, qd, u): the staged pipeline gives the
+    full path's gradient."""
+    vol = fused_inputs(15, 50, oracle, flags)
+    minv = np.stack([oracle.fd(v[:7], v[7:14], v[14:21], flags)[1].T.ravel() for v in vol])
+    dqdd = oracle.finish(minv, oracle.id_gradient(vol, flags))
+    full = oracle.fdgrad(vol[:, :21], flags)[:, 7:]
+    np.testing.assert_allclose(dqdd, full, rtol=0, atol=1e-11 * np.abs(full).max())
+
+
 def test_op_count_is_frozen(oracle):
     """The algorithmic FLOP count per knot used for the roofline (bench.py)."""
     c = oracle.op_count(0)
