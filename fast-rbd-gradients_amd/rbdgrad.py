@@ -22,6 +22,8 @@ DQDD_PER_KNOT = 98
 FLAG_VEL_DAMPING = 1
 FLAG_NO_GRAVITY = 2
 
+VEL_DAMPING = 1        # RBD_FLAG_VEL_DAMPING
+NO_GRAVITY = 2         # RBD_FLAG_NO_GRAVITY (the reference's MPC_MODE)
 PREC_FP32 = 0
 PREC_FP64_PIVOT = 1
 PREC_FP64_COLUMNS = 2
