@@ -254,6 +254,9 @@ def main():
     threshold, tune_table = ctx.autotune(stream)
     log(f"autotune: lane-group mapping up to {threshold} knots; table (n: us one-lane, us lane-group) "
         f"{ {k: (round(a, 2), round(b, 2)) for k, (a, b) in tune_table.items()} }")
+    bthreshold, btune_table = ctx.autotune_batch(stream)
+    log(f"autotune (batched): lane-group mapping up to {bthreshold} knots; table (total knots: us one-lane, "
+        f"us lane-group) { {k: (round(a, 2), round(b, 2)) for k, (a, b) in btune_table.items()} }")
 
     # ---- headline: per-call kernel, one trajectory of 128 knots per launch ----
     n1 = 128
@@ -269,7 +272,10 @@ def main():
 
     workloads = {"autotune": {"group_threshold_knots": threshold,
                               "us_per_launch": {str(k): {"one_lane": a, "lane_group": b}
-                                                for k, (a, b) in tune_table.items()}}}
+                                                for k, (a, b) in tune_table.items()},
+                              "batch_group_threshold_knots": bthreshold,
+                              "batch_us_per_batch": {str(k): {"one_lane": a, "lane_group": b}
+                                                     for k, (a, b) in btune_table.items()}}}
     # the same per-call workload with the mapping forced to one lane per knot
     one = rbdgrad.Context(device=0, precision=args.precision, lanes_per_knot=1)
     ms1b, _ = run_flat(one, x1, args.steps // 4, args.warmup, stream)

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "batch_kernels.hpp"
 #include "context.hpp"
@@ -15,10 +16,9 @@ using namespace rbdi;
 struct rbd_batch {
   rbd_context* ctx = nullptr;
   int max_traj = 0;
-  // device: [BatchHeader | TrajDesc[max_traj] | long prefix[max_traj + 1] | BatchQueue]
+  // device: [BatchHeader | TrajDesc[max_traj] | long prefix[max_traj + 1]]
   char* d_mem = nullptr;
   rbd::BatchHeader* d_hdr = nullptr;
-  rbd::BatchQueue* d_queue = nullptr;
   rbd::TrajDesc* d_desc = nullptr;
   long* d_prefix = nullptr;
   size_t upload_bytes = 0;            // header + descriptors
@@ -44,7 +44,7 @@ template <class P, int LANES, bool SMALL>
 void launch_persistent(rbd_batch* b, hipStream_t s) {
   const unsigned grid = b->grid[LANES == 1 ? 0 : 1];
   hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, SMALL>), dim3(grid), dim3(kBlock), 0, s, b->d_desc,
-                     b->d_hdr, b->d_queue, b->d_prefix, b->ctx->opt);
+                     b->d_hdr, b->d_prefix, b->ctx->opt);
 }
 
 template <class P>
@@ -113,17 +113,13 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
   b->max_traj = max_trajectories;
   const size_t hdr = align_up(sizeof(rbd::BatchHeader), 256);
   const size_t desc = align_up(sizeof(rbd::TrajDesc) * size_t(max_trajectories), 256);
-  const size_t pre = align_up(sizeof(long) * (size_t(max_trajectories) + 1), 256);
+  const size_t pre = sizeof(long) * (size_t(max_trajectories) + 1);
   b->upload_bytes = hdr + sizeof(rbd::TrajDesc) * size_t(max_trajectories);
-  hipError_t e = hipMalloc(&b->d_mem, hdr + desc + pre + sizeof(rbd::BatchQueue));
+  hipError_t e = hipMalloc(&b->d_mem, hdr + desc + pre);
   if (e != hipSuccess) { delete b; return fail(RBD_ERR_OUT_OF_MEMORY, "hipMalloc (batch)"); }
   b->d_hdr = reinterpret_cast<rbd::BatchHeader*>(b->d_mem);
   b->d_desc = reinterpret_cast<rbd::TrajDesc*>(b->d_mem + hdr);
   b->d_prefix = reinterpret_cast<long*>(b->d_mem + hdr + desc);
-  b->d_queue = reinterpret_cast<rbd::BatchQueue*>(b->d_mem + hdr + desc + pre);
-  // queue counters start at zero (set once, outside any captured graph)
-  e = hipMemset(b->d_queue, 0, sizeof(rbd::BatchQueue));
-  if (e != hipSuccess) { (void)hipFree(b->d_mem); delete b; return hip_fail(e, "hipMemset (batch queue)"); }
   for (int i = 0; i < rbd_batch::kRing && e == hipSuccess; ++i) {
     e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stage[i]), b->upload_bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
@@ -132,9 +128,10 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
     rbd_batch_destroy(b);
     return hip_fail(e, "batch staging");
   }
-  // persistent grid: one resident wave per SIMD (the kernels run at one wave
-  // per SIMD: 256+ registers per lane)
-  b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4);
+  // persistent grid: as many waves as the SIMDs (4 per CU) keep resident --
+  // two per SIMD with FP32 columns, one with FP64 columns
+  const int waves = (ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT) ? 2 : 1;
+  b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4 * waves);
   *out = b;
   return RBD_OK;
 }
@@ -174,6 +171,14 @@ rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves) 
 
 rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph,
                                   rbd_stream_t stream) {
+  return rbdi::batch_run(b, trajs, n_traj, use_graph, stream, 0);
+}
+
+}  // extern "C"
+
+// lanes = 0: by the context (lanes_for_batch)
+rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph, rbd_stream_t stream,
+                           int lanes_req) {
   if (!b) return fail(RBD_ERR_INVALID_ARGUMENT, "null batch");
   if (n_traj < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative trajectory count");
   if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
@@ -189,7 +194,7 @@ rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_tra
   if (total == 0) return RBD_OK;
   if (rbd_status s = use_device(b->ctx)) return s;
   const hipStream_t st = (hipStream_t)stream;
-  const int lanes = lanes_for(b->ctx, total);
+  const int lanes = lanes_req ? lanes_req : lanes_for_batch(b->ctx, total);
   b->last_lanes = lanes;
   // stage header + descriptors in pinned memory, copy to the device on `stream`
   const int r = b->ring;
@@ -198,7 +203,7 @@ rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_tra
   if (e != hipSuccess) return hip_fail(e, "staging wait");
   const size_t hdr = align_up(sizeof(rbd::BatchHeader), 256);
   rbd::BatchHeader* h = reinterpret_cast<rbd::BatchHeader*>(b->h_stage[r]);
-  std::memset(h, 0, sizeof(*h));
+  std::memset(h, 0, sizeof(*h));             // cursor = 0
   h->n_traj = n_traj;
   h->chunk_knots = 64 / lanes;
   rbd::TrajDesc* d = reinterpret_cast<rbd::TrajDesc*>(b->h_stage[r] + hdr);
@@ -219,6 +224,80 @@ rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_tra
   return RBD_OK;
 }
 
+extern "C" {
+
 int rbd_batch_last_lanes(const rbd_batch* b) { return b ? b->last_lanes : 0; }
+
+// Times both mappings of the batched path on batches of 132-knot trajectories
+// (the mean length of lengths uniform over 8..256) at 8 ... 512 trajectories
+// and sets the largest total at which the lane-group mapping is faster.
+rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
+                              int table_len) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (rbd_status s = use_device(ctx)) return s;
+  constexpr int64_t kLen = 132;
+  static const int counts[] = {8, 16, 32, 64, 128, 256, 512};
+  constexpr int kSizes = sizeof(counts) / sizeof(counts[0]);
+  const int max_traj = counts[kSizes - 1];
+  const size_t knots = size_t(max_traj) * kLen;
+  const hipStream_t st = (hipStream_t)stream;
+  float* buf = nullptr;
+  hipError_t e = hipMalloc(&buf, knots * (rbd::kInPerKnot + rbd::kOutPerKnot) * sizeof(float));
+  if (e != hipSuccess) return fail(RBD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  // any finite inputs do: the work per knot does not depend on the values
+  e = hipMemsetAsync(buf, 0, knots * rbd::kInPerKnot * sizeof(float), st);
+  rbd_batch* b = nullptr;
+  rbd_status rs = e == hipSuccess ? rbd_batch_create(ctx, max_traj, &b) : RBD_OK;
+  std::vector<rbd_traj> trajs(static_cast<size_t>(max_traj));
+  for (int t = 0; t < max_traj; ++t)
+    trajs[size_t(t)] = rbd_traj{buf + size_t(t) * kLen * rbd::kInPerKnot,
+                                buf + knots * rbd::kInPerKnot + size_t(t) * kLen * rbd::kOutPerKnot, kLen};
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess && rs == RBD_OK) e = hipEventCreate(&t0);
+  if (e == hipSuccess && rs == RBD_OK) e = hipEventCreate(&t1);
+  float best_us[2][kSizes] = {};
+  int64_t threshold = 0;
+  for (int si = 0; si < kSizes && e == hipSuccess && rs == RBD_OK; ++si) {
+    for (int li = 0; li < 2 && e == hipSuccess && rs == RBD_OK; ++li) {
+      const int lanes = li ? rbd::kLanesPerKnotGroup : 1;
+      float best = 1e30f;
+      for (int rep = 0; rep < 3 && e == hipSuccess && rs == RBD_OK; ++rep) {
+        rs = rbdi::batch_run(b, trajs.data(), counts[si], 0, stream, lanes);   // warm
+        if (rs == RBD_OK) e = hipEventRecord(t0, st);
+        for (int it = 0; it < 4 && e == hipSuccess && rs == RBD_OK; ++it)
+          rs = rbdi::batch_run(b, trajs.data(), counts[si], 0, stream, lanes);
+        if (e == hipSuccess && rs == RBD_OK) e = hipEventRecord(t1, st);
+        if (e == hipSuccess && rs == RBD_OK) e = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (e == hipSuccess && rs == RBD_OK) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e == hipSuccess && rs == RBD_OK && ms * 250.0f < best) best = ms * 250.0f;
+      }
+      best_us[li][si] = best;
+    }
+    if (e == hipSuccess && rs == RBD_OK && best_us[1][si] < best_us[0][si]) threshold = counts[si] * kLen;
+  }
+  (void)hipStreamSynchronize(st);
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  if (b) rbd_batch_destroy(b);
+  (void)hipFree(buf);
+  if (rs != RBD_OK) return rs;
+  if (e != hipSuccess) return hip_fail(e, "batch autotune");
+  ctx->batch_group_threshold.store(threshold);
+  if (threshold_out) *threshold_out = threshold;
+  if (table_us)
+    for (int i = 0; i < table_len && i < 3 * kSizes; ++i)
+      table_us[i] = (i % 3 == 0) ? float(counts[i / 3] * kLen) : best_us[(i % 3) - 1][i / 3];
+  return RBD_OK;
+}
+
+int64_t rbd_batch_group_threshold(rbd_context* ctx) { return ctx ? ctx->batch_group_threshold.load() : -1; }
+
+rbd_status rbd_set_batch_group_threshold(rbd_context* ctx, int64_t n_knots) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative threshold");
+  ctx->batch_group_threshold.store(n_knots);
+  return RBD_OK;
+}
 
 }  // extern "C"

@@ -3,18 +3,21 @@
 //
 // A batch is a list of trajectories, each with its own input/output buffers
 // and length.  A chunk is the knots one wave processes per iteration: 64 in the
-// one-lane mapping, 4 in the 16-lane-group mapping.  The persistent kernel's
-// waves take chunk indices from a queue cursor with one atomic add per wave per
-// chunk and map a chunk to its trajectory through the prefix sum of chunk
-// counts:
+// one-lane mapping, 4 in the 16-lane-group mapping.  Wave w of the persistent
+// grid takes chunk w first; after that each wave takes chunk
+// gridDim.x + atomicAdd(cursor, 1), so a batch no larger than the grid costs
+// one atomic per busy wave and idle waves none (all waves contending for one
+// address was the kernel's main overhead).  A chunk's trajectory comes from
+// the prefix sum of chunk counts:
 //   * up to 64 trajectories (SMALL): every wave scans the counts itself, one
 //     trajectory per lane, and finds a chunk's trajectory with a ballot and a
 //     population count -- no plan kernel, no memory traffic per chunk;
 //   * more: the plan kernel writes the prefix sum and the total first, and the
 //     waves binary-search it.
-// Every wave leaves the loop once the cursor passes the chunk total, which is
-// fixed before the persistent kernel starts, so the grid always drains; the
-// last wave to leave resets the cursor for the next launch.
+// The cursor is zeroed by the stream-ordered header upload before every
+// launch.  Every wave leaves the loop once its chunk index passes the total,
+// which is fixed before the persistent kernel starts, and the cursor only
+// grows, so the grid always drains.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,13 +36,7 @@ struct BatchHeader {
   int n_traj;                     // written by the host copy with the descriptors
   int chunk_knots;                // knots per chunk
   long total_chunks;              // written by the plan kernel (more than 64 trajectories)
-};
-
-// Queue counters; zero before the first launch, returned to zero by the last
-// wave of every launch.
-struct BatchQueue {
-  unsigned long long cursor;      // next chunk index
-  unsigned long long done;        // waves that have left the loop
+  unsigned long long cursor;      // dynamic chunks handed out; zeroed by the host copy
 };
 
 constexpr int kSmallBatch = 64;   // trajectories handled without the plan kernel
@@ -83,10 +80,10 @@ __global__ __launch_bounds__(kPlanThreads) void batch_plan_kernel(const TrajDesc
 // k0 + i of a 64-knot chunk; LANES = 16: lane group g takes knot k0 + g of a
 // 4-knot chunk.  One wave per workgroup.
 template <class P, int LANES, bool SMALL>
-__global__ __launch_bounds__(64) void batch_persistent_kernel(const TrajDesc* __restrict__ descs,
-                                                              const BatchHeader* __restrict__ hdr,
-                                                              BatchQueue* __restrict__ queue,
-                                                              const long* __restrict__ prefix, Options opt) {
+__global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(const TrajDesc* __restrict__ descs,
+                                                                                BatchHeader* __restrict__ hdr,
+                                                                                const long* __restrict__ prefix,
+                                                                                Options opt) {
   const int n_traj = hdr->n_traj;
   const int chunk = 64 / LANES;
   const int lane = threadIdx.x;
@@ -104,43 +101,34 @@ __global__ __launch_bounds__(64) void batch_persistent_kernel(const TrajDesc* __
   } else {
     total = hdr->total_chunks;
   }
-  for (;;) {
-    unsigned long long c = 0;
-    if (lane == 0) c = atomicAdd(&queue->cursor, 1ull);
-    c = __shfl(c, 0, 64);
-    if ((long)c >= total) break;
+  for (long c = blockIdx.x; c < total;) {
     int t;
     long base;
     if constexpr (SMALL) {
       // trajectories that end at or before chunk c are the lanes with incl <= c
-      t = __popcll(__ballot(incl <= (long)c));
+      t = __popcll(__ballot(incl <= c));
       base = __shfl(incl, t > 0 ? t - 1 : 0, 64);
       if (t == 0) base = 0;
     } else {
       int lo = 0, hi = n_traj;               // prefix[lo] <= c < prefix[hi]
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= (long)c) lo = mid; else hi = mid;
+        if (prefix[mid] <= c) lo = mid; else hi = mid;
       }
       t = lo;
       base = prefix[lo];
     }
     const TrajDesc d = descs[t];
-    const long k = ((long)c - base) * chunk + slot;
+    const long k = (c - base) * chunk + slot;
     if (k < d.n) {
       InAoS i{d.in + kInPerKnot * k};
       OutAoS o{d.out + kOutPerKnot * k};
       if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
       else fdgrad_knot_lanes<P>(i, o, opt, role);
     }
-  }
-  // the last wave out returns the queue to zero for the next launch
-  if (lane == 0) {
-    const unsigned long long d = atomicAdd(&queue->done, 1ull);
-    if (d == (unsigned long long)gridDim.x - 1) {
-      atomicExch(&queue->cursor, 0ull);
-      atomicExch(&queue->done, 0ull);
-    }
+    unsigned long long next = 0;
+    if (lane == 0) next = atomicAdd(&hdr->cursor, 1ull);
+    c = long(gridDim.x) + long(__shfl(next, 0, 64));
   }
 }
 

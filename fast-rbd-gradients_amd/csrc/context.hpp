@@ -15,11 +15,14 @@ namespace rbdi {
 
 constexpr int kBlock = 64;   // one wave per workgroup
 
-// Below this many knots a call uses the 16-lane-group mapping (when the
+// Up to this many knots a call uses the 16-lane-group mapping (when the
 // context's lanes_per_knot is 0): the crossover measured on MI355X by
-// rbd_autotune (one-lane 22.6 us vs lane-group 16.1 us at 8192 knots,
-// 23.1 vs 29.2 us at 16384).
-constexpr int64_t kDefaultGroupThreshold = 8192;
+// rbd_autotune (one-lane 22.8 us vs lane-group 12.5 us at 8192 knots,
+// 23.3 vs 22.0 us at 16384, 24.9 vs 39.2 us at 32768).
+constexpr int64_t kDefaultGroupThreshold = 16384;
+// The same crossover for batches (rbd_batch_*, total knots of the batch),
+// measured by rbd_autotune_batch on batches of 132-knot trajectories.
+constexpr int64_t kDefaultBatchGroupThreshold = 4096;
 
 // Records msg as the calling thread's last error and returns s.
 rbd_status fail(rbd_status s, const std::string& msg);
@@ -37,6 +40,7 @@ struct FlatCall {
   int64_t ld_out;         // SoA only
   rbd::Options opt;
   hipStream_t stream;
+  int simds = 0;          // SIMDs of the device (4 per CU); 0: unknown
 };
 
 hipError_t launch_flat(int precision, const FlatCall& c);
@@ -49,6 +53,7 @@ struct rbd_context {
   int precision = RBD_PREC_FP32;
   int lanes_per_knot = 0;                        // 0 = by batch size, 1, or 16
   std::atomic<int64_t> group_threshold{rbdi::kDefaultGroupThreshold};
+  std::atomic<int64_t> batch_group_threshold{rbdi::kDefaultBatchGroupThreshold};
   int compute_units = 0;                         // of the device
   rbd::Options opt;
   // device buffer of the host-pointer convenience call
@@ -63,6 +68,16 @@ inline int lanes_for(const rbd_context* ctx, int64_t n) {
   if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
   return n <= ctx->group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
 }
+
+inline int lanes_for_batch(const rbd_context* ctx, int64_t total_knots) {
+  if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
+  return total_knots <= ctx->batch_group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
+}
+
+// One batch with a given mapping (batch.hip); rbd_fdgrad_batched_f32 is this
+// with lanes_for_batch.
+rbd_status batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph, rbd_stream_t stream,
+                     int lanes);
 
 inline rbd_status check_ctx(const rbd_context* ctx) {
   if (!ctx) return fail(RBD_ERR_INVALID_ARGUMENT, "null context");

@@ -235,6 +235,13 @@ using PrecFP64Pivot = Precision<float, double, float, float>;    // FP64 pivots 
 using PrecFP64Col   = Precision<float, float, float, double>;    // FP64 derivative columns
 using PrecFP64      = Precision<double, double, double, double>; // all FP64
 
+// Waves each SIMD keeps resident for the kernels of precision P.  With FP32
+// RNEA and columns a knot's state fits in 256 registers per lane, so two waves
+// share a SIMD and each hides the other's latency (1.65x the throughput of one
+// wave at 2^20 knots); with FP64 columns it needs about 500 and runs one.
+template <class P>
+constexpr int kWavesPerSimd = (sizeof(typename P::Col) == 4 && sizeof(typename P::Rnea) == 4) ? 2 : 1;
+
 // ---------------------------------------------------------------------------
 // Minv = M^-1 (upper triangle, Mi[r][c] for r <= c) by the articulated-body
 // recursion (computeMinv, minv.h:423-478; equation numbers as in its comments).

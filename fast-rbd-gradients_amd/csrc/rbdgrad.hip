@@ -45,7 +45,7 @@ using rbdi::kBlock;
 namespace rbd {
 
 template <class P>
-__global__ __launch_bounds__(kBlock) void fdgrad_aos_kernel(long n, const float* __restrict__ in,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_aos_kernel(long n, const float* __restrict__ in,
                                                             float* __restrict__ out, Options opt) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kBlock) void fdgrad_aos_kernel(long n, const float*
 }
 
 template <class P>
-__global__ __launch_bounds__(kBlock) void fdgrad_soa_kernel(long n, const float* __restrict__ in, long ld_in,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_soa_kernel(long n, const float* __restrict__ in, long ld_in,
                                                             float* __restrict__ out, long ld_out, Options opt) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void fdgrad_soa_kernel(long n, const float*
 }
 
 template <class P>
-__global__ __launch_bounds__(kBlock) void fdgrad_refmode_kernel(long n, const float* __restrict__ vol,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_refmode_kernel(long n, const float* __restrict__ vol,
                                                                 float* __restrict__ dqdd, Options opt) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
@@ -80,8 +80,11 @@ __global__ __launch_bounds__(kBlock) void fdgrad_refmode_kernel(long n, const fl
 // One knot per group of kLanesPerKnotGroup lanes (4 knots per wave): the
 // low-latency mapping for small batches.  Every lane of a group runs the
 // shared steps, then one derivative column; the group's lanes finish together.
-template <class P>
-__global__ __launch_bounds__(kBlock) void fdgrad_aos_lanes_kernel(long n, const float* __restrict__ in,
+// W: waves per SIMD the kernel is compiled for.  W = 1 lets the compiler use
+// up to 512 registers (no spills: the latency variant, for calls whose waves
+// do not outnumber the SIMDs); W = kWavesPerSimd<P> is the throughput variant.
+template <class P, int W>
+__global__ __launch_bounds__(kBlock, W) void fdgrad_aos_lanes_kernel(long n, const float* __restrict__ in,
                                                                   float* __restrict__ out, Options opt) {
   const int role = threadIdx.x % kLanesPerKnotGroup;
   const long groups = long(gridDim.x) * (blockDim.x / kLanesPerKnotGroup);
@@ -92,8 +95,8 @@ __global__ __launch_bounds__(kBlock) void fdgrad_aos_lanes_kernel(long n, const 
   }
 }
 
-template <class P>
-__global__ __launch_bounds__(kBlock) void fdgrad_soa_lanes_kernel(long n, const float* __restrict__ in, long ld_in,
+template <class P, int W>
+__global__ __launch_bounds__(kBlock, W) void fdgrad_soa_lanes_kernel(long n, const float* __restrict__ in, long ld_in,
                                                                   float* __restrict__ out, long ld_out, Options opt) {
   const int role = threadIdx.x % kLanesPerKnotGroup;
   const long groups = long(gridDim.x) * (blockDim.x / kLanesPerKnotGroup);
@@ -104,8 +107,8 @@ __global__ __launch_bounds__(kBlock) void fdgrad_soa_lanes_kernel(long n, const 
   }
 }
 
-template <class P>
-__global__ __launch_bounds__(kBlock) void fdgrad_refmode_lanes_kernel(long n, const float* __restrict__ vol,
+template <class P, int W>
+__global__ __launch_bounds__(kBlock, W) void fdgrad_refmode_lanes_kernel(long n, const float* __restrict__ vol,
                                                                       float* __restrict__ dqdd, Options opt) {
   const int role = threadIdx.x % kLanesPerKnotGroup;
   const long groups = long(gridDim.x) * (blockDim.x / kLanesPerKnotGroup);
@@ -118,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void fdgrad_refmode_lanes_kernel(long n, co
 
 // Full path plus dqdd/du = Minv (49 per knot, dense), both mappings.
 template <class P, int LANES>
-__global__ __launch_bounds__(kBlock) void fdgrad_minv_kernel(long n, const float* __restrict__ in,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_minv_kernel(long n, const float* __restrict__ in,
                                                              float* __restrict__ out, float* __restrict__ minv,
                                                              Options opt) {
   const int role = threadIdx.x % LANES;
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void fdgrad_minv_kernel(long n, const float
 // Full path with external link forces: knot k reads its 42 forces at
 // fext + k * fext_stride (stride 0: one set for every knot).  Both mappings.
 template <class P, int LANES>
-__global__ __launch_bounds__(kBlock) void fdgrad_fext_kernel(long n, const float* __restrict__ in,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_fext_kernel(long n, const float* __restrict__ in,
                                                              const float* __restrict__ fext, long fext_stride,
                                                              float* __restrict__ out, Options opt) {
   const int role = threadIdx.x % LANES;
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void fdgrad_fext_kernel(long n, const float
 // Forward dynamics only (qdd and dense Minv), one lane per knot; FEXT: with
 // external link forces as above.
 template <class P, bool FEXT>
-__global__ __launch_bounds__(kBlock) void fd_kernel(long n, const float* __restrict__ in,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fd_kernel(long n, const float* __restrict__ in,
                                                     const float* __restrict__ fext, long fext_stride,
                                                     float* __restrict__ qdd, float* __restrict__ minv, Options opt) {
   const long stride = long(gridDim.x) * blockDim.x;
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void fd_kernel(long n, const float* __restr
 // 28 per knot, the fused layout) or from q, qd and precomputed v, a, f
 // (SPLIT = true: 154 per knot, the split layout).  Both mappings.
 template <class P, int LANES, bool SPLIT>
-__global__ __launch_bounds__(kBlock) void id_gradient_kernel(long n, const float* __restrict__ vol,
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void id_gradient_kernel(long n, const float* __restrict__ vol,
                                                              float* __restrict__ cdu, Options opt) {
   const int role = threadIdx.x % LANES;
   const long groups = long(gridDim.x) * (blockDim.x / LANES);
@@ -187,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void id_gradient_kernel(long n, const float
 // v, a, f of every knot of a split-layout buffer, written in place after its
 // q, qd, u, qdd.  One lane per knot.
 template <class P>
-__global__ __launch_bounds__(kBlock) void vaf_kernel(long n, float* vol, Options opt) {
+__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void vaf_kernel(long n, float* vol, Options opt) {
   const long stride = long(gridDim.x) * blockDim.x;
   for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
     InAoS i{vol + kSplitPerKnot * k};
@@ -219,6 +222,23 @@ unsigned grid_for(int64_t n, int lanes) {
   return unsigned(blocks < cap ? blocks : cap);
 }
 
+template <class P, int W>
+void launch_lanes(const FlatCall& c, dim3 grid, dim3 block) {
+  const long n = long(c.n);
+  switch (c.layout) {
+    case Layout::AoS:
+      hipLaunchKernelGGL((rbd::fdgrad_aos_lanes_kernel<P, W>), grid, block, 0, c.stream, n, c.in, c.out, c.opt);
+      break;
+    case Layout::SoA:
+      hipLaunchKernelGGL((rbd::fdgrad_soa_lanes_kernel<P, W>), grid, block, 0, c.stream, n, c.in, long(c.ld_in),
+                         c.out, long(c.ld_out), c.opt);
+      break;
+    case Layout::Ref:
+      hipLaunchKernelGGL((rbd::fdgrad_refmode_lanes_kernel<P, W>), grid, block, 0, c.stream, n, c.in, c.out, c.opt);
+      break;
+  }
+}
+
 template <class P>
 hipError_t launch_flat_p(const FlatCall& c) {
   const dim3 grid(grid_for(c.n, c.lanes)), block(kBlock);
@@ -237,18 +257,9 @@ hipError_t launch_flat_p(const FlatCall& c) {
         break;
     }
   } else {
-    switch (c.layout) {
-      case Layout::AoS:
-        hipLaunchKernelGGL(rbd::fdgrad_aos_lanes_kernel<P>, grid, block, 0, c.stream, n, c.in, c.out, c.opt);
-        break;
-      case Layout::SoA:
-        hipLaunchKernelGGL(rbd::fdgrad_soa_lanes_kernel<P>, grid, block, 0, c.stream, n, c.in, long(c.ld_in), c.out,
-                           long(c.ld_out), c.opt);
-        break;
-      case Layout::Ref:
-        hipLaunchKernelGGL(rbd::fdgrad_refmode_lanes_kernel<P>, grid, block, 0, c.stream, n, c.in, c.out, c.opt);
-        break;
-    }
+    const int64_t waves = (c.n * c.lanes + 63) / 64;
+    if (c.simds > 0 && waves <= c.simds) launch_lanes<P, 1>(c, grid, block);
+    else launch_lanes<P, rbd::kWavesPerSimd<P>>(c, grid, block);
   }
   return hipGetLastError();
 }
@@ -348,7 +359,8 @@ rbd_status rbd_fdgrad_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, f
   if (n == 0) return RBD_OK;
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (rbd_status s = use_device(ctx)) return s;
-  const FlatCall c{Layout::AoS, lanes_for(ctx, n), n, d_in, 0, d_out, 0, ctx->opt, (hipStream_t)stream};
+  const FlatCall c{Layout::AoS, lanes_for(ctx, n), n, d_in, 0, d_out, 0, ctx->opt,
+                   (hipStream_t)stream, 4 * ctx->compute_units};
   hipError_t e = launch_flat(ctx->precision, c);
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
 }
@@ -361,7 +373,8 @@ rbd_status rbd_fdgrad_traj_soa_f32(rbd_context* ctx, int64_t n, const float* d_i
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (ld_in < n || ld_out < n) return fail(RBD_ERR_INVALID_ARGUMENT, "leading dimension smaller than knot count");
   if (rbd_status s = use_device(ctx)) return s;
-  const FlatCall c{Layout::SoA, lanes_for(ctx, n), n, d_in, ld_in, d_out, ld_out, ctx->opt, (hipStream_t)stream};
+  const FlatCall c{Layout::SoA, lanes_for(ctx, n), n, d_in, ld_in, d_out, ld_out, ctx->opt,
+                   (hipStream_t)stream, 4 * ctx->compute_units};
   hipError_t e = launch_flat(ctx->precision, c);
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
 }
@@ -374,7 +387,8 @@ rbd_status rbd_dynamics_gradient_kernel_f32(rbd_context* ctx, int64_t n, float* 
   if (n == 0) return RBD_OK;
   if (!d_dqdd || !d_mem_vol) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (rbd_status s = use_device(ctx)) return s;
-  const FlatCall c{Layout::Ref, lanes_for(ctx, n), n, d_mem_vol, 0, d_dqdd, 0, ctx->opt, (hipStream_t)stream};
+  const FlatCall c{Layout::Ref, lanes_for(ctx, n), n, d_mem_vol, 0, d_dqdd, 0, ctx->opt,
+                   (hipStream_t)stream, 4 * ctx->compute_units};
   hipError_t e = launch_flat(ctx->precision, c);
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
 }
@@ -400,7 +414,8 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
   float* d_out = ctx->d_scratch + size_t(n) * rbd::kInPerKnot;
   hipError_t e = hipMemcpy(d_in, h_in, in_bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
-  const FlatCall c{Layout::AoS, lanes_for(ctx, n), n, d_in, 0, d_out, 0, ctx->opt, (hipStream_t)0};
+  const FlatCall c{Layout::AoS, lanes_for(ctx, n), n, d_in, 0, d_out, 0, ctx->opt,
+                   (hipStream_t)0, 4 * ctx->compute_units};
   e = launch_flat(ctx->precision, c);
   if (e != hipSuccess) return hip_fail(e, "fdgrad kernel launch");
   e = hipMemcpy(h_out, d_out, out_bytes, hipMemcpyDeviceToHost);
@@ -640,7 +655,7 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
   for (int si = 0; si < kSizes && e == hipSuccess; ++si) {
     for (int li = 0; li < 2 && e == hipSuccess; ++li) {
       const FlatCall c{Layout::AoS, li ? rbd::kLanesPerKnotGroup : 1, sizes[si], in, 0, out, 0, ctx->opt,
-                       (hipStream_t)stream};
+                       (hipStream_t)stream, 4 * ctx->compute_units};
       float best = 1e30f;
       for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
         e = launch_flat(ctx->precision, c);                       // warm

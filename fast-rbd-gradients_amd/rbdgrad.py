@@ -37,6 +37,7 @@ EXPORTED = [
     "rbd_abi_version", "rbd_fdgrad_traj_f32", "rbd_fdgrad_traj_soa_f32",
     "rbd_dynamics_gradient_kernel_f32", "rbd_fdgrad_traj_host_f32", "rbd_kernel_name",
     "rbd_autotune", "rbd_group_threshold", "rbd_set_group_threshold",
+    "rbd_autotune_batch", "rbd_batch_group_threshold", "rbd_set_batch_group_threshold",
     "rbd_batch_create", "rbd_batch_destroy", "rbd_fdgrad_batched_f32", "rbd_batch_set_grid",
     "rbd_batch_last_lanes",
     "rbd_server_config_default", "rbd_server_create", "rbd_server_destroy", "rbd_server_submit",
@@ -121,6 +122,10 @@ def lib():
         L.rbd_group_threshold.argtypes = [vp]
         L.rbd_group_threshold.restype = i64
         L.rbd_set_group_threshold.argtypes = [vp, i64]
+        L.rbd_autotune_batch.argtypes = [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+        L.rbd_batch_group_threshold.argtypes = [vp]
+        L.rbd_batch_group_threshold.restype = i64
+        L.rbd_set_batch_group_threshold.argtypes = [vp, i64]
         L.rbd_batch_create.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
         L.rbd_batch_destroy.argtypes = [vp]
         L.rbd_fdgrad_batched_f32.argtypes = [vp, ctypes.POINTER(Traj), ctypes.c_int, ctypes.c_int, vp]
@@ -336,6 +341,23 @@ class Context:
         _check(lib().rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 27), "rbd_autotune")
         table = {int(tab[i]): (tab[i + 1], tab[i + 2]) for i in range(0, 27, 3)}
         return thr.value, table
+
+    def autotune_batch(self, stream=None):
+        """Time both mappings of the batched path; sets and returns its crossover
+        (total knots) plus the table {total knots: (us one-lane, us lane-group)}."""
+        thr = ctypes.c_int64()
+        tab = (ctypes.c_float * 21)()
+        _check(lib().rbd_autotune_batch(self._h, _sh(stream), ctypes.byref(thr), tab, 21), "rbd_autotune_batch")
+        table = {int(tab[i]): (tab[i + 1], tab[i + 2]) for i in range(0, 21, 3)}
+        return thr.value, table
+
+    @property
+    def batch_group_threshold(self):
+        return lib().rbd_batch_group_threshold(self._h)
+
+    @batch_group_threshold.setter
+    def batch_group_threshold(self, n):
+        _check(lib().rbd_set_batch_group_threshold(self._h, int(n)), "rbd_set_batch_group_threshold")
 
     @property
     def group_threshold(self):

@@ -214,6 +214,17 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
 int64_t rbd_group_threshold(rbd_context* ctx);
 rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots);
 
+/* The batched path (rbd_fdgrad_batched_f32) has its own crossover on a
+ * batch's total knots: its persistent kernel pays for the queue, so the
+ * one-lane mapping wins earlier.  rbd_autotune_batch times both mappings on
+ * batches of 8 ... 512 trajectories of 132 knots and sets it; the table holds
+ * (total knots, us one-lane, us lane-group) triples.  Same conditions as
+ * rbd_autotune.  No reference counterpart. */
+rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
+                              int table_len);
+int64_t rbd_batch_group_threshold(rbd_context* ctx);
+rbd_status rbd_set_batch_group_threshold(rbd_context* ctx, int64_t n_knots);
+
 /* ------------------------------------------------------------------------
  * Batched trajectories of uneven length.
  * A batch object owns a device work queue (trajectory descriptors, chunk
