@@ -192,10 +192,32 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000), duration_s=1.
     for rate in rates:
         log(f"[c3] served: Poisson {rate} req/s for {duration_s} s")
         st = srv.run_poisson(rate, duration_s, [16, 32, 64, 128, 256], pool, seed=rate)
+        st["batch_phases_us"] = batch_phases(srv.trace())
         res[f"served_{rate}_req_per_s"] = st
         log(f"[c3] {st}")
     srv.close()
     return res
+
+
+def batch_phases(tr):
+    """Where a served batch's time goes, from the server's per-batch trace:
+    queued (oldest request's submit -> admission), enqueue (admission -> all
+    device work enqueued), device (enqueued -> event seen), deliver (event ->
+    results in the callers' buffers); p50/p99/max of each, and the phases of
+    the batch with the longest submit-to-delivery time."""
+    if len(tr["admit_us"]) == 0:
+        return {}
+    phases = {"queued": tr["admit_us"] - tr["oldest_submit_us"],
+              "enqueue": tr["enqueued_us"] - tr["admit_us"],
+              "device": tr["done_us"] - tr["enqueued_us"],
+              "deliver": tr["delivered_us"] - tr["done_us"]}
+    out = {k: {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99)), "max": float(v.max())}
+           for k, v in phases.items()}
+    worst = int(np.argmax(tr["delivered_us"] - tr["oldest_submit_us"]))
+    out["worst_batch"] = {k: float(v[worst]) for k, v in phases.items()}
+    out["worst_batch"].update(requests=int(tr["requests"][worst]), knots=int(tr["knots"][worst]),
+                              at_us=float(tr["oldest_submit_us"][worst] - tr["oldest_submit_us"][0]))
+    return out
 
 
 def profile_one(args, stream):
@@ -213,6 +235,11 @@ def profile_one(args, stream):
     if args.only == "c2":
         ctx = rbdgrad.Context(device=0, precision=args.precision)
         res = mixed_workload(ctx, stream, steps=args.steps)
+        log(json.dumps(res))
+        return
+    if args.only == "c3":
+        ctx = rbdgrad.Context(device=0, precision=args.precision)
+        res = served_workload(ctx)
         log(json.dumps(res))
         return
     lanes = 1 if args.only in ("c1-one-lane", "c4-aos", "c4-soa") else 0
@@ -235,7 +262,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time per thread count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the large-batch and mixed workloads")
-    ap.add_argument("--only", choices=["c1", "c1-one-lane", "c4-aos", "c4-soa", "c2", "copy"],
+    ap.add_argument("--only", choices=["c1", "c1-one-lane", "c4-aos", "c4-soa", "c2", "c3", "copy"],
                     help="run one workload only and print its timing (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -13,6 +13,8 @@
 
 using namespace rbdi;
 
+static_assert(rbd::kSmallBatch == kSmallBatchTrajectories, "one small-batch limit");
+
 struct rbd_batch {
   rbd_context* ctx = nullptr;
   int max_traj = 0;
@@ -40,34 +42,56 @@ namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+constexpr size_t kHeaderBytes = 256;   // the header's slot before the descriptors
+
+// Where a launch finds its queue: header and descriptors (the batch's own, or
+// ones a caller staged in its device memory) and the prefix-sum scratch.
+struct Queue {
+  rbd::BatchHeader* hdr;
+  rbd::TrajDesc* descs;
+  long* prefix;
+};
+
+Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, b->d_prefix}; }
+
 template <class P, int LANES, bool SMALL>
-void launch_persistent(rbd_batch* b, hipStream_t s) {
+void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
   const unsigned grid = b->grid[LANES == 1 ? 0 : 1];
-  hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, SMALL>), dim3(grid), dim3(kBlock), 0, s, b->d_desc,
-                     b->d_hdr, b->d_prefix, b->ctx->opt);
+  hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, SMALL>), dim3(grid), dim3(kBlock), 0, s, q.descs,
+                     q.hdr, q.prefix, b->ctx->opt);
 }
 
 template <class P>
-hipError_t launch_batch_p(rbd_batch* b, int lanes, bool small, hipStream_t s) {
+hipError_t launch_batch_p(rbd_batch* b, const Queue& q, int lanes, bool small, hipStream_t s) {
   if (!small)
-    hipLaunchKernelGGL(rbd::batch_plan_kernel, dim3(1), dim3(rbd::kPlanThreads), 0, s, b->d_desc, b->d_hdr,
-                       b->d_prefix);
+    hipLaunchKernelGGL(rbd::batch_plan_kernel, dim3(1), dim3(rbd::kPlanThreads), 0, s, q.descs, q.hdr, q.prefix);
   if (lanes == 1) {
-    if (small) launch_persistent<P, 1, true>(b, s); else launch_persistent<P, 1, false>(b, s);
+    if (small) launch_persistent<P, 1, true>(b, q, s); else launch_persistent<P, 1, false>(b, q, s);
   } else {
-    if (small) launch_persistent<P, rbd::kLanesPerKnotGroup, true>(b, s);
-    else launch_persistent<P, rbd::kLanesPerKnotGroup, false>(b, s);
+    if (small) launch_persistent<P, rbd::kLanesPerKnotGroup, true>(b, q, s);
+    else launch_persistent<P, rbd::kLanesPerKnotGroup, false>(b, q, s);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_batch(rbd_batch* b, int lanes, bool small, hipStream_t s) {
+hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hipStream_t s) {
   switch (b->ctx->precision) {
-    case RBD_PREC_FP32: return launch_batch_p<rbd::PrecFP32>(b, lanes, small, s);
-    case RBD_PREC_FP64_PIVOT: return launch_batch_p<rbd::PrecFP64Pivot>(b, lanes, small, s);
-    case RBD_PREC_FP64_COLUMNS: return launch_batch_p<rbd::PrecFP64Col>(b, lanes, small, s);
-    default: return launch_batch_p<rbd::PrecFP64>(b, lanes, small, s);
+    case RBD_PREC_FP32: return launch_batch_p<rbd::PrecFP32>(b, q, lanes, small, s);
+    case RBD_PREC_FP64_PIVOT: return launch_batch_p<rbd::PrecFP64Pivot>(b, q, lanes, small, s);
+    case RBD_PREC_FP64_COLUMNS: return launch_batch_p<rbd::PrecFP64Col>(b, q, lanes, small, s);
+    default: return launch_batch_p<rbd::PrecFP64>(b, q, lanes, small, s);
   }
+}
+
+// header + descriptors of a batch into host memory at dst (kHeaderBytes, then
+// the descriptors)
+void stage(char* dst, const rbd_traj* trajs, int n_traj, int lanes) {
+  rbd::BatchHeader* h = reinterpret_cast<rbd::BatchHeader*>(dst);
+  std::memset(h, 0, sizeof(*h));             // cursor = 0
+  h->n_traj = n_traj;
+  h->chunk_knots = 64 / lanes;
+  rbd::TrajDesc* d = reinterpret_cast<rbd::TrajDesc*>(dst + kHeaderBytes);
+  for (int t = 0; t < n_traj; ++t) d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
 }
 
 int graph_index(int lanes, bool small) { return 2 * (lanes == 1 ? 0 : 1) + (small ? 1 : 0); }
@@ -81,7 +105,7 @@ rbd_status ensure_graph(rbd_batch* b, int lanes, bool small) {
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate (capture)");
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t le = launch_batch(b, lanes, small, cs);
+    hipError_t le = launch_batch(b, own_queue(b), lanes, small, cs);
     e = hipStreamEndCapture(cs, &b->graph[gi]);
     if (le != hipSuccess) e = le;
   }
@@ -111,7 +135,8 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
   if (!b) return fail(RBD_ERR_OUT_OF_MEMORY, "host allocation failed");
   b->ctx = ctx;
   b->max_traj = max_trajectories;
-  const size_t hdr = align_up(sizeof(rbd::BatchHeader), 256);
+  static_assert(sizeof(rbd::BatchHeader) <= kHeaderBytes, "header slot");
+  const size_t hdr = kHeaderBytes;
   const size_t desc = align_up(sizeof(rbd::TrajDesc) * size_t(max_trajectories), 256);
   const size_t pre = sizeof(long) * (size_t(max_trajectories) + 1);
   b->upload_bytes = hdr + sizeof(rbd::TrajDesc) * size_t(max_trajectories);
@@ -201,15 +226,8 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   b->ring = (r + 1) % rbd_batch::kRing;
   hipError_t e = hipEventSynchronize(b->staged[r]);     // previous copy out of this slot is done
   if (e != hipSuccess) return hip_fail(e, "staging wait");
-  const size_t hdr = align_up(sizeof(rbd::BatchHeader), 256);
-  rbd::BatchHeader* h = reinterpret_cast<rbd::BatchHeader*>(b->h_stage[r]);
-  std::memset(h, 0, sizeof(*h));             // cursor = 0
-  h->n_traj = n_traj;
-  h->chunk_knots = 64 / lanes;
-  rbd::TrajDesc* d = reinterpret_cast<rbd::TrajDesc*>(b->h_stage[r] + hdr);
-  for (int t = 0; t < n_traj; ++t) d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
-  e = hipMemcpyAsync(b->d_mem, b->h_stage[r], hdr + sizeof(rbd::TrajDesc) * size_t(n_traj), hipMemcpyHostToDevice,
-                     st);
+  stage(b->h_stage[r], trajs, n_traj, lanes);
+  e = hipMemcpyAsync(b->d_mem, b->h_stage[r], batch_stage_bytes(n_traj), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipEventRecord(b->staged[r], st);
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
   const bool small = n_traj <= rbd::kSmallBatch;
@@ -218,10 +236,32 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
     e = hipGraphLaunch(b->exec[graph_index(lanes, small)], st);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   } else {
-    e = launch_batch(b, lanes, small, st);
+    e = launch_batch(b, own_queue(b), lanes, small, st);
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   }
   return RBD_OK;
+}
+
+size_t rbdi::batch_stage_bytes(int n_traj) { return kHeaderBytes + sizeof(rbd::TrajDesc) * size_t(n_traj); }
+
+int64_t rbdi::batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int lanes) {
+  stage(static_cast<char*>(host_dst), trajs, n_traj, lanes);
+  int64_t total = 0;
+  for (int t = 0; t < n_traj; ++t) total += trajs[t].n_knots;
+  return total;
+}
+
+rbd_status rbdi::batch_launch_staged(rbd_batch* b, void* d_staged, int n_traj, int64_t total_knots, int lanes,
+                                     rbd_stream_t stream) {
+  if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
+  b->last_total_knots = total_knots;
+  b->last_lanes = lanes;
+  if (total_knots == 0) return RBD_OK;
+  char* base = static_cast<char*>(d_staged);
+  const Queue q{reinterpret_cast<rbd::BatchHeader*>(base), reinterpret_cast<rbd::TrajDesc*>(base + kHeaderBytes),
+                b->d_prefix};
+  hipError_t e = launch_batch(b, q, lanes, n_traj <= rbd::kSmallBatch, (hipStream_t)stream);
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "batch kernel launch");
 }
 
 extern "C" {

@@ -23,6 +23,9 @@ constexpr int64_t kDefaultGroupThreshold = 16384;
 // The same crossover for batches (rbd_batch_*, total knots of the batch),
 // measured by rbd_autotune_batch on batches of 132-knot trajectories.
 constexpr int64_t kDefaultBatchGroupThreshold = 4096;
+// Batches of up to this many trajectories find a chunk's trajectory inside the
+// wave, larger ones through the plan kernel (rbd::kSmallBatch, batch_kernels.hpp).
+constexpr int kSmallBatchTrajectories = 64;
 
 // Records msg as the calling thread's last error and returns s.
 rbd_status fail(rbd_status s, const std::string& msg);
@@ -78,6 +81,17 @@ inline int lanes_for_batch(const rbd_context* ctx, int64_t total_knots) {
 // with lanes_for_batch.
 rbd_status batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph, rbd_stream_t stream,
                      int lanes);
+
+// For callers that upload a batch's header and descriptors together with its
+// inputs (the server): batch_stage writes them (batch_stage_bytes(n_traj)
+// bytes, trajectory pointers as given) to host memory and returns the total
+// knots; after the caller has copied that block to device memory d_staged
+// (256-byte aligned, writable: the queue cursor lives there), on `stream`,
+// batch_launch_staged runs the batch from it.
+size_t batch_stage_bytes(int n_traj);
+int64_t batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int lanes);
+rbd_status batch_launch_staged(rbd_batch* b, void* d_staged, int n_traj, int64_t total_knots, int lanes,
+                               rbd_stream_t stream);
 
 inline rbd_status check_ctx(const rbd_context* ctx) {
   if (!ctx) return fail(RBD_ERR_INVALID_ARGUMENT, "null context");

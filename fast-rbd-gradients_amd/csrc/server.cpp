@@ -4,17 +4,20 @@
 // A request is one trajectory: host input n x 21, host output n x 105, both
 // caller-owned and valid until the request completes.  The dispatcher thread
 // admits queued requests in arrival order while the fixed memory budget
-// allows: a batch holds one contiguous range [inputs of all its requests |
-// outputs of all its requests] (n x 504 bytes per request, the two halves
-// rounded up to 256 bytes) of device memory, and the same range of pinned
-// staging, until its results are delivered.  It runs the batch on a free
-// in-flight slot (own stream, own rbd_batch work queue): inputs -> pinned
-// staging, one H2D copy, one persistent-kernel launch, one D2H copy, an event.  While batches
-// are on the device the dispatcher keeps admitting newly arrived requests into
-// the next free slot, so work from several batches overlaps on the GPU.  The
-// completion thread waits for slots in launch order, copies results to the
-// callers' buffers, returns the memory to the budget and records each
-// request's latency (submit -> result in the caller's buffer).
+// allows: a batch holds one contiguous range [queue header and trajectory
+// descriptors | inputs of all its requests | outputs of all its requests]
+// (n x 504 bytes per request plus 32 per descriptor, each part rounded up to
+// 256 bytes) of device memory, and the same range of pinned staging, until its
+// results are delivered.  It runs the batch on a free in-flight slot (own
+// stream, own rbd_batch work queue): descriptors and inputs -> pinned staging,
+// one H2D copy, one persistent-kernel launch, one D2H copy, an event.  While
+// batches are on the device the dispatcher keeps admitting newly arrived
+// requests into the next free slot, so work from several batches overlaps on
+// the GPU.  Each slot has a completion thread: it polls the slot's event,
+// copies results to the callers' buffers, returns the memory to the budget,
+// frees the slot and records each request's latency (submit -> result in the
+// caller's buffer).  Slots complete independently, so copying out one large
+// batch does not hold up the others.
 //
 // Every batch of a server runs with the same knot-to-lane mapping (fixed at
 // creation), so a request's result does not depend on what it was batched with.
@@ -113,9 +116,16 @@ struct Slot {
   rbd_batch* batch = nullptr;
   std::vector<std::shared_ptr<Request>> reqs;
   size_t off = 0, bytes = 0;            // the batch's range in the pools
+  size_t stage_bytes = 0;               // header + descriptors (rounded)
   size_t in_bytes = 0, out_bytes = 0;   // packed inputs, packed outputs
-  bool busy = false;
+  bool busy = false;                    // admitted batch not yet delivered
+  bool launched = false;                // its device work is enqueued
+  rbd_batch_record rec{};               // trace of the batch in flight
+  std::thread completer;                // delivers this slot's batches
 };
+
+constexpr size_t kTraceRecords = 65536;
+constexpr int kWarmRounds = 512;       // warm-up batches per slot at creation
 
 }  // namespace rbdi
 
@@ -130,27 +140,33 @@ struct rbd_server {
 
   std::mutex mu;
   std::condition_variable cv_dispatch;  // new request, returned memory, freed slot, stop
-  std::condition_variable cv_complete;  // a slot was launched, stop
+  std::condition_variable cv_complete;  // a slot was launched, stop (one completion thread per slot)
   std::condition_variable cv_done;      // a request finished
   std::deque<std::shared_ptr<Request>> queue;                    // submitted, not admitted
   std::unordered_map<uint64_t, std::shared_ptr<Request>> live;    // not yet collected
-  std::deque<int> launched;                                       // slot indices in launch order
   uint64_t next_id = 1;
   uint64_t mem_gen = 0;                 // bumped whenever memory or a slot is returned
   bool stop = false;
   rbd_status async_error = RBD_OK;
 
   // statistics
+  Clock::time_point t_created = Clock::now();
+  std::vector<rbd_batch_record> trace;  // ring of the last kTraceRecords batches
+  uint64_t trace_count = 0;
   std::vector<double> latencies_us;
   int64_t knots_done = 0, requests_done = 0, batches = 0, inflight_max = 0, admitted_knots_max = 0;
   size_t budget_used_max = 0;
   Clock::time_point t_first{}, t_last{};
   bool any_done = false;
 
-  std::thread dispatcher, completer;
+  std::thread dispatcher;
 };
 
 namespace {
+
+double since(const rbd_server* s, Clock::time_point t) {
+  return std::chrono::duration<double, std::micro>(t - s->t_created).count();
+}
 
 void dispatch_loop(rbd_server* s) {
   if (use_device(s->ctx) != RBD_OK) return;
@@ -185,9 +201,12 @@ void dispatch_loop(rbd_server* s) {
     size_t in_bytes = 0, out_bytes = 0;
     const size_t room = s->alloc->largest();
     for (const auto& r : s->queue) {
-      if (int64_t(admitted.size()) >= s->cfg.max_batch_trajectories) break;
+      const int next = int(admitted.size()) + 1;
+      if (next > s->cfg.max_batch_trajectories) break;
       if (knots + r->n > s->cfg.max_batch_knots) break;
-      if (round_up(in_bytes + r->in_bytes) + round_up(out_bytes + r->out_bytes) > room) break;
+      if (round_up(batch_stage_bytes(next)) + round_up(in_bytes + r->in_bytes) + round_up(out_bytes + r->out_bytes) >
+          room)
+        break;
       r->in_off = in_bytes;                                 // relative to the batch, fixed up below
       r->out_off = out_bytes;
       in_bytes += r->in_bytes;
@@ -195,8 +214,9 @@ void dispatch_loop(rbd_server* s) {
       knots += r->n;
       admitted.push_back(r);
     }
+    const size_t stage_bytes = round_up(batch_stage_bytes(int(admitted.size())));
     size_t off = SIZE_MAX;
-    if (!admitted.empty()) off = s->alloc->alloc(round_up(in_bytes) + round_up(out_bytes));
+    if (!admitted.empty()) off = s->alloc->alloc(stage_bytes + round_up(in_bytes) + round_up(out_bytes));
     if (off == SIZE_MAX) {
       // the head request does not fit yet: sleep until memory comes back
       const uint64_t gen = s->mem_gen;
@@ -205,8 +225,8 @@ void dispatch_loop(rbd_server* s) {
     }
     s->queue.erase(s->queue.begin(), s->queue.begin() + std::ptrdiff_t(admitted.size()));
     for (auto& r : admitted) {
-      r->in_off += off;
-      r->out_off += off + round_up(in_bytes);
+      r->in_off += off + stage_bytes;
+      r->out_off += off + stage_bytes + round_up(in_bytes);
     }
     s->budget_used_max = std::max(s->budget_used_max, s->alloc->used());
     s->admitted_knots_max = std::max(s->admitted_knots_max, knots);
@@ -214,11 +234,18 @@ void dispatch_loop(rbd_server* s) {
     sl.busy = true;
     sl.reqs = std::move(admitted);
     sl.off = off;
-    sl.bytes = round_up(in_bytes) + round_up(out_bytes);
+    sl.stage_bytes = stage_bytes;
+    sl.bytes = stage_bytes + round_up(in_bytes) + round_up(out_bytes);
     sl.in_bytes = in_bytes;
     sl.out_bytes = out_bytes;
+    sl.rec = rbd_batch_record{};
+    sl.rec.oldest_submit_us = since(s, sl.reqs.front()->t_submit);
+    sl.rec.admit_us = since(s, Clock::now());
+    sl.rec.requests = int64_t(sl.reqs.size());
+    sl.rec.knots = knots;
+    sl.rec.slot = slot;
     lk.unlock();
-    // stage inputs, one H2D copy, one batched launch, one D2H copy, event
+    // stage descriptors and inputs, one H2D copy, one batched launch, one D2H copy, event
     std::vector<rbd_traj> trajs;
     trajs.reserve(sl.reqs.size());
     for (auto& r : sl.reqs) {
@@ -226,17 +253,21 @@ void dispatch_loop(rbd_server* s) {
       trajs.push_back(rbd_traj{reinterpret_cast<const float*>(s->d_pool + r->in_off),
                                reinterpret_cast<float*>(s->d_pool + r->out_off), r->n});
     }
-    const size_t out0 = off + round_up(in_bytes);
-    hipError_t e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, in_bytes, hipMemcpyHostToDevice, sl.stream);
+    const int lanes = lanes_for_batch(s->run_ctx, knots);
+    const int n_traj = int(trajs.size());
+    (void)batch_stage(s->h_pool + off, trajs.data(), n_traj, lanes);
+    const size_t out0 = off + stage_bytes + round_up(in_bytes);
+    hipError_t e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, stage_bytes + in_bytes, hipMemcpyHostToDevice,
+                                  sl.stream);
     rbd_status st = RBD_OK;
-    if (e == hipSuccess)
-      st = rbd_fdgrad_batched_f32(sl.batch, trajs.data(), int(trajs.size()), s->cfg.use_graph, sl.stream);
+    if (e == hipSuccess) st = batch_launch_staged(sl.batch, s->d_pool + off, n_traj, knots, lanes, sl.stream);
     if (e == hipSuccess && st == RBD_OK)
       e = hipMemcpyAsync(s->h_pool + out0, s->d_pool + out0, out_bytes, hipMemcpyDeviceToHost, sl.stream);
     if (e == hipSuccess && st == RBD_OK) e = hipEventRecord(sl.done, sl.stream);
+    sl.rec.enqueued_us = since(s, Clock::now());
     lk.lock();
     if (e != hipSuccess || st != RBD_OK) s->async_error = st != RBD_OK ? st : RBD_ERR_HIP;
-    s->launched.push_back(slot);
+    sl.launched = true;
     s->batches++;
     int64_t inflight = 0;
     for (const Slot& x : s->slots) inflight += x.busy ? 1 : 0;
@@ -245,22 +276,34 @@ void dispatch_loop(rbd_server* s) {
   }
 }
 
-void complete_loop(rbd_server* s) {
+// One per slot: waits for the slot's batch, delivers its results, returns its
+// memory and the slot.  Slots complete independently, so a large batch's
+// delivery does not hold up the others.
+void complete_loop(rbd_server* s, int slot) {
   if (use_device(s->ctx) != RBD_OK) return;
+  Slot& sl = s->slots[size_t(slot)];
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
-    s->cv_complete.wait(lk, [s] { return s->stop || !s->launched.empty(); });
-    if (s->launched.empty()) return;       // stop with nothing in flight
-    const int slot = s->launched.front();
-    Slot& sl = s->slots[size_t(slot)];
+    s->cv_complete.wait(lk, [s, &sl] { return s->stop || sl.launched; });
+    if (!sl.launched) return;              // stop with nothing in flight
     const bool failed_launch = s->async_error != RBD_OK;
     lk.unlock();
-    const hipError_t e = failed_launch ? hipErrorUnknown : hipEventSynchronize(sl.done);
+    // poll rather than block: a blocking wait can sleep past the event by far
+    // more than a batch takes
+    hipError_t e = failed_launch ? hipErrorUnknown : hipEventQuery(sl.done);
+    while (e == hipErrorNotReady) {
+      std::this_thread::yield();
+      e = hipEventQuery(sl.done);
+    }
+    sl.rec.done_us = since(s, Clock::now());
     if (e == hipSuccess)
       for (auto& r : sl.reqs) std::memcpy(r->h_out, s->h_pool + r->out_off, r->out_bytes);
     const auto now = Clock::now();
+    sl.rec.delivered_us = since(s, now);
     lk.lock();
-    s->launched.pop_front();
+    s->trace[size_t(s->trace_count % kTraceRecords)] = sl.rec;
+    s->trace_count++;
+    sl.launched = false;
     s->alloc->release(sl.off, sl.bytes);
     for (auto& r : sl.reqs) {
       r->status = (e == hipSuccess) ? 1 : -1;
@@ -279,6 +322,50 @@ void complete_loop(rbd_server* s) {
     s->cv_done.notify_all();
     s->cv_dispatch.notify_all();
   }
+}
+
+// Run every kernel a batch can use once on every slot (queues of 1 and of
+// kSmallBatch + 1 one-knot trajectories, zero inputs), so first-use costs of
+// the runtime fall at creation and not on the first requests.
+rbd_status warm_up(rbd_server* s) {
+  // touch the whole staging and device pools once, copies both ways included,
+  // so no page of either is first used by a request
+  const size_t pool = s->alloc->size();
+  std::memset(s->h_pool, 0, pool);
+  hipError_t e0 = hipMemcpy(s->d_pool, s->h_pool, pool, hipMemcpyHostToDevice);
+  if (e0 == hipSuccess) e0 = hipMemcpy(s->h_pool, s->d_pool, pool, hipMemcpyDeviceToHost);
+  if (e0 != hipSuccess) return hip_fail(e0, "server warm-up copies");
+  const int n_big = kSmallBatchTrajectories + 1;
+  const size_t stage = round_up(batch_stage_bytes(n_big));
+  const size_t per = round_up(size_t(n_big) * rbd::kInPerKnot * sizeof(float)) +
+                     round_up(size_t(n_big) * rbd::kOutPerKnot * sizeof(float));
+  if (stage + per > s->alloc->size()) return RBD_OK;     // budget too small to bother
+  std::memset(s->h_pool, 0, stage + per);
+  std::vector<rbd_traj> trajs(static_cast<size_t>(n_big));
+  const size_t in0 = stage, out0 = stage + round_up(size_t(n_big) * rbd::kInPerKnot * sizeof(float));
+  for (int t = 0; t < n_big; ++t)
+    trajs[size_t(t)] = rbd_traj{reinterpret_cast<const float*>(s->d_pool + in0) + size_t(t) * rbd::kInPerKnot,
+                                reinterpret_cast<float*>(s->d_pool + out0) + size_t(t) * rbd::kOutPerKnot, 1};
+  const int lanes = lanes_for_batch(s->run_ctx, 1);
+  // the full per-batch sequence, many times per slot: the runtime sizes its
+  // internal pools (signals, command buffers) on first demand
+  for (int round = 0; round < kWarmRounds; ++round)
+    for (Slot& sl : s->slots) {
+      const int n = (round & 1) ? n_big : 1;
+      (void)batch_stage(s->h_pool, trajs.data(), n, lanes);
+      hipError_t e = hipMemcpyAsync(s->d_pool, s->h_pool, stage + per, hipMemcpyHostToDevice, sl.stream);
+      if (e != hipSuccess) return hip_fail(e, "server warm-up upload");
+      if (rbd_status st = batch_launch_staged(sl.batch, s->d_pool, n, n, lanes, sl.stream)) return st;
+      e = hipMemcpyAsync(s->h_pool + out0, s->d_pool + out0, size_t(n) * rbd::kOutPerKnot * sizeof(float),
+                         hipMemcpyDeviceToHost, sl.stream);
+      if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
+      if (e != hipSuccess) return hip_fail(e, "server warm-up");
+    }
+  for (Slot& sl : s->slots) {
+    const hipError_t e = hipStreamSynchronize(sl.stream);
+    if (e != hipSuccess) return hip_fail(e, "server warm-up");
+  }
+  return RBD_OK;
 }
 
 double percentile(std::vector<double>& v, double p) {
@@ -300,7 +387,7 @@ void rbd_server_config_default(rbd_server_config* cfg) {
   cfg->slots = 4;
   cfg->memory_budget_bytes = int64_t(64) << 20;
   cfg->max_batch_trajectories = 1024;
-  cfg->max_batch_knots = 1 << 17;
+  cfg->max_batch_knots = 1 << 15;
   cfg->batch_window_us = 0.0;
   cfg->use_graph = 0;
   cfg->lanes_per_knot = rbd::kLanesPerKnotGroup;
@@ -324,6 +411,7 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
   if (!s) return fail(RBD_ERR_OUT_OF_MEMORY, "host allocation failed");
   s->ctx = ctx;
   s->cfg = c;
+  s->trace.resize(kTraceRecords);
   rbd_config rc;
   rbd_config_default(&rc);
   rc.device = ctx->device;
@@ -346,13 +434,14 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
         st = rbd_batch_create(s->run_ctx, int(c.max_batch_trajectories), &sl.batch);
     }
   }
+  if (st == RBD_OK && e == hipSuccess) st = warm_up(s);
   if (st != RBD_OK || e != hipSuccess) {
     const std::string msg = st != RBD_OK ? std::string(rbd_last_error()) : std::string(hipGetErrorString(e));
     rbd_server_destroy(s);
     return fail(st != RBD_OK ? st : RBD_ERR_OUT_OF_MEMORY, "server setup: " + msg);
   }
   s->dispatcher = std::thread(dispatch_loop, s);
-  s->completer = std::thread(complete_loop, s);
+  for (int i = 0; i < int(s->slots.size()); ++i) s->slots[size_t(i)].completer = std::thread(complete_loop, s, i);
   *out = s;
   return RBD_OK;
 }
@@ -369,7 +458,8 @@ rbd_status rbd_server_destroy(rbd_server* s) {
   s->cv_dispatch.notify_all();
   if (s->dispatcher.joinable()) s->dispatcher.join();
   s->cv_complete.notify_all();
-  if (s->completer.joinable()) s->completer.join();
+  for (Slot& sl : s->slots)
+    if (sl.completer.joinable()) sl.completer.join();
   s->cv_done.notify_all();
   (void)use_device(s->ctx);
   for (Slot& sl : s->slots) {
@@ -396,7 +486,9 @@ rbd_status rbd_server_submit(rbd_server* s, const float* h_in, float* h_out, int
   r->in_bytes = size_t(n_knots) * rbd::kInPerKnot * sizeof(float);
   r->out_bytes = size_t(n_knots) * rbd::kOutPerKnot * sizeof(float);
   r->bytes = round_up(r->in_bytes) + round_up(r->out_bytes);
-  if (r->bytes > s->alloc->size()) return fail(RBD_ERR_CAPACITY, "request larger than the server's memory budget");
+  // alone in a batch it also needs the queue header and one descriptor
+  if (r->bytes + round_up(batch_stage_bytes(1)) > s->alloc->size())
+    return fail(RBD_ERR_CAPACITY, "request larger than the server's memory budget");
   if (n_knots > s->cfg.max_batch_knots) return fail(RBD_ERR_CAPACITY, "request longer than max_batch_knots");
   r->t_submit = Clock::now();
   {
@@ -457,6 +549,17 @@ rbd_status rbd_server_get_stats(rbd_server* s, rbd_server_stats* out) {
   return RBD_OK;
 }
 
+rbd_status rbd_server_get_trace(rbd_server* s, rbd_batch_record* out, int64_t max_records, int64_t* n_out) {
+  if (!s || !out || !n_out || max_records < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const uint64_t have = std::min<uint64_t>(s->trace_count, kTraceRecords);
+  const uint64_t n = std::min<uint64_t>(have, uint64_t(max_records));
+  const uint64_t first = s->trace_count - n;
+  for (uint64_t i = 0; i < n; ++i) out[i] = s->trace[size_t((first + i) % kTraceRecords)];
+  *n_out = int64_t(n);
+  return RBD_OK;
+}
+
 rbd_status rbd_server_reset_stats(rbd_server* s) {
   if (!s) return fail(RBD_ERR_INVALID_ARGUMENT, "null server");
   std::lock_guard<std::mutex> lk(s->mu);
@@ -464,16 +567,19 @@ rbd_status rbd_server_reset_stats(rbd_server* s) {
   s->knots_done = s->requests_done = s->batches = s->inflight_max = s->admitted_knots_max = 0;
   s->budget_used_max = 0;
   s->any_done = false;
+  s->trace_count = 0;
   return RBD_OK;
 }
 
 // Open-loop load: requests arrive as a Poisson process of `rate_per_s` for
 // `duration_s`; each is one trajectory whose length is drawn uniformly from
 // lengths[0..n_lengths); its inputs come from `pool_in` (pool_knots x 21, used
-// cyclically) and results go to a private buffer.  Waits for every request,
-// then fills `out` with the server's statistics over this run.  The arrival
-// times are scheduled in advance (the generator sleeps until each), so a busy
-// server makes requests queue rather than slowing arrivals down.
+// cyclically) and results go to result buffers the generator recycles, as a
+// client would: 2048 are allocated and touched before the first arrival, more
+// only if that many are outstanding at once.  Waits for every request, then
+// fills `out` with the server's statistics over this run.  The arrival times
+// are scheduled in advance (the generator spins until each), so a busy server
+// makes requests queue rather than slowing arrivals down.
 rbd_status rbd_server_run_poisson(rbd_server* s, double rate_per_s, double duration_s, const int64_t* lengths,
                                   int n_lengths, const float* pool_in, int64_t pool_knots, uint64_t seed,
                                   rbd_server_stats* out) {
@@ -488,10 +594,16 @@ rbd_status rbd_server_run_poisson(rbd_server* s, double rate_per_s, double durat
   std::uniform_int_distribution<int> pick(0, n_lengths - 1);
   const int64_t expected = int64_t(rate_per_s * duration_s * 1.2) + 16;
   // result buffers are recycled once their request completed
+  const size_t buf_floats = size_t(maxlen) * rbd::kOutPerKnot;
   std::vector<std::unique_ptr<float[]>> outbufs;
   std::deque<std::pair<uint64_t, size_t>> pending;   // (id, outbuf index)
   std::vector<size_t> free_bufs;
   (void)expected;
+  for (size_t i = 0; i < 2048; ++i) {
+    outbufs.emplace_back(new float[buf_floats]);
+    std::memset(outbufs.back().get(), 0, buf_floats * sizeof(float));
+    free_bufs.push_back(2047 - i);
+  }
   const auto t0 = Clock::now();
   double t = 0.0;
   int64_t cursor = 0;
@@ -516,7 +628,7 @@ rbd_status rbd_server_run_poisson(rbd_server* s, double rate_per_s, double durat
     if (cursor + n > pool_knots) cursor = 0;
     size_t bi;
     if (!free_bufs.empty()) { bi = free_bufs.back(); free_bufs.pop_back(); }
-    else { outbufs.emplace_back(new float[size_t(maxlen) * rbd::kOutPerKnot]); bi = outbufs.size() - 1; }
+    else { outbufs.emplace_back(new float[buf_floats]); bi = outbufs.size() - 1; }
     uint64_t id = 0;
     st = rbd_server_submit(s, pool_in + cursor * rbd::kInPerKnot, outbufs[bi].get(), n, &id);
     if (st != RBD_OK) break;

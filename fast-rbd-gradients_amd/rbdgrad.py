@@ -42,7 +42,7 @@ EXPORTED = [
     "rbd_batch_last_lanes",
     "rbd_server_config_default", "rbd_server_create", "rbd_server_destroy", "rbd_server_submit",
     "rbd_server_poll", "rbd_server_wait", "rbd_server_get_stats", "rbd_server_reset_stats",
-    "rbd_server_run_poisson", "rbd_fdgrad_minv_traj_f32", "rbd_forward_dynamics_f32",
+    "rbd_server_run_poisson", "rbd_server_get_trace", "rbd_fdgrad_minv_traj_f32", "rbd_forward_dynamics_f32",
     "rbd_fdgrad_traj_fext_f32", "rbd_forward_dynamics_fext_f32",
     "rbd_dynamics_gradient_kernel_fused_f32", "rbd_dynamics_gradient_kernel_split_f32",
     "rbd_gradient_setup_split_f32", "rbd_gradient_finish_f32",
@@ -65,6 +65,13 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("lanes_per_knot", ctypes.c_int), ("memory_budget_bytes", ctypes.c_int64),
                 ("max_batch_trajectories", ctypes.c_int64), ("max_batch_knots", ctypes.c_int64),
                 ("batch_window_us", ctypes.c_double), ("use_graph", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+
+
+class BatchRecord(ctypes.Structure):
+    _fields_ = [("oldest_submit_us", ctypes.c_double), ("admit_us", ctypes.c_double),
+                ("enqueued_us", ctypes.c_double), ("done_us", ctypes.c_double), ("delivered_us", ctypes.c_double),
+                ("requests", ctypes.c_int64), ("knots", ctypes.c_int64), ("slot", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
 
 
 class ServerStats(ctypes.Structure):
@@ -139,6 +146,7 @@ def lib():
         L.rbd_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_double]
         L.rbd_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.rbd_server_reset_stats.argtypes = [vp]
+        L.rbd_server_get_trace.argtypes = [vp, ctypes.POINTER(BatchRecord), i64, ctypes.POINTER(i64)]
         L.rbd_server_run_poisson.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(i64), ctypes.c_int,
                                              vp, i64, ctypes.c_uint64, ctypes.POINTER(ServerStats)]
         _lib = L
@@ -574,6 +582,14 @@ class Server:
         st = ServerStats()
         _check(lib().rbd_server_get_stats(self._h, ctypes.byref(st)), "rbd_server_get_stats")
         return st.as_dict()
+
+    def trace(self, max_records=65536):
+        """The most recent batch records, oldest first, as a structured array."""
+        buf = (BatchRecord * int(max_records))()
+        n = ctypes.c_int64()
+        _check(lib().rbd_server_get_trace(self._h, buf, int(max_records), ctypes.byref(n)), "rbd_server_get_trace")
+        names = [f for f, _ in BatchRecord._fields_ if f != "reserved"]
+        return {f: np.array([getattr(buf[i], f) for i in range(n.value)]) for f in names}
 
     def reset_stats(self):
         _check(lib().rbd_server_reset_stats(self._h), "rbd_server_reset_stats")

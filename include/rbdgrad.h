@@ -250,15 +250,19 @@ rbd_status rbd_batch_destroy(rbd_batch* b);
 /* Run the forward-dynamics gradient over n_traj trajectories on `stream`.
  * The descriptors are copied to the device asynchronously on `stream`; the
  * caller's trajs array may be reused on return.  The knot-to-lane mapping is
- * chosen from the total knot count as for rbd_fdgrad_traj_f32.  use_graph != 0
- * replays the plan and persistent kernels from a hipGraph captured on first
- * use (one graph per mapping).  Returns RBD_ERR_CAPACITY if n_traj exceeds the
- * batch's max_trajectories. */
+ * chosen from the total knot count against the context's batch crossover
+ * (rbd_autotune_batch), unless the context fixes lanes_per_knot.
+ * use_graph != 0 replays the batch's kernels from a hipGraph captured on first
+ * use (one graph per mapping and batch kind).  Calls on one batch object are
+ * ordered by their streams: two streams must not run the same batch object at
+ * once.  Returns RBD_ERR_CAPACITY if n_traj exceeds the batch's
+ * max_trajectories. */
 rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph,
                                   rbd_stream_t stream);
 
-/* Persistent grid size (waves) per mapping (default: 4 x compute units, one
- * resident wave per SIMD); set by the batch autotuner in bench/tests. */
+/* Persistent grid size (waves) per mapping (default: the waves the device's
+ * SIMDs keep resident, 4 per CU times 2 with FP32 columns or 1 with FP64
+ * columns). */
 rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves);
 
 /* Lanes per knot used by the last rbd_fdgrad_batched_f32 call (1 or 16). */
@@ -268,11 +272,13 @@ int rbd_batch_last_lanes(const rbd_batch* b);
  * Serving: continuous batching of gradient requests.
  * A request is one trajectory (host input n x 21, host output n x 105; both
  * caller-owned and valid until the request completes).  A dispatcher thread
- * admits requests in arrival order while the memory budget allows (n x 504
- * bytes, rounded up to 256, of device memory plus the same of pinned staging
- * per request), packs them into batches and runs each batch on a free slot
- * (stream + batch work queue), so batches overlap on the device; a completion
- * thread delivers results and records latency (submit -> result in h_out).
+ * admits requests in arrival order while the memory budget allows (a batch
+ * holds one range of the budget: n x 504 bytes per request plus 32 bytes per
+ * descriptor and 256 for its queue header, and the same of pinned staging),
+ * packs them into batches and runs each batch on a free slot (stream + batch
+ * work queue) with one upload, one launch and one download, so batches overlap
+ * on the device; each slot's completion thread delivers its batches' results
+ * and records latency (submit -> result in h_out).
  * All batches of one server use the same knot-to-lane mapping, so a result does
  * not depend on what it was batched with.  No reference counterpart (the
  * reference has no serving layer); the latency statistics follow its
@@ -287,7 +293,7 @@ typedef struct rbd_server_config {
   int64_t max_batch_trajectories; /* per batch */
   int64_t max_batch_knots;        /* per batch; also the longest admissible request */
   double batch_window_us;         /* after the oldest queued request arrived, wait this long for more (0: none) */
-  int use_graph;                  /* replay each slot's batch kernels from a hipGraph */
+  int use_graph;                  /* ignored: each batch's queue is uploaded with its inputs and launched directly */
   int reserved[3];
 } rbd_server_config;
 
@@ -306,8 +312,31 @@ typedef struct rbd_server_stats {
   double knots_per_s;             /* completed knots / (last completion - first submit) */
 } rbd_server_stats;
 
-/* slots 4, lanes_per_knot 16, 64 MiB budget, 1024 trajectories and 2^17 knots
- * per batch, no batching window, no graphs. */
+/* Per-batch trace record (the server keeps the last 65536 batches).  Times
+ * are microseconds since the server was created:
+ *   oldest_submit_us  submit time of the batch's first (oldest) request
+ *   admit_us          the dispatcher admitted the batch (memory reserved)
+ *   enqueued_us       upload, launch, download and event were enqueued
+ *   done_us           the completion thread saw the batch's event
+ *   delivered_us      results were in the callers' buffers */
+typedef struct rbd_batch_record {
+  double oldest_submit_us;
+  double admit_us;
+  double enqueued_us;
+  double done_us;
+  double delivered_us;
+  int64_t requests;
+  int64_t knots;
+  int slot;
+  int reserved;
+} rbd_batch_record;
+
+/* Copy up to max_records of the most recent batch records, oldest first;
+ * *n_out receives the number copied.  No reference counterpart. */
+rbd_status rbd_server_get_trace(rbd_server* s, rbd_batch_record* out, int64_t max_records, int64_t* n_out);
+
+/* slots 4, lanes_per_knot 16, 64 MiB budget, 1024 trajectories and 2^15 knots
+ * per batch, no batching window. */
 void rbd_server_config_default(rbd_server_config* cfg);
 rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd_server** out);
 /* Finishes the batches on the device, fails the queued requests, joins the threads. */

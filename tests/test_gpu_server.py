@@ -37,7 +37,6 @@ def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes):
     st = srv.stats()
     say("stats", st)
     assert st["requests_completed"] == 302
-    assert st["max_inflight_batches"] >= 2          # batches overlapped on the device
     for x, o in zip(others[::37], outs[::37]):
         err = common.max_block_errors(o, oracle.fdgrad(x.astype(np.float64)))
         assert max(err.values()) < common.REL_TOL, err
@@ -76,13 +75,33 @@ def test_request_larger_than_budget_is_rejected(rbd, ctx):
     srv.close()
 
 
-@pytest.mark.parametrize("use_graph", [0, 1])
-def test_poisson_load(rbd, ctx, use_graph):
-    srv = rbd.Server(ctx, slots=4, use_graph=use_graph)
+@pytest.mark.parametrize("slots", [1, 4])
+def test_poisson_load(rbd, ctx, slots):
+    srv = rbd.Server(ctx, slots=slots)
     pool = common.knots(8192, seed=800)
     st = srv.run_poisson(rate=20000, duration_s=0.5, lengths=[16, 32, 64, 128, 256], pool=pool, seed=1)
     say("poisson 20k/s", st)
     assert st["requests_completed"] > 5000
+    assert st["max_inflight_batches"] <= slots
     assert 0 < st["p50_latency_us"] <= st["p99_latency_us"] <= st["max_latency_us"]
     assert st["knots_per_s"] > 0
+    srv.close()
+
+
+def test_batches_overlap_on_the_device(rbd, ctx):
+    """Requests arriving while batches are in flight are admitted into free
+    slots: under a 100k req/s load several batches are on the device at once."""
+    srv = rbd.Server(ctx, slots=4)
+    pool = common.knots(8192, seed=801)
+    st = srv.run_poisson(rate=100000, duration_s=0.2, lengths=[16, 32, 64, 128, 256], pool=pool, seed=2)
+    say("poisson 100k/s", st)
+    assert st["requests_completed"] > 15000
+    assert 2 <= st["max_inflight_batches"] <= 4
+    tr = srv.trace()
+    assert len(tr["admit_us"]) == st["batches"]
+    assert np.all(tr["admit_us"] >= tr["oldest_submit_us"])
+    assert np.all(tr["enqueued_us"] >= tr["admit_us"])
+    assert np.all(tr["done_us"] >= tr["enqueued_us"])
+    assert np.all(tr["delivered_us"] >= tr["done_us"])
+    assert int(tr["requests"].sum()) == st["requests_completed"]
     srv.close()
