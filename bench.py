@@ -188,14 +188,16 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000), duration_s=1.
     4 in-flight slots.  Latency = submit -> result in the caller's host buffer."""
     pool = knots(1 << 16, seed=103)
     res = {}
-    srv = rbdgrad.Server(ctx, slots=4, lanes_per_knot=16, memory_budget_bytes=256 << 20)
-    for rate in rates:
-        log(f"[c3] served: Poisson {rate} req/s for {duration_s} s")
-        st = srv.run_poisson(rate, duration_s, [16, 32, 64, 128, 256], pool, seed=rate)
-        st["batch_phases_us"] = batch_phases(srv.trace())
-        res[f"served_{rate}_req_per_s"] = st
-        log(f"[c3] {st}")
-    srv.close()
+    for zero_copy in (0, 1):
+        mode = "zero_copy" if zero_copy else "device_buffers"
+        srv = rbdgrad.Server(ctx, slots=4, lanes_per_knot=16, memory_budget_bytes=256 << 20, zero_copy=zero_copy)
+        for rate in rates:
+            log(f"[c3] served ({mode}): Poisson {rate} req/s for {duration_s} s")
+            st = srv.run_poisson(rate, duration_s, [16, 32, 64, 128, 256], pool, seed=rate)
+            st["batch_phases_us"] = batch_phases(srv.trace())
+            res[f"served_{mode}_{rate}_req_per_s"] = st
+            log(f"[c3] {st}")
+        srv.close()
     return res
 
 

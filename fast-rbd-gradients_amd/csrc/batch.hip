@@ -23,6 +23,10 @@ struct rbd_batch {
   rbd::BatchHeader* d_hdr = nullptr;
   rbd::TrajDesc* d_desc = nullptr;
   long* d_prefix = nullptr;
+  // queue cursor of staged launches (device memory, never reset: each launch
+  // advances it by its chunk total, which counter_base tracks)
+  unsigned long long* d_counter = nullptr;
+  unsigned long long counter_base = 0;
   size_t upload_bytes = 0;            // header + descriptors
   // pinned host staging ring for the descriptor upload
   static constexpr int kRing = 4;
@@ -47,18 +51,21 @@ constexpr size_t kHeaderBytes = 256;   // the header's slot before the descripto
 // Where a launch finds its queue: header and descriptors (the batch's own, or
 // ones a caller staged in its device memory) and the prefix-sum scratch.
 struct Queue {
-  rbd::BatchHeader* hdr;
-  rbd::TrajDesc* descs;
+  const rbd::BatchHeader* hdr;
+  const rbd::TrajDesc* descs;
+  unsigned long long* cursor;
+  unsigned long long base;
   long* prefix;
 };
 
-Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, b->d_prefix}; }
+// the batch's own header and descriptors, uploaded with a zeroed cursor
+Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, &b->d_hdr->cursor, 0ull, b->d_prefix}; }
 
 template <class P, int LANES, bool SMALL>
 void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
   const unsigned grid = b->grid[LANES == 1 ? 0 : 1];
   hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, SMALL>), dim3(grid), dim3(kBlock), 0, s, q.descs,
-                     q.hdr, q.prefix, b->ctx->opt);
+                     q.hdr, q.cursor, q.base, q.prefix, b->ctx->opt);
 }
 
 template <class P>
@@ -84,14 +91,21 @@ hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hip
 }
 
 // header + descriptors of a batch into host memory at dst (kHeaderBytes, then
-// the descriptors)
-void stage(char* dst, const rbd_traj* trajs, int n_traj, int lanes) {
+// the descriptors); returns the chunk total
+long stage(char* dst, const rbd_traj* trajs, int n_traj, int lanes) {
   rbd::BatchHeader* h = reinterpret_cast<rbd::BatchHeader*>(dst);
   std::memset(h, 0, sizeof(*h));             // cursor = 0
-  h->n_traj = n_traj;
-  h->chunk_knots = 64 / lanes;
+  const long chunk = 64 / lanes;
+  long total = 0;
   rbd::TrajDesc* d = reinterpret_cast<rbd::TrajDesc*>(dst + kHeaderBytes);
-  for (int t = 0; t < n_traj; ++t) d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
+  for (int t = 0; t < n_traj; ++t) {
+    d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
+    total += (long(trajs[t].n_knots) + chunk - 1) / chunk;
+  }
+  h->n_traj = n_traj;
+  h->chunk_knots = int(chunk);
+  h->total_chunks = total;
+  return total;
 }
 
 int graph_index(int lanes, bool small) { return 2 * (lanes == 1 ? 0 : 1) + (small ? 1 : 0); }
@@ -145,6 +159,14 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
   b->d_hdr = reinterpret_cast<rbd::BatchHeader*>(b->d_mem);
   b->d_desc = reinterpret_cast<rbd::TrajDesc*>(b->d_mem + hdr);
   b->d_prefix = reinterpret_cast<long*>(b->d_mem + hdr + desc);
+  e = hipMalloc(&b->d_counter, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(b->d_counter, 0, sizeof(unsigned long long));   // once, outside any graph
+  if (e != hipSuccess) {
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    (void)hipFree(b->d_mem);
+    delete b;
+    return fail(RBD_ERR_OUT_OF_MEMORY, "batch queue counter");
+  }
   for (int i = 0; i < rbd_batch::kRing && e == hipSuccess; ++i) {
     e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stage[i]), b->upload_bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
@@ -173,6 +195,7 @@ rbd_status rbd_batch_destroy(rbd_batch* b) {
     if (b->h_stage[i]) (void)hipHostFree(b->h_stage[i]);
   }
   if (b->d_mem) (void)hipFree(b->d_mem);
+  if (b->d_counter) (void)hipFree(b->d_counter);
   delete b;
   return RBD_OK;
 }
@@ -245,23 +268,23 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
 size_t rbdi::batch_stage_bytes(int n_traj) { return kHeaderBytes + sizeof(rbd::TrajDesc) * size_t(n_traj); }
 
 int64_t rbdi::batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int lanes) {
-  stage(static_cast<char*>(host_dst), trajs, n_traj, lanes);
-  int64_t total = 0;
-  for (int t = 0; t < n_traj; ++t) total += trajs[t].n_knots;
-  return total;
+  return stage(static_cast<char*>(host_dst), trajs, n_traj, lanes);
 }
 
-rbd_status rbdi::batch_launch_staged(rbd_batch* b, void* d_staged, int n_traj, int64_t total_knots, int lanes,
-                                     rbd_stream_t stream) {
+rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, int n_traj, int64_t total_knots,
+                                     int64_t total_chunks, int lanes, rbd_stream_t stream) {
   if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
   b->last_total_knots = total_knots;
   b->last_lanes = lanes;
   if (total_knots == 0) return RBD_OK;
-  char* base = static_cast<char*>(d_staged);
-  const Queue q{reinterpret_cast<rbd::BatchHeader*>(base), reinterpret_cast<rbd::TrajDesc*>(base + kHeaderBytes),
+  const char* base = static_cast<const char*>(d_staged);
+  const Queue q{reinterpret_cast<const rbd::BatchHeader*>(base),
+                reinterpret_cast<const rbd::TrajDesc*>(base + kHeaderBytes), b->d_counter, b->counter_base,
                 b->d_prefix};
   hipError_t e = launch_batch(b, q, lanes, n_traj <= rbd::kSmallBatch, (hipStream_t)stream);
-  return e == hipSuccess ? RBD_OK : hip_fail(e, "batch kernel launch");
+  if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+  b->counter_base += (unsigned long long)total_chunks;
+  return RBD_OK;
 }
 
 extern "C" {

@@ -82,16 +82,17 @@ inline int lanes_for_batch(const rbd_context* ctx, int64_t total_knots) {
 rbd_status batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph, rbd_stream_t stream,
                      int lanes);
 
-// For callers that upload a batch's header and descriptors together with its
-// inputs (the server): batch_stage writes them (batch_stage_bytes(n_traj)
-// bytes, trajectory pointers as given) to host memory and returns the total
-// knots; after the caller has copied that block to device memory d_staged
-// (256-byte aligned, writable: the queue cursor lives there), on `stream`,
-// batch_launch_staged runs the batch from it.
+// For callers that place a batch's header and descriptors themselves (the
+// server): batch_stage writes them (batch_stage_bytes(n_traj) bytes,
+// trajectory pointers as given) to host memory and returns the chunk total;
+// batch_launch_staged runs the batch from d_staged, a device-visible address
+// of that block (8-byte aligned, read only: device memory the caller copied it
+// to on `stream`, or the pinned host memory itself), with the queue cursor in
+// the batch object's device memory.
 size_t batch_stage_bytes(int n_traj);
 int64_t batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int lanes);
-rbd_status batch_launch_staged(rbd_batch* b, void* d_staged, int n_traj, int64_t total_knots, int lanes,
-                               rbd_stream_t stream);
+rbd_status batch_launch_staged(rbd_batch* b, const void* d_staged, int n_traj, int64_t total_knots,
+                               int64_t total_chunks, int lanes, rbd_stream_t stream);
 
 inline rbd_status check_ctx(const rbd_context* ctx) {
   if (!ctx) return fail(RBD_ERR_INVALID_ARGUMENT, "null context");

@@ -133,8 +133,12 @@ struct rbd_server {
   rbd_context* ctx = nullptr;          // the caller's context (device, flags, precision)
   rbd_context* run_ctx = nullptr;      // same, with the knot-to-lane mapping fixed
   rbd_server_config cfg{};
-  char* d_pool = nullptr;              // device memory of the budget
+  char* d_pool = nullptr;              // device memory of the budget (copy mode)
   char* h_pool = nullptr;              // pinned staging of the same size
+  char* dh_pool = nullptr;             // device address of h_pool (zero-copy mode)
+  // where the kernels find a batch's queue, inputs and outputs: the device
+  // pool (copy mode) or the pinned pool itself (zero-copy mode)
+  char* work_pool() const { return cfg.zero_copy ? dh_pool : d_pool; }
   std::unique_ptr<RangeAllocator> alloc;
   std::vector<Slot> slots;
 
@@ -245,23 +249,27 @@ void dispatch_loop(rbd_server* s) {
     sl.rec.knots = knots;
     sl.rec.slot = slot;
     lk.unlock();
-    // stage descriptors and inputs, one H2D copy, one batched launch, one D2H copy, event
+    // stage descriptors and inputs; copy mode: one H2D copy, the launch, one
+    // D2H copy; zero-copy mode: the launch alone; then the event
+    char* const work = s->work_pool();
+    const bool copy = !s->cfg.zero_copy;
     std::vector<rbd_traj> trajs;
     trajs.reserve(sl.reqs.size());
     for (auto& r : sl.reqs) {
       std::memcpy(s->h_pool + r->in_off, r->h_in, r->in_bytes);
-      trajs.push_back(rbd_traj{reinterpret_cast<const float*>(s->d_pool + r->in_off),
-                               reinterpret_cast<float*>(s->d_pool + r->out_off), r->n});
+      trajs.push_back(rbd_traj{reinterpret_cast<const float*>(work + r->in_off),
+                               reinterpret_cast<float*>(work + r->out_off), r->n});
     }
     const int lanes = lanes_for_batch(s->run_ctx, knots);
     const int n_traj = int(trajs.size());
-    (void)batch_stage(s->h_pool + off, trajs.data(), n_traj, lanes);
+    const int64_t chunks = batch_stage(s->h_pool + off, trajs.data(), n_traj, lanes);
     const size_t out0 = off + stage_bytes + round_up(in_bytes);
-    hipError_t e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, stage_bytes + in_bytes, hipMemcpyHostToDevice,
-                                  sl.stream);
+    hipError_t e = hipSuccess;
+    if (copy)
+      e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, stage_bytes + in_bytes, hipMemcpyHostToDevice, sl.stream);
     rbd_status st = RBD_OK;
-    if (e == hipSuccess) st = batch_launch_staged(sl.batch, s->d_pool + off, n_traj, knots, lanes, sl.stream);
-    if (e == hipSuccess && st == RBD_OK)
+    if (e == hipSuccess) st = batch_launch_staged(sl.batch, work + off, n_traj, knots, chunks, lanes, sl.stream);
+    if (copy && e == hipSuccess && st == RBD_OK)
       e = hipMemcpyAsync(s->h_pool + out0, s->d_pool + out0, out_bytes, hipMemcpyDeviceToHost, sl.stream);
     if (e == hipSuccess && st == RBD_OK) e = hipEventRecord(sl.done, sl.stream);
     sl.rec.enqueued_us = since(s, Clock::now());
@@ -331,33 +339,42 @@ rbd_status warm_up(rbd_server* s) {
   // touch the whole staging and device pools once, copies both ways included,
   // so no page of either is first used by a request
   const size_t pool = s->alloc->size();
+  const bool copy = !s->cfg.zero_copy;
+  char* const work = s->work_pool();
   std::memset(s->h_pool, 0, pool);
-  hipError_t e0 = hipMemcpy(s->d_pool, s->h_pool, pool, hipMemcpyHostToDevice);
-  if (e0 == hipSuccess) e0 = hipMemcpy(s->h_pool, s->d_pool, pool, hipMemcpyDeviceToHost);
-  if (e0 != hipSuccess) return hip_fail(e0, "server warm-up copies");
+  if (copy) {
+    hipError_t e0 = hipMemcpy(s->d_pool, s->h_pool, pool, hipMemcpyHostToDevice);
+    if (e0 == hipSuccess) e0 = hipMemcpy(s->h_pool, s->d_pool, pool, hipMemcpyDeviceToHost);
+    if (e0 != hipSuccess) return hip_fail(e0, "server warm-up copies");
+  }
   const int n_big = kSmallBatchTrajectories + 1;
   const size_t stage = round_up(batch_stage_bytes(n_big));
-  const size_t per = round_up(size_t(n_big) * rbd::kInPerKnot * sizeof(float)) +
-                     round_up(size_t(n_big) * rbd::kOutPerKnot * sizeof(float));
-  if (stage + per > s->alloc->size()) return RBD_OK;     // budget too small to bother
-  std::memset(s->h_pool, 0, stage + per);
-  std::vector<rbd_traj> trajs(static_cast<size_t>(n_big));
-  const size_t in0 = stage, out0 = stage + round_up(size_t(n_big) * rbd::kInPerKnot * sizeof(float));
-  for (int t = 0; t < n_big; ++t)
-    trajs[size_t(t)] = rbd_traj{reinterpret_cast<const float*>(s->d_pool + in0) + size_t(t) * rbd::kInPerKnot,
-                                reinterpret_cast<float*>(s->d_pool + out0) + size_t(t) * rbd::kOutPerKnot, 1};
+  const size_t in_sz = round_up(size_t(n_big) * rbd::kInPerKnot * sizeof(float));
+  const size_t block = stage + in_sz + round_up(size_t(n_big) * rbd::kOutPerKnot * sizeof(float));
+  if (block * s->slots.size() > pool) return RBD_OK;     // budget too small to bother
   const int lanes = lanes_for_batch(s->run_ctx, 1);
-  // the full per-batch sequence, many times per slot: the runtime sizes its
-  // internal pools (signals, command buffers) on first demand
+  // the full per-batch sequence, many times per slot, each slot in its own
+  // block of the pools: the runtime sizes its internal pools (signals, command
+  // buffers) on first demand
   for (int round = 0; round < kWarmRounds; ++round)
-    for (Slot& sl : s->slots) {
+    for (size_t si = 0; si < s->slots.size(); ++si) {
+      Slot& sl = s->slots[si];
+      const size_t off = si * block;
+      std::vector<rbd_traj> trajs(static_cast<size_t>(n_big));
+      for (int t = 0; t < n_big; ++t)
+        trajs[size_t(t)] =
+            rbd_traj{reinterpret_cast<const float*>(work + off + stage) + size_t(t) * rbd::kInPerKnot,
+                     reinterpret_cast<float*>(work + off + stage + in_sz) + size_t(t) * rbd::kOutPerKnot, 1};
       const int n = (round & 1) ? n_big : 1;
-      (void)batch_stage(s->h_pool, trajs.data(), n, lanes);
-      hipError_t e = hipMemcpyAsync(s->d_pool, s->h_pool, stage + per, hipMemcpyHostToDevice, sl.stream);
+      hipError_t e = hipEventSynchronize(sl.done);        // this slot's previous round is done with its block
+      const int64_t chunks = batch_stage(s->h_pool + off, trajs.data(), n, lanes);
+      if (copy && e == hipSuccess)
+        e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, stage + in_sz, hipMemcpyHostToDevice, sl.stream);
       if (e != hipSuccess) return hip_fail(e, "server warm-up upload");
-      if (rbd_status st = batch_launch_staged(sl.batch, s->d_pool, n, n, lanes, sl.stream)) return st;
-      e = hipMemcpyAsync(s->h_pool + out0, s->d_pool + out0, size_t(n) * rbd::kOutPerKnot * sizeof(float),
-                         hipMemcpyDeviceToHost, sl.stream);
+      if (rbd_status st = batch_launch_staged(sl.batch, work + off, n, n, chunks, lanes, sl.stream)) return st;
+      if (copy)
+        e = hipMemcpyAsync(s->h_pool + off + stage + in_sz, s->d_pool + off + stage + in_sz,
+                           size_t(n) * rbd::kOutPerKnot * sizeof(float), hipMemcpyDeviceToHost, sl.stream);
       if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
       if (e != hipSuccess) return hip_fail(e, "server warm-up");
     }
@@ -421,9 +438,14 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
   rbd_status st = rbd_create(&rc, &s->run_ctx);
   const size_t budget = round_up(size_t(c.memory_budget_bytes));
   hipError_t e = hipSuccess;
-  if (st == RBD_OK) e = hipMalloc(&s->d_pool, budget);
+  if (st == RBD_OK && !c.zero_copy) e = hipMalloc(&s->d_pool, budget);
+  // zero-copy: the kernels read and write the pinned pool directly, so it is
+  // mapped and fine-grained (coherent with the host without cache flushes)
   if (st == RBD_OK && e == hipSuccess)
-    e = hipHostMalloc(reinterpret_cast<void**>(&s->h_pool), budget, hipHostMallocDefault);
+    e = hipHostMalloc(reinterpret_cast<void**>(&s->h_pool), budget,
+                      c.zero_copy ? (hipHostMallocMapped | hipHostMallocCoherent) : hipHostMallocDefault);
+  if (st == RBD_OK && e == hipSuccess && c.zero_copy)
+    e = hipHostGetDevicePointer(reinterpret_cast<void**>(&s->dh_pool), s->h_pool, 0);
   if (st == RBD_OK && e == hipSuccess) {
     s->alloc.reset(new RangeAllocator(budget));
     s->slots.resize(size_t(c.slots));

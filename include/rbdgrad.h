@@ -294,7 +294,10 @@ typedef struct rbd_server_config {
   int64_t max_batch_knots;        /* per batch; also the longest admissible request */
   double batch_window_us;         /* after the oldest queued request arrived, wait this long for more (0: none) */
   int use_graph;                  /* ignored: each batch's queue is uploaded with its inputs and launched directly */
-  int reserved[3];
+  int zero_copy;                  /* 0: requests' buffers in device memory, one upload and one download per batch;
+                                     1: the kernels read inputs from and write results to the pinned staging
+                                     directly (no copies: lower latency, PCIe traffic from the kernels) */
+  int reserved[2];
 } rbd_server_config;
 
 typedef struct rbd_server_stats {

@@ -19,9 +19,10 @@ def ctx(rbd):
     c.close()
 
 
+@pytest.mark.parametrize("zero_copy", [0, 1])
 @pytest.mark.parametrize("lanes", [1, 16])
-def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes):
-    srv = rbd.Server(ctx, lanes_per_knot=lanes, slots=4)
+def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_copy):
+    srv = rbd.Server(ctx, lanes_per_knot=lanes, slots=4, zero_copy=zero_copy)
     rng = np.random.default_rng(21)
     probe = common.knots(128, seed=500)
     alone = srv.wait(srv.submit(probe))
@@ -47,12 +48,13 @@ def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes):
     srv.close()
 
 
-def test_memory_budget_is_respected(rbd, ctx):
+@pytest.mark.parametrize("zero_copy", [0, 1])
+def test_memory_budget_is_respected(rbd, ctx, zero_copy):
     """A budget that holds only a few requests: everything still completes, in
     several batches, and the pool never exceeds the budget."""
     per_knot = 21 * 4 + 105 * 4
     budget = 3 * 256 * per_knot + 4096
-    srv = rbd.Server(ctx, memory_budget_bytes=budget, slots=2)
+    srv = rbd.Server(ctx, memory_budget_bytes=budget, slots=2, zero_copy=zero_copy)
     xs = [common.knots(256, seed=700 + i) for i in range(20)]
     ids = [srv.submit(x) for x in xs]
     outs = [srv.wait(i) for i in ids]
@@ -75,9 +77,10 @@ def test_request_larger_than_budget_is_rejected(rbd, ctx):
     srv.close()
 
 
+@pytest.mark.parametrize("zero_copy", [0, 1])
 @pytest.mark.parametrize("slots", [1, 4])
-def test_poisson_load(rbd, ctx, slots):
-    srv = rbd.Server(ctx, slots=slots)
+def test_poisson_load(rbd, ctx, slots, zero_copy):
+    srv = rbd.Server(ctx, slots=slots, zero_copy=zero_copy)
     pool = common.knots(8192, seed=800)
     st = srv.run_poisson(rate=20000, duration_s=0.5, lengths=[16, 32, 64, 128, 256], pool=pool, seed=1)
     say("poisson 20k/s", st)
@@ -88,10 +91,11 @@ def test_poisson_load(rbd, ctx, slots):
     srv.close()
 
 
-def test_batches_overlap_on_the_device(rbd, ctx):
+@pytest.mark.parametrize("zero_copy", [0, 1])
+def test_batches_overlap_on_the_device(rbd, ctx, zero_copy):
     """Requests arriving while batches are in flight are admitted into free
     slots: under a 100k req/s load several batches are on the device at once."""
-    srv = rbd.Server(ctx, slots=4)
+    srv = rbd.Server(ctx, slots=4, zero_copy=zero_copy)
     pool = common.knots(8192, seed=801)
     st = srv.run_poisson(rate=100000, duration_s=0.2, lengths=[16, 32, 64, 128, 256], pool=pool, seed=2)
     say("poisson 100k/s", st)
