@@ -244,7 +244,7 @@ def profile_one(args, stream):
         res = served_workload(ctx)
         log(json.dumps(res))
         return
-    lanes = 1 if args.only in ("c1-one-lane", "c4-aos", "c4-soa") else 0
+    lanes = 1 if args.only in ("c1-one-lane", "c4-aos", "c4-soa") else (16 if args.only == "c1-lane-group" else 0)
     ctx = rbdgrad.Context(device=0, precision=args.precision, lanes_per_knot=lanes)
     if args.only.startswith("c1"):
         n, x, layout = 128, knots(128, seed=101), "aos"
@@ -264,7 +264,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time per thread count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the large-batch and mixed workloads")
-    ap.add_argument("--only", choices=["c1", "c1-one-lane", "c4-aos", "c4-soa", "c2", "c3", "copy"],
+    ap.add_argument("--only", choices=["c1", "c1-one-lane", "c1-lane-group", "c4-aos", "c4-soa", "c2", "c3", "copy"],
                     help="run one workload only and print its timing (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -281,8 +281,9 @@ def main():
         return
     ctx = rbdgrad.Context(device=0, precision=args.precision)
     threshold, tune_table = ctx.autotune(stream)
-    log(f"autotune: lane-group mapping up to {threshold} knots; table (n: us one-lane, us lane-group) "
-        f"{ {k: (round(a, 2), round(b, 2)) for k, (a, b) in tune_table.items()} }")
+    log(f"autotune: two-wave mapping up to {ctx.two_wave_threshold} knots, lane groups up to {threshold}; "
+        f"table (n: us one-lane, us lane-group, us two-wave) "
+        f"{ {k: tuple(round(t, 2) for t in v) for k, v in tune_table.items()} }")
     bthreshold, btune_table = ctx.autotune_batch(stream)
     log(f"autotune (batched): lane-group mapping up to {bthreshold} knots; table (total knots: us one-lane, "
         f"us lane-group) { {k: (round(a, 2), round(b, 2)) for k, (a, b) in btune_table.items()} }")
@@ -300,8 +301,9 @@ def main():
     log(f"[c1] {kname}: {value:.3e} knots/s, {ms_step * 1e3:.2f} us/launch, acc {acc1}")
 
     workloads = {"autotune": {"group_threshold_knots": threshold,
-                              "us_per_launch": {str(k): {"one_lane": a, "lane_group": b}
-                                                for k, (a, b) in tune_table.items()},
+                              "two_wave_threshold_knots": ctx.two_wave_threshold,
+                              "us_per_launch": {str(k): {"one_lane": a, "lane_group": b, "two_wave": c}
+                                                for k, (a, b, c) in tune_table.items()},
                               "batch_group_threshold_knots": bthreshold,
                               "batch_us_per_batch": {str(k): {"one_lane": a, "lane_group": b}
                                                      for k, (a, b) in btune_table.items()}}}
@@ -313,6 +315,14 @@ def main():
         "knots_per_s": n1 * (args.steps // 4) / (ms1b * 1e-3)}
     log(f"[c1] one lane per knot: {workloads['per_call_n128_one_lane_per_knot']}")
     one.close()
+    # ... and to one 16-lane group per knot in one wave
+    grp = rbdgrad.Context(device=0, precision=args.precision, lanes_per_knot=16)
+    ms1c, _ = run_flat(grp, x1, args.steps // 4, args.warmup, stream)
+    workloads["per_call_n128_lane_group"] = {
+        "kernel": grp.kernel_name(n1), "us_per_launch": ms1c / (args.steps // 4) * 1e3,
+        "knots_per_s": n1 * (args.steps // 4) / (ms1c * 1e-3)}
+    log(f"[c1] one-wave lane group: {workloads['per_call_n128_lane_group']}")
+    grp.close()
     if not args.quick:
         # ---- large batch: 2^20 knots, 10% stretched-arm poses ----
         n4 = 1 << 20
@@ -352,7 +362,7 @@ def main():
         "data": "synthetic: seeded uniform q~U(-pi,pi), qd~U(-2,2), u~U(-50,50); iiwa 14 model constants",
         "config": {"workload": "per-call kernel, one trajectory of 128 knot points per launch (config 1)",
                    "knots_per_call": n1, "layout": "aos", "precision": args.precision, "kernel": kname,
-                   "lane_group_threshold": threshold},
+                   "lane_group_threshold": threshold, "two_wave_threshold": ctx.two_wave_threshold},
         "roofline": {"bound": "hbm", "achieved": gbs1, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs1 / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_knot": BYTES_PER_KNOT, "knots_per_launch": n1},

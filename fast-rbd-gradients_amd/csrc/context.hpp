@@ -20,6 +20,11 @@ constexpr int kBlock = 64;   // one wave per workgroup
 // rbd_autotune (one-lane 22.8 us vs lane-group 12.5 us at 8192 knots,
 // 23.3 vs 22.0 us at 16384, 24.9 vs 39.2 us at 32768).
 constexpr int64_t kDefaultGroupThreshold = 16384;
+// The two-wave mapping (Minv and the RNEA + columns of a knot on two waves)
+// serves calls up to the context's two_wave_threshold knots: by default the
+// largest call whose workgroups (two waves per 4 knots) all fit the SIMDs at
+// once, 2 x SIMDs knots; rbd_autotune measures it.
+constexpr int kTwoWaveLanes = 32;   // lanes_per_knot value of the two-wave mapping (16 lanes in each of 2 waves)
 // The same crossover for batches (rbd_batch_*, total knots of the batch),
 // measured by rbd_autotune_batch on batches of 132-knot trajectories.
 constexpr int64_t kDefaultBatchGroupThreshold = 4096;
@@ -56,6 +61,7 @@ struct rbd_context {
   int precision = RBD_PREC_FP32;
   int lanes_per_knot = 0;                        // 0 = by batch size, 1, or 16
   std::atomic<int64_t> group_threshold{rbdi::kDefaultGroupThreshold};
+  std::atomic<int64_t> two_wave_threshold{0};    // set by rbd_create / rbd_autotune
   std::atomic<int64_t> batch_group_threshold{rbdi::kDefaultBatchGroupThreshold};
   int compute_units = 0;                         // of the device
   rbd::Options opt;
@@ -67,12 +73,18 @@ struct rbd_context {
 
 namespace rbdi {
 
-inline int lanes_for(const rbd_context* ctx, int64_t n) {
+// Mapping of a call of n knots: 1, 16 or kTwoWaveLanes.  Only the full path
+// (AoS, SoA) has a two-wave kernel; other entry points pass two_wave = false
+// and get the one-wave lane group instead.
+inline int lanes_for(const rbd_context* ctx, int64_t n, bool two_wave = false) {
+  if (ctx->lanes_per_knot == kTwoWaveLanes) return two_wave ? kTwoWaveLanes : rbd::kLanesPerKnotGroup;
   if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
+  if (two_wave && n <= ctx->two_wave_threshold.load(std::memory_order_relaxed)) return kTwoWaveLanes;
   return n <= ctx->group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
 }
 
 inline int lanes_for_batch(const rbd_context* ctx, int64_t total_knots) {
+  if (ctx->lanes_per_knot == kTwoWaveLanes) return rbd::kLanesPerKnotGroup;
   if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
   return total_knots <= ctx->batch_group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
 }

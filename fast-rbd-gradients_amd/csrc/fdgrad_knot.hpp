@@ -773,6 +773,215 @@ RBD_HD RBD_INLINE void fdgrad_knot_lanes(const In& in, Out& out, const Options& 
   gradient_column_lane(PC, opt.damping, role, out);
 }
 
+// ---------------------------------------------------------------------------
+// Two-wave latency mapping.  For one knot, one wave (M) runs the Minv
+// recursion while a second wave (R) runs the RNEA at qdd = 0 and all thirteen
+// derivative columns with the bias accelerations and forces a_b, f_b; both
+// are ~1.7k instructions, so they overlap instead of adding up.  The columns
+// need a, f at the true qdd only in the d/dq seeds (a_c x S at link c) and the
+// backward extra term (f_c at link c), in which a d/dq column is linear.  So
+// with da = a - a_b and df = f - f_b, both linear in qdd:
+//   dtau/dq_c = column(a_b, f_b) + column'(da_c x S, df_c)
+// where column' is the sweep with v = 0 (no velocity terms, ~400
+// instructions).  After M has Minv it forms qdd = Minv (u - c - damping qd),
+// the chain da_k = T_k da_{k-1} + S qdd_k, dF_k = I_k da_k, df = RNEA
+// backward pass of dF, and each d/dq lane's correction column'; R then adds
+// the corrections and forms -Minv dtau.  Stage functions are per (knot, lane
+// role); wave M runs M1, waits for R1a's bias, runs M2 and signals; wave R
+// runs R1a, signals, runs R1b, waits for M2, runs R2.  The critical path is
+// M1 + M2 + R2 (~2.7k instructions against ~3.5k for the one-wave lane group).
+// ---------------------------------------------------------------------------
+
+// Exchange between the waves for one knot (LDS in the kernel).
+template <class P>
+struct TwoWaveShared {
+  typename P::Rnea bias[NQ];                        // u - c - damping qd (R1 -> M2)
+  typename P::Minv mi[NQ * (NQ + 1) / 2];           // upper Minv, row-major packed (M2 -> R2)
+  typename P::Col corr[kLanesPerKnotGroup][NQ];     // d/dq corrections by lane role (M2 -> R2)
+};
+
+// index of (r, c), r <= c, in the packed upper triangle
+RBD_HD constexpr int upper_index(int r, int c) { return r * NQ - r * (r - 1) / 2 + (c - r); }
+
+// column'(da_c x S, df_c) for lane role's d/dq column c, uniform over lanes
+// like dtau_column_uniform: dax[k] = (da_k x S) as (a1, -a0, a4, -a3),
+// ndfx[k] = -(df_k x S) as (-f1, f0, -f4, f3).
+template <typename S>
+RBD_HD RBD_INLINE void dq_correction_uniform(const CS<S>* t, const S dax[NQ][4], const S ndfx[NQ][4], int c,
+                                             S tau[NQ]) {
+  S da[6], df[NQ][6];
+  for (int i = 0; i < 6; ++i) da[i] = S(0);
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    S pa[6];
+    if constexpr (k > 0) {
+      xmotion<k>(t[k], da, pa);
+    } else {
+      for (int i = 0; i < 6; ++i) pa[i] = S(0);
+    }
+    const S ax[6] = {dax[k][0], dax[k][1], S(0), dax[k][2], dax[k][3], S(0)};
+    const bool seed = (k == c);
+    for (int i = 0; i < 6; ++i) da[i] = seed ? ax[i] : pa[i];
+    imv<k>(da, df[k]);
+  });
+  S g[6];
+  for (int i = 0; i < 6; ++i) g[i] = df[NQ - 1][i];
+  static_for_down<NQ, 1>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    tau[k] = g[2];
+    const bool extra = (k == c);
+    g[0] += extra ? ndfx[k][0] : S(0); g[1] += extra ? ndfx[k][1] : S(0);
+    g[3] += extra ? ndfx[k][2] : S(0); g[4] += extra ? ndfx[k][3] : S(0);
+    S y[6];
+    xforce_t<k>(t[k], g, y);
+    for (int i = 0; i < 6; ++i) g[i] = y[i] + df[k - 1][i];
+  });
+  tau[0] = g[2];
+}
+
+// Wave R's state between its stages.
+template <class P>
+struct TwoWaveR {
+  using SR = typename P::Rnea; using SC = typename P::Col;
+  CS<SC> tc[NQ];
+  SR qd[NQ], v[NQ][6], a[NQ][6], f[NQ][6];
+};
+
+// Stage R1a (wave R): RNEA at qdd = 0 and the bias into sh.
+template <class P, class In>
+RBD_HD RBD_INLINE void two_wave_r1a(const In& in, const Options& opt, int role, TwoWaveShared<P>& sh,
+                                    TwoWaveR<P>& st) {
+  using SR = typename P::Rnea; using SC = typename P::Col;
+  using ST = Wider<SR, SC>;
+  CS<SR> tr[NQ];
+  SR u[NQ];
+  for (int k = 0; k < NQ; ++k) {
+    ST s, c;
+    sincos_s(ST(in.q(k)), &s, &c);
+    tr[k] = CS<SR>{SR(c), SR(s)}; st.tc[k] = CS<SC>{SC(c), SC(s)};
+    st.qd[k] = SR(in.qd(k));
+    u[k] = SR(in.u(k));
+  }
+  const SR zero[NQ] = {SR(0), SR(0), SR(0), SR(0), SR(0), SR(0), SR(0)};
+  rnea_fwd(tr, st.qd, zero, SR(opt.gravity), st.v, st.a, st.f);
+  subtract_fext(in, st.f);
+  rnea_bwd(tr, st.f);
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    if (role == k) sh.bias[k] = u[k] - st.f[k][2] - SR(opt.damping) * st.qd[k];
+  });
+}
+
+// Stage R1b (wave R): lane role's column with a_b, f_b (damping on the
+// d/dqd diagonal) into tau_col.
+template <class P>
+RBD_HD RBD_INLINE void two_wave_r1b(const Options& opt, int role, const TwoWaveR<P>& st,
+                                    typename P::Col tau_col[NQ]) {
+  using SR = typename P::Rnea; using SC = typename P::Col;
+  SR Mz[NQ][NQ];
+  for (int r = 0; r < NQ; ++r)
+    for (int c = 0; c < NQ; ++c) Mz[r][c] = SR(0);
+  PhaseC<SC> PC;
+  make_phase_c(PC, st.tc, st.qd, st.v, st.a, st.f, Mz);
+  const bool dq = role >= NQ;
+  const int c = dq ? role - (NQ - 1) : role;
+  dtau_column_uniform(PC, c, dq, tau_col);
+  add_on_diagonal(tau_col, dq ? -1 : c, SC(opt.damping));
+}
+
+// sin/cos type of wave M (shared by its two stages)
+template <class P>
+using TwoWaveTrig = Wider<typename P::Minv, Wider<typename P::Rnea, typename P::Col>>;
+
+// Stage M1 (wave M): sin/cos (kept for M2) and Minv.
+template <class P, class In>
+RBD_HD RBD_INLINE void two_wave_m1(const In& in, CS<TwoWaveTrig<P>> tw[NQ], typename P::Minv Mi[NQ][NQ]) {
+  using SM = typename P::Minv; using SP = typename P::Pivot; using ST = TwoWaveTrig<P>;
+  CS<SM> tm[NQ];
+  for (int k = 0; k < NQ; ++k) {
+    ST s, c;
+    sincos_s(ST(in.q(k)), &s, &c);
+    tw[k] = CS<ST>{c, s};
+    tm[k] = CS<SM>{SM(c), SM(s)};
+  }
+  compute_minv<SM, SP>(tm, Mi);
+}
+
+// Stage M2 (wave M, after R1): qdd (stored by lane role kRoleQdd), the
+// qdd-dependent chain, lane role's d/dq correction and Minv into sh.
+template <class P, class Out>
+RBD_HD RBD_INLINE void two_wave_m2(int role, const CS<TwoWaveTrig<P>> tw[NQ], const typename P::Minv Mi[NQ][NQ],
+                                   TwoWaveShared<P>& sh, Out& out) {
+  using SR = typename P::Rnea; using SC = typename P::Col;
+  CS<SR> tr[NQ]; CS<SC> tc[NQ];
+  for (int k = 0; k < NQ; ++k) {
+    tr[k] = CS<SR>{SR(tw[k].c), SR(tw[k].s)};
+    tc[k] = CS<SC>{SC(tw[k].c), SC(tw[k].s)};
+  }
+  SR bias[NQ], qdd[NQ];
+  for (int k = 0; k < NQ; ++k) bias[k] = sh.bias[k];
+  minv_times<false>(Mi, bias, qdd);
+  if (role == kRoleQdd)
+    for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
+  // da_k = T_k da_{k-1} + S qdd_k, dF_k = I_k da_k, then the backward pass
+  SR dA[NQ][6], dF[NQ][6];
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    if constexpr (k == 0) {
+      for (int i = 0; i < 6; ++i) dA[0][i] = SR(0);
+    } else {
+      xmotion<k>(tr[k], dA[k - 1], dA[k]);
+    }
+    dA[k][2] += qdd[k];
+    imv<k>(dA[k], dF[k]);
+  });
+  rnea_bwd(tr, dF);
+  SC dax[NQ][4], ndfx[NQ][4];
+  for (int k = 0; k < NQ; ++k) {
+    dax[k][0] = SC(dA[k][1]); dax[k][1] = -SC(dA[k][0]); dax[k][2] = SC(dA[k][4]); dax[k][3] = -SC(dA[k][3]);
+    ndfx[k][0] = -SC(dF[k][1]); ndfx[k][1] = SC(dF[k][0]); ndfx[k][2] = -SC(dF[k][4]); ndfx[k][3] = SC(dF[k][3]);
+  }
+  SC corr[NQ];
+  dq_correction_uniform(tc, dax, ndfx, role - (NQ - 1), corr);
+  for (int r = 0; r < NQ; ++r) sh.corr[role][r] = corr[r];
+  // the packed upper Minv, two entries per lane role
+  static_for<0, NQ>([&](auto Rc) {
+    constexpr int r = decltype(Rc)::value;
+    static_for<r, NQ>([&](auto Cc) {
+      constexpr int c = decltype(Cc)::value;
+      constexpr int idx = upper_index(r, c);
+      if (role == idx % kLanesPerKnotGroup) sh.mi[idx] = Mi[r][c];
+    });
+  });
+}
+
+// Stage R2 (wave R, after M2): corrections added, -Minv dtau, stores.
+template <class P, class Out>
+RBD_HD RBD_INLINE void two_wave_r2(int role, const TwoWaveShared<P>& sh, typename P::Col tau_col[NQ], Out& out) {
+  using SC = typename P::Col;
+  const bool dq = role >= NQ;
+  const int c = dq ? role - (NQ - 1) : role;
+  if (dq)
+    for (int r = 0; r < NQ; ++r) tau_col[r] += sh.corr[role][r];
+  SC Mi[NQ][NQ];
+  static_for<0, NQ>([&](auto Rc) {
+    constexpr int r = decltype(Rc)::value;
+    static_for<r, NQ>([&](auto Cc) {
+      constexpr int cc = decltype(Cc)::value;
+      Mi[r][cc] = SC(sh.mi[upper_index(r, cc)]);
+    });
+  });
+  SC col[NQ];
+  minv_times<true>(Mi, tau_col, col);
+  if (role < NQ) {
+    for (int r = 0; r < NQ; ++r) out.dqd(r, c, float(col[r]));
+  } else if (role < 2 * NQ - 1) {
+    for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
+  } else if (role == kRoleQdd) {
+    for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
+  }
+}
+
 // Reference-compatible mode (QDD_MINV_PASSED_IN, the reference's timed
 // configuration; dynamicsGradientKernel, helpers_GPU/dynamicsGradient.cuh:1257-1291):
 // qdd and a dense Minv are inputs; out is dqdd/dq, dqdd/dqd.  The upper triangle
@@ -923,6 +1132,15 @@ struct OutAoSMinv {
   RBD_HD void minv(int r, int c, float x) const { m[NQ * c + r] = x; }
 };
 // forward dynamics only: qdd (7 per knot) and Minv (49 per knot, dense column-major)
+// an output accessor whose stores are dropped when `on` is false (lanes that
+// only take part in a workgroup's barriers)
+template <class Out>
+struct OutMasked {
+  Out o; bool on;
+  RBD_HD void qdd(int i, float x) const { if (on) o.qdd(i, x); }
+  RBD_HD void dq(int r, int c, float x) const { if (on) o.dq(r, c, x); }
+  RBD_HD void dqd(int r, int c, float x) const { if (on) o.dqd(r, c, x); }
+};
 struct OutFD {
   float* qdd_p; float* minv_p;
   RBD_HD void qdd(int i, float x) const { qdd_p[i] = x; }

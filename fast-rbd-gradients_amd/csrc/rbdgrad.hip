@@ -119,6 +119,83 @@ __global__ __launch_bounds__(kBlock, W) void fdgrad_refmode_lanes_kernel(long n,
   }
 }
 
+// Two-wave latency mapping: a workgroup of two waves takes 4 knots, 16 lanes
+// per knot in each wave; wave 0 runs Minv, wave 1 the RNEA and the columns
+// (two_wave_* in fdgrad_knot.hpp).  The waves hand data over in LDS and
+// signal with one LDS flag per direction: each wave sets its flag
+// unconditionally before it waits for the other's, so every wait ends (and
+// gives up after ~2^24 sleeps in any case).  Lanes past the last knot compute
+// a copy of it and store nothing.
+constexpr int kTwoWaveKnots = 4;
+
+// The fences cover LDS only: a release that also waited for the wave's
+// outstanding global stores would hold the flag back by microseconds.  The
+// scheduling barriers keep the flag store and the polling loop where they are
+// in the instruction stream.
+__device__ inline void lds_signal(int* flag) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  if (threadIdx.x % kBlock == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __builtin_amdgcn_sched_barrier(0);
+}
+__device__ inline void lds_wait(int* flag) {
+  __builtin_amdgcn_sched_barrier(0);
+  for (int spin = 0; spin < (1 << 24); ++spin) {
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+// Materialise a value before a wait: the compiler would otherwise sink work
+// whose results are only used after the wait below it, and the two waves
+// would run one after the other instead of side by side.
+template <typename S>
+__device__ inline void pin(S x) { asm volatile("" ::"v"(x)); }
+template <class P, bool SOA>
+__global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, const float* __restrict__ in,
+                                                                       long ld_in, float* __restrict__ out,
+                                                                       long ld_out, Options opt) {
+  __shared__ TwoWaveShared<P> sh[kTwoWaveKnots];
+  __shared__ int flags[2];                 // [0]: bias ready (R -> M), [1]: Minv and corrections ready (M -> R)
+  if (threadIdx.x == 0) { flags[0] = 0; flags[1] = 0; }
+  __syncthreads();
+  const int wave = threadIdx.x / kBlock;
+  const int lane = threadIdx.x % kBlock;
+  const int g = lane / kLanesPerKnotGroup;
+  const int role = lane % kLanesPerKnotGroup;
+  const long k = long(blockIdx.x) * kTwoWaveKnots + g;
+  const bool valid = k < n;
+  const long kk = valid ? k : n - 1;
+  using In = typename std::conditional<SOA, InSoA, InAoS>::type;
+  using O = typename std::conditional<SOA, OutSoA, OutAoS>::type;
+  In i;
+  O o0;
+  if constexpr (SOA) { i = InSoA{in + kk, ld_in}; o0 = OutSoA{out + kk, ld_out}; }
+  else { i = InAoS{in + kInPerKnot * kk}; o0 = OutAoS{out + kOutPerKnot * kk}; }
+  OutMasked<O> o{o0, valid};
+  if (wave == 0) {
+    CS<TwoWaveTrig<P>> tw[NQ];
+    typename P::Minv Mi[NQ][NQ];
+    two_wave_m1<P>(i, tw, Mi);
+    for (int r = 0; r < NQ; ++r) {
+      pin(tw[r].c);
+      pin(tw[r].s);
+      for (int c = r; c < NQ; ++c) pin(Mi[r][c]);
+    }
+    lds_wait(&flags[0]);
+    two_wave_m2<P>(role, tw, Mi, sh[g], o);
+    lds_signal(&flags[1]);
+  } else {
+    TwoWaveR<P> st;
+    typename P::Col tau_col[NQ];
+    two_wave_r1a<P>(i, opt, role, sh[g], st);
+    lds_signal(&flags[0]);
+    two_wave_r1b<P>(opt, role, st, tau_col);
+    for (int r = 0; r < NQ; ++r) pin(tau_col[r]);
+    lds_wait(&flags[1]);
+    two_wave_r2<P>(role, sh[g], tau_col, o);
+  }
+}
+
 // Full path plus dqdd/du = Minv (49 per knot, dense), both mappings.
 template <class P, int LANES>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_minv_kernel(long n, const float* __restrict__ in,
@@ -256,8 +333,20 @@ hipError_t launch_flat_p(const FlatCall& c) {
         hipLaunchKernelGGL(rbd::fdgrad_refmode_kernel<P>, grid, block, 0, c.stream, n, c.in, c.out, c.opt);
         break;
     }
+  } else if (c.lanes == kTwoWaveLanes && c.layout != Layout::Ref) {
+    {
+      // latency: Minv and the RNEA + columns on two waves per knot
+      const int64_t two_wave_blocks = (c.n + rbd::kTwoWaveKnots - 1) / rbd::kTwoWaveKnots;
+      const dim3 g2{unsigned(two_wave_blocks), 1, 1}, b2{2 * kBlock, 1, 1};
+      if (c.layout == Layout::AoS)
+        hipLaunchKernelGGL((rbd::fdgrad_two_wave_kernel<P, false>), g2, b2, 0, c.stream, n, c.in, 0L, c.out, 0L,
+                           c.opt);
+      else
+        hipLaunchKernelGGL((rbd::fdgrad_two_wave_kernel<P, true>), g2, b2, 0, c.stream, n, c.in, long(c.ld_in),
+                           c.out, long(c.ld_out), c.opt);
+    }
   } else {
-    const int64_t waves = (c.n * c.lanes + 63) / 64;
+    const int64_t waves = (c.n * rbd::kLanesPerKnotGroup + 63) / 64;
     if (c.simds > 0 && waves <= c.simds) launch_lanes<P, 1>(c, grid, block);
     else launch_lanes<P, rbd::kWavesPerSimd<P>>(c, grid, block);
   }
@@ -319,8 +408,9 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
     return fail(RBD_ERR_INVALID_ARGUMENT, "unknown flag bits");
   for (int r : c.reserved)
     if (r != 0) return fail(RBD_ERR_INVALID_ARGUMENT, "reserved config fields must be zero");
-  if (c.lanes_per_knot != 0 && c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup)
-    return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 0, 1 or 16");
+  if (c.lanes_per_knot != 0 && c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup &&
+      c.lanes_per_knot != kTwoWaveLanes)
+    return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 0, 1, 16 or 32");
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev <= 0) return fail(RBD_ERR_NO_DEVICE, "no HIP device visible");
@@ -334,6 +424,7 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   ctx->flags = c.flags;
   ctx->precision = c.precision;
   ctx->lanes_per_knot = c.lanes_per_knot;
+  ctx->two_wave_threshold.store(int64_t(2) * 4 * ctx->compute_units);
   ctx->opt.damping = (c.flags & RBD_FLAG_VEL_DAMPING) ? 0.5f : 0.0f;
   ctx->opt.gravity = (c.flags & RBD_FLAG_NO_GRAVITY) ? 0.0f : float(rbd::kGravity);
   *out = ctx;
@@ -359,7 +450,7 @@ rbd_status rbd_fdgrad_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, f
   if (n == 0) return RBD_OK;
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (rbd_status s = use_device(ctx)) return s;
-  const FlatCall c{Layout::AoS, lanes_for(ctx, n), n, d_in, 0, d_out, 0, ctx->opt,
+  const FlatCall c{Layout::AoS, lanes_for(ctx, n, true), n, d_in, 0, d_out, 0, ctx->opt,
                    (hipStream_t)stream, 4 * ctx->compute_units};
   hipError_t e = launch_flat(ctx->precision, c);
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
@@ -373,7 +464,7 @@ rbd_status rbd_fdgrad_traj_soa_f32(rbd_context* ctx, int64_t n, const float* d_i
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (ld_in < n || ld_out < n) return fail(RBD_ERR_INVALID_ARGUMENT, "leading dimension smaller than knot count");
   if (rbd_status s = use_device(ctx)) return s;
-  const FlatCall c{Layout::SoA, lanes_for(ctx, n), n, d_in, ld_in, d_out, ld_out, ctx->opt,
+  const FlatCall c{Layout::SoA, lanes_for(ctx, n, true), n, d_in, ld_in, d_out, ld_out, ctx->opt,
                    (hipStream_t)stream, 4 * ctx->compute_units};
   hipError_t e = launch_flat(ctx->precision, c);
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
@@ -414,7 +505,7 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
   float* d_out = ctx->d_scratch + size_t(n) * rbd::kInPerKnot;
   hipError_t e = hipMemcpy(d_in, h_in, in_bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
-  const FlatCall c{Layout::AoS, lanes_for(ctx, n), n, d_in, 0, d_out, 0, ctx->opt,
+  const FlatCall c{Layout::AoS, lanes_for(ctx, n, true), n, d_in, 0, d_out, 0, ctx->opt,
                    (hipStream_t)0, 4 * ctx->compute_units};
   e = launch_flat(ctx->precision, c);
   if (e != hipSuccess) return hip_fail(e, "fdgrad kernel launch");
@@ -425,17 +516,21 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
 
 const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knots) {
   if (!ctx) return "";
-  static const char* names[2][2][4] = {
+  static const char* names[2][3][4] = {
       {{"fdgrad_aos_kernel<PrecFP32>", "fdgrad_aos_kernel<PrecFP64Pivot>", "fdgrad_aos_kernel<PrecFP64Col>",
         "fdgrad_aos_kernel<PrecFP64>"},
        {"fdgrad_aos_lanes_kernel<PrecFP32>", "fdgrad_aos_lanes_kernel<PrecFP64Pivot>",
-        "fdgrad_aos_lanes_kernel<PrecFP64Col>", "fdgrad_aos_lanes_kernel<PrecFP64>"}},
+        "fdgrad_aos_lanes_kernel<PrecFP64Col>", "fdgrad_aos_lanes_kernel<PrecFP64>"},
+       {"fdgrad_two_wave_kernel<PrecFP32>", "fdgrad_two_wave_kernel<PrecFP64Pivot>",
+        "fdgrad_two_wave_kernel<PrecFP64Col>", "fdgrad_two_wave_kernel<PrecFP64>"}},
       {{"fdgrad_refmode_kernel<PrecFP32>", "fdgrad_refmode_kernel<PrecFP64Pivot>",
         "fdgrad_refmode_kernel<PrecFP64Col>", "fdgrad_refmode_kernel<PrecFP64>"},
        {"fdgrad_refmode_lanes_kernel<PrecFP32>", "fdgrad_refmode_lanes_kernel<PrecFP64Pivot>",
-        "fdgrad_refmode_lanes_kernel<PrecFP64Col>", "fdgrad_refmode_lanes_kernel<PrecFP64>"}}};
-  const int grouped = lanes_for(ctx, n_knots) == rbd::kLanesPerKnotGroup ? 1 : 0;
-  return names[reference_mode ? 1 : 0][grouped][ctx->precision];
+        "fdgrad_refmode_lanes_kernel<PrecFP64Col>", "fdgrad_refmode_lanes_kernel<PrecFP64>"},
+       {"", "", "", ""}}};
+  const int lanes = lanes_for(ctx, n_knots, !reference_mode);
+  const int mapping = lanes == 1 ? 0 : (lanes == kTwoWaveLanes ? 2 : 1);
+  return names[reference_mode ? 1 : 0][mapping][ctx->precision];
 }
 
 rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, float* d_out, float* d_minv,
@@ -626,6 +721,15 @@ rbd_status rbd_gradient_finish_f32(rbd_context* ctx, int64_t n, float* d_dqdd, c
 
 int64_t rbd_group_threshold(rbd_context* ctx) { return ctx ? ctx->group_threshold.load() : -1; }
 
+int64_t rbd_two_wave_threshold(rbd_context* ctx) { return ctx ? ctx->two_wave_threshold.load() : -1; }
+
+rbd_status rbd_set_two_wave_threshold(rbd_context* ctx, int64_t n_knots) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative threshold");
+  ctx->two_wave_threshold.store(n_knots);
+  return RBD_OK;
+}
+
 rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots) {
   if (rbd_status s = check_ctx(ctx)) return s;
   if (n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative threshold");
@@ -650,11 +754,12 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&t0);
   if (e == hipSuccess) e = hipEventCreate(&t1);
-  int64_t threshold = 0;
-  float best_us[2][kSizes] = {};
+  int64_t threshold = 0, two_wave = 0;
+  static const int kLanes[3] = {1, rbd::kLanesPerKnotGroup, kTwoWaveLanes};
+  float best_us[3][kSizes] = {};
   for (int si = 0; si < kSizes && e == hipSuccess; ++si) {
-    for (int li = 0; li < 2 && e == hipSuccess; ++li) {
-      const FlatCall c{Layout::AoS, li ? rbd::kLanesPerKnotGroup : 1, sizes[si], in, 0, out, 0, ctx->opt,
+    for (int li = 0; li < 3 && e == hipSuccess; ++li) {
+      const FlatCall c{Layout::AoS, kLanes[li], sizes[si], in, 0, out, 0, ctx->opt,
                        (hipStream_t)stream, 4 * ctx->compute_units};
       float best = 1e30f;
       for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
@@ -670,16 +775,19 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
       best_us[li][si] = best;
     }
     if (e == hipSuccess && best_us[1][si] < best_us[0][si]) threshold = sizes[si];
+    if (e == hipSuccess && best_us[2][si] < best_us[1][si] && best_us[2][si] < best_us[0][si])
+      two_wave = sizes[si];
   }
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);
   (void)hipFree(buf);
   if (e != hipSuccess) return hip_fail(e, "autotune");
   ctx->group_threshold.store(threshold);
+  ctx->two_wave_threshold.store(two_wave);
   if (threshold_out) *threshold_out = threshold;
   if (table_us)
-    for (int i = 0; i < table_len && i < 3 * kSizes; ++i)
-      table_us[i] = (i % 3 == 0) ? float(sizes[i / 3]) : best_us[(i % 3) - 1][i / 3];
+    for (int i = 0; i < table_len && i < 4 * kSizes; ++i)
+      table_us[i] = (i % 4 == 0) ? float(sizes[i / 4]) : best_us[(i % 4) - 1][i / 4];
   return RBD_OK;
 }
 

@@ -36,7 +36,8 @@ EXPORTED = [
     "rbd_config_default", "rbd_create", "rbd_destroy", "rbd_status_string", "rbd_last_error",
     "rbd_abi_version", "rbd_fdgrad_traj_f32", "rbd_fdgrad_traj_soa_f32",
     "rbd_dynamics_gradient_kernel_f32", "rbd_fdgrad_traj_host_f32", "rbd_kernel_name",
-    "rbd_autotune", "rbd_group_threshold", "rbd_set_group_threshold",
+    "rbd_autotune", "rbd_group_threshold", "rbd_set_group_threshold", "rbd_two_wave_threshold",
+    "rbd_set_two_wave_threshold",
     "rbd_autotune_batch", "rbd_batch_group_threshold", "rbd_set_batch_group_threshold",
     "rbd_batch_create", "rbd_batch_destroy", "rbd_fdgrad_batched_f32", "rbd_batch_set_grid",
     "rbd_batch_last_lanes",
@@ -130,6 +131,9 @@ def lib():
         L.rbd_group_threshold.argtypes = [vp]
         L.rbd_group_threshold.restype = i64
         L.rbd_set_group_threshold.argtypes = [vp, i64]
+        L.rbd_two_wave_threshold.argtypes = [vp]
+        L.rbd_two_wave_threshold.restype = i64
+        L.rbd_set_two_wave_threshold.argtypes = [vp, i64]
         L.rbd_autotune_batch.argtypes = [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         L.rbd_batch_group_threshold.argtypes = [vp]
         L.rbd_batch_group_threshold.restype = i64
@@ -343,13 +347,22 @@ class Context:
         return lib().rbd_kernel_name(self._h, int(bool(reference_mode)), int(n_knots)).decode()
 
     def autotune(self, stream=None):
-        """Time both knot-to-lane mappings on the device; sets and returns the
-        crossover plus the table {n: (us one-lane, us lane-group)}."""
+        """Time the three knot-to-lane mappings on the device; sets and returns
+        the crossover plus the table {n: (us one-lane, us lane-group, us two-wave)}
+        (the two-wave limit is then self.two_wave_threshold)."""
         thr = ctypes.c_int64()
-        tab = (ctypes.c_float * 27)()
-        _check(lib().rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 27), "rbd_autotune")
-        table = {int(tab[i]): (tab[i + 1], tab[i + 2]) for i in range(0, 27, 3)}
+        tab = (ctypes.c_float * 36)()
+        _check(lib().rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 36), "rbd_autotune")
+        table = {int(tab[i]): (tab[i + 1], tab[i + 2], tab[i + 3]) for i in range(0, 36, 4)}
         return thr.value, table
+
+    @property
+    def two_wave_threshold(self):
+        return lib().rbd_two_wave_threshold(self._h)
+
+    @two_wave_threshold.setter
+    def two_wave_threshold(self, n):
+        _check(lib().rbd_set_two_wave_threshold(self._h, int(n)), "rbd_set_two_wave_threshold")
 
     def autotune_batch(self, stream=None):
         """Time both mappings of the batched path; sets and returns its crossover

@@ -71,7 +71,10 @@ typedef struct rbd_config {
   unsigned flags;     /* RBD_FLAG_* */
   int precision;      /* rbd_precision */
   int lanes_per_knot; /* 0: chosen per call from the batch size; 1: one lane per
-                         knot (throughput); 16: one knot per 16-lane group (latency) */
+                         knot (throughput); 16: one knot per 16-lane group (latency);
+                         32: one knot per 16-lane group in each of two waves, Minv
+                         on one and the RNEA and columns on the other (lowest
+                         latency; full path only, other calls use 16) */
   int reserved[4];    /* must be zero */
 } rbd_config;
 
@@ -200,19 +203,24 @@ rbd_status rbd_gradient_finish_f32(rbd_context* ctx, int64_t n_knots, float* d_d
  * it (template arguments abbreviated).  No reference counterpart. */
 const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knots);
 
-/* Autotuner for the knot-to-lane mapping (lanes_per_knot = 0): times both
- * mappings on the device at batch sizes 64 ... 65536 and sets the largest size
- * at which the 16-lane-group mapping is faster as the context's crossover.
+/* Autotuner for the knot-to-lane mapping (lanes_per_knot = 0): times the
+ * three mappings (1, 16, 32 lanes per knot) on the device at batch sizes
+ * 64 ... 65536; sets the largest size at which the 16-lane-group mapping beats
+ * one lane per knot as the context's crossover (returned in threshold_out) and
+ * the largest at which the two-wave mapping beats both as the two-wave limit.
  * Allocates and frees a temporary device buffer; synchronises `stream`.  Not
  * to be called while other threads launch on the same context.  table_us, if
- * not NULL, receives up to table_len floats as (n, us one-lane, us lane-group)
- * triples.  No reference counterpart (the reference fixes dim3(32,7) per knot,
- * time_GPU.cu:10-19). */
+ * not NULL, receives up to table_len floats as (n, us one-lane, us lane-group,
+ * us two-wave) quadruples.  No reference counterpart (the reference fixes
+ * dim3(32,7) per knot, time_GPU.cu:10-19). */
 rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                         int table_len);
-/* Current crossover (knots) and its manual override. */
+/* Current crossover (knots) and its manual override; the same for the
+ * two-wave limit (calls of up to that many knots use 32 lanes per knot). */
 int64_t rbd_group_threshold(rbd_context* ctx);
 rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots);
+int64_t rbd_two_wave_threshold(rbd_context* ctx);
+rbd_status rbd_set_two_wave_threshold(rbd_context* ctx, int64_t n_knots);
 
 /* The batched path (rbd_fdgrad_batched_f32) has its own crossover on a
  * batch's total knots: its persistent kernel pays for the queue, so the

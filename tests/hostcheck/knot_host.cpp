@@ -48,6 +48,27 @@ static void by_precision(int prec, bool ref, int flags, long n, const float* in,
   }
 }
 
+// two-wave latency mapping, emulated: for each knot, stage M1, R1a and R1b
+// for every lane role, M2 for every role, R2 for every role (an order the
+// kernel's flags allow)
+template <class P>
+static void two_wave(long n, const float* in, float* out, int flags) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    InAoS i{in + kInPerKnot * k};
+    OutAoS o{out + kOutPerKnot * k};
+    TwoWaveShared<P> sh{};
+    CS<TwoWaveTrig<P>> tw[NQ];
+    typename P::Minv Mi[NQ][NQ];
+    typename P::Col tau[kLanesPerKnotGroup][NQ];
+    static TwoWaveR<P> st[kLanesPerKnotGroup];
+    two_wave_m1<P>(i, tw, Mi);
+    for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_r1a<P>(i, opt, role, sh, st[role]);
+    for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_r1b<P>(opt, role, st[role], tau[role]);
+    for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_m2<P>(role, tw, Mi, sh, o);
+    for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_r2<P>(role, sh, tau[role], o);
+  }
+}
 extern "C" {
 // The device model tables: spatial inertia of link k (6x6 column-major) and the
 // joint transform T_k(q) (6x6 column-major, built by applying T to unit vectors).
@@ -160,6 +181,16 @@ void host_finish_f32(long n, const float* minv, const float* cdu, float* dqdd) {
     InFinish i{minv + NQ * NQ * k, cdu + kDqddPerKnot * k};
     OutFinish o{dqdd + kDqddPerKnot * k};
     finish_knot(i, o);
+  }
+}
+
+void host_fdgrad_two_wave_f32(long n, const float* in, float* out, int flags, int prec) {
+  switch (prec) {
+    case 0: two_wave<PrecFP32>(n, in, out, flags); break;
+    case 1: two_wave<PrecFP64Pivot>(n, in, out, flags); break;
+    case 2: two_wave<PrecFP64Col>(n, in, out, flags); break;
+    case 3: two_wave<PrecFP64>(n, in, out, flags); break;
+    default: break;
   }
 }
 

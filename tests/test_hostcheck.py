@@ -31,6 +31,7 @@ def host():
     L.host_id_gradient_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.host_vaf_f32.argtypes = [ctypes.c_long, fp, ctypes.c_int]
     L.host_finish_f32.argtypes = [ctypes.c_long, fp, fp, fp]
+    L.host_fdgrad_two_wave_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int]
     return L
 
 
@@ -197,3 +198,22 @@ def test_staged_variants(host, oracle, flags):
     ref = oracle.finish(minv32.astype(np.float64), cdu32.astype(np.float64))
     err = {k: float(v.max()) for k, v in common.dqdd_block_errors(dqdd, ref).items()}
     assert max(err.values()) < 1e-6, err
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_two_wave_mapping(host, oracle, flags, prec):
+    """The two-wave latency mapping (Minv on one wave, RNEA and columns with
+    the bias accelerations plus qdd corrections on the other) against the
+    oracle, at random and stretched-arm poses."""
+    n = 300
+    x = np.ascontiguousarray(common.knots(n, seed=280 + flags, near_singular_frac=0.1))
+    out = np.full((n, 105), np.nan, np.float32)
+    host.host_fdgrad_two_wave_f32(n, _p(x), _p(out), flags, prec)
+    assert np.all(np.isfinite(out))
+    err = common.max_block_errors(out, oracle.fdgrad(x.astype(np.float64), flags))
+    bar = common.REL_TOL if prec in (0, 1) else 1e-5
+    assert max(err.values()) < bar, err
+    # the same values as the one-knot-per-lane-group mapping up to rounding
+    ref = run(host, x, flags, prec, lanes=1)
+    assert np.abs(out - ref).max() <= 1e-3 * np.abs(ref).max()
