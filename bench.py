@@ -33,6 +33,15 @@ METRIC = "dynamics-gradient knot points/s @<=1e-4 rel err vs FP64; served p50/p9
 BYTES_PER_KNOT = 504          # 21 FP32 in + 105 FP32 out
 FLOPS_PER_KNOT = 31340        # algorithmic count of the reference method (oracle op count)
 HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (x2.000 and x1.000, the factors a device-to-device copy of known size gives):
+# profiles/fdgrad_per_call_n128_and_large_batch_2p20.md.  Keyed by (kernel,
+# knots per launch); other configurations report null.
+MEASURED_TRAFFIC_BYTES = {
+    ("fdgrad_two_wave_kernel<PrecFP32>", 128): int((273.5 * 2.000 + 53.0 * 1.000) * 1024),
+    ("fdgrad_soa_kernel<PrecFP32>", 1 << 20): int((41188.1 * 2.000 + 430080.0) * 1024),
+    ("fdgrad_aos_kernel<PrecFP32>", 1 << 20): int((61694.9 * 2.000 + 1066783.0) * 1024),
+}
 FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 (packed vector / matrix) peak
 
 
@@ -338,7 +347,10 @@ def main():
                 workloads[f"large_batch_2p20_{prec}_{layout}"] = {
                     "knots_per_launch": n4, "ms_per_launch": per, "knots_per_s": n4 / (per * 1e-3),
                     "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_tflops": tfl,
-                    "fp32_frac": tfl / FP32_PEAK_TFLOPS, "accuracy": accuracy(out4, x4, sample=8192)}
+                    "fp32_frac": tfl / FP32_PEAK_TFLOPS,
+                    "traffic_bytes": MEASURED_TRAFFIC_BYTES.get(
+                        (f"fdgrad_{layout}_kernel<PrecFP32>", n4)) if prec == "fp32" else None,
+                    "accuracy": accuracy(out4, x4, sample=8192)}
                 log(f"[c4] {workloads[f'large_batch_2p20_{prec}_{layout}']}")
             if c4 is not ctx:
                 c4.close()
@@ -364,7 +376,7 @@ def main():
                    "knots_per_call": n1, "layout": "aos", "precision": args.precision, "kernel": kname,
                    "lane_group_threshold": threshold, "two_wave_threshold": ctx.two_wave_threshold},
         "roofline": {"bound": "hbm", "achieved": gbs1, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs1 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs1 / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_BYTES.get((kname, n1)),
                      "bytes_per_knot": BYTES_PER_KNOT, "knots_per_launch": n1},
         "roofline_fp32": {"bound": "fp32-valu", "achieved": tfl1, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": tfl1 / FP32_PEAK_TFLOPS, "flops_per_knot": FLOPS_PER_KNOT},

@@ -21,9 +21,10 @@ constexpr int kBlock = 64;   // one wave per workgroup
 // 23.3 vs 22.0 us at 16384, 24.9 vs 39.2 us at 32768).
 constexpr int64_t kDefaultGroupThreshold = 16384;
 // The two-wave mapping (Minv and the RNEA + columns of a knot on two waves)
-// serves calls up to the context's two_wave_threshold knots: by default the
-// largest call whose workgroups (two waves per 4 knots) all fit the SIMDs at
-// once, 2 x SIMDs knots; rbd_autotune measures it.
+// serves calls up to the context's two_wave_threshold knots: by default
+// 16 x compute units (4,096 knots on 256 CUs, the crossover rbd_autotune
+// measured on MI355X: 8.4 us against 9.0 us for one-wave lane groups at 4,096
+// knots, 18.4 against 12.6 us at 8,192); rbd_autotune measures it.
 constexpr int kTwoWaveLanes = 32;   // lanes_per_knot value of the two-wave mapping (16 lanes in each of 2 waves)
 // The same crossover for batches (rbd_batch_*, total knots of the batch),
 // measured by rbd_autotune_batch on batches of 132-knot trajectories.

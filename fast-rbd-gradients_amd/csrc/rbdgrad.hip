@@ -424,7 +424,7 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   ctx->flags = c.flags;
   ctx->precision = c.precision;
   ctx->lanes_per_knot = c.lanes_per_knot;
-  ctx->two_wave_threshold.store(int64_t(2) * 4 * ctx->compute_units);
+  ctx->two_wave_threshold.store(int64_t(16) * ctx->compute_units);
   ctx->opt.damping = (c.flags & RBD_FLAG_VEL_DAMPING) ? 0.5f : 0.0f;
   ctx->opt.gravity = (c.flags & RBD_FLAG_NO_GRAVITY) ? 0.0f : float(rbd::kGravity);
   *out = ctx;
