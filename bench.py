@@ -9,9 +9,14 @@ kernel on one trajectory of N = 128 knots, one launch per call; the large batch
 are reported beside it under "workloads".  Knot points/s is also given as a
 fraction of the HBM roofline (504 algorithmic bytes per knot: 84 in, 420 out)
 and of the FP32 roofline (31,340 algorithmic FLOP per knot, the op count of
-the oracle's restatement of the reference method).  The CPU baseline is the
-oracle's FP32 restatement of the reference's multithreaded path (time_CPU.cpp
-ReusableThreads loop) timed on this host.
+the oracle's restatement of the reference method).  The same per-call and
+large-batch workloads also run on a robot read from a URDF (the synthetic arm
+of tests/golden/synthetic_arm7.urdf, librbdgrad_arm7.so).
+
+The CPU leg comes last: the CPU baseline is the oracle's FP32 restatement of
+the reference's multithreaded path (time_CPU.cpp ReusableThreads loop) timed
+on this host, and the FP64 oracle then checks a sample of every GPU output
+recorded above ("accuracy" entries; skipped with --no-cpu).
 """
 import argparse
 import json
@@ -32,6 +37,7 @@ import rbdgrad  # noqa: E402
 METRIC = "dynamics-gradient knot points/s @<=1e-4 rel err vs FP64; served p50/p99 latency"
 BYTES_PER_KNOT = 504          # 21 FP32 in + 105 FP32 out
 FLOPS_PER_KNOT = 31340        # algorithmic count of the reference method (oracle op count)
+FLOPS_PER_KNOT_ARM7 = 31464   # the same for the synthetic arm of tests/golden/synthetic_arm7.urdf
 HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (x2.000 and x1.000, the factors a device-to-device copy of known size gives):
@@ -80,19 +86,44 @@ def time_launches(launch, steps, warmup, stream):
     return ms
 
 
-def accuracy(out, x, sample=None):
-    """Max relative error per block vs the FP64 oracle on the same FP32 inputs."""
-    from oracle import pyoracle
-    idx = np.arange(len(x)) if sample is None or sample >= len(x) else \
-        np.random.default_rng(5).choice(len(x), sample, replace=False)
-    ref = pyoracle.fdgrad(x[idx].astype(np.float64), threads=min(16, os.cpu_count() or 1))
-    o = out[idx].astype(np.float64)
-    res = {}
-    for name, sl in (("qdd", slice(0, 7)), ("dqdd_dq", slice(7, 56)), ("dqdd_dqd", slice(56, 105))):
-        e = np.max(np.abs(o[:, sl] - ref[:, sl]), axis=1) / np.maximum(np.max(np.abs(ref[:, sl]), axis=1), 1e-6)
-        res[name] = {"max": float(e.max()), "p99.9": float(np.percentile(e, 99.9))}
-    res["knots_checked"] = int(len(idx))
-    return res
+class OracleChecks:
+    """GPU outputs kept for the CPU leg, which checks them against the FP64
+    oracle (max relative error per block on the same FP32 inputs).  A sample of
+    rows is taken when the output is recorded, so large outputs are not held;
+    the result is written into target[key] when the CPU leg runs."""
+
+    def __init__(self):
+        self.items = []
+
+    def add(self, target, key, out, x, sample=None, model=None):
+        idx = np.arange(len(x)) if sample is None or sample >= len(x) else \
+            np.random.default_rng(5).choice(len(x), sample, replace=False)
+        target[key] = {"pending": "checked against the FP64 oracle in the CPU leg"}
+        self.items.append((target, key, out[idx].astype(np.float64), x[idx].astype(np.float64), model))
+
+    def run(self):
+        from oracle import pyoracle
+        threads = min(16, os.cpu_count() or 1)
+        for target, key, o, x, model in self.items:
+            if model is not None:
+                with pyoracle.using_model(*model):
+                    ref = pyoracle.fdgrad(x, threads=threads)
+            else:
+                ref = pyoracle.fdgrad(x, threads=threads)
+            res = {}
+            for name, sl in (("qdd", slice(0, 7)), ("dqdd_dq", slice(7, 56)), ("dqdd_dqd", slice(56, 105))):
+                e = np.max(np.abs(o[:, sl] - ref[:, sl]), axis=1) / np.maximum(np.max(np.abs(ref[:, sl]), axis=1),
+                                                                               1e-6)
+                res[name] = {"max": float(e.max()), "p99.9": float(np.percentile(e, 99.9))}
+            res["knots_checked"] = int(len(x))
+            target[key] = res
+
+    def skip(self, why):
+        for target, key, *_ in self.items:
+            target[key] = {"skipped": why}
+
+
+CHECKS = OracleChecks()
 
 
 def roofline(knots_per_launch, ms_per_launch):
@@ -181,7 +212,8 @@ def mixed_workload(ctx, stream, steps=500):
         res["mixed_64_traj_batched" + ("_graph" if use_graph else "")] = {
             "knots_per_batch": n2, "ms_per_batch": ms, "knots_per_s": n2 / (ms * 1e-3),
             "lanes_per_knot": batch.last_lanes, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
-            "fp32_tflops": tfl, "fp32_frac": tfl / FP32_PEAK_TFLOPS, "accuracy": accuracy(out, x2, sample=4096)}
+            "fp32_tflops": tfl, "fp32_frac": tfl / FP32_PEAK_TFLOPS}
+        CHECKS.add(res["mixed_64_traj_batched" + ("_graph" if use_graph else "")], "accuracy", out, x2, sample=4096)
         log(f"[c2] {res['mixed_64_traj_batched' + ('_graph' if use_graph else '')]}")
     batch.close()
     for d_in, d_out, _ in bufs:
@@ -207,6 +239,44 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000), duration_s=1.
             res[f"served_{mode}_{rate}_req_per_s"] = st
             log(f"[c3] {st}")
         srv.close()
+    return res
+
+
+def urdf_model_workload(stream, steps, warmup, name="arm7",
+                        urdf=os.path.join(ROOT, "tests", "golden", "synthetic_arm7.urdf")):
+    """A robot read from a URDF: librbdgrad_arm7.so, compiled on the tables that
+    urdf_model.py generates from the synthetic 7-joint arm (joint frames at
+    general angles, so the transforms have no structural zeros).  The same
+    per-call and 2^20-knot SoA workloads as for the iiwa."""
+    if not os.path.exists(rbdgrad.lib_path(name)):
+        log(f"[model] {rbdgrad.lib_path(name)} not built: skipped")
+        return {}
+    import urdf_model
+    tab = urdf_model.tables(urdf_model.read_urdf(urdf))
+    tables = (np.array(tab["links"]),
+              np.concatenate([np.array(tab["rot"]).reshape(7, 9), np.array(tab["offset"])], axis=1))
+    ctx = rbdgrad.Context(device=0, model=name)
+    ctx.autotune(stream)
+    res = {}
+    x1 = knots(128, seed=101)
+    ms, out = run_flat(ctx, x1, steps, warmup, stream)
+    r = res[f"urdf_model_{name}_per_call_n128"] = {
+        "kernel": ctx.kernel_name(128), "us_per_launch": ms / steps * 1e3, "knots_per_s": 128 * steps / (ms * 1e-3)}
+    CHECKS.add(r, "accuracy", out, x1, model=tables)
+    log(f"[model] {name} per call: {r}")
+    n4 = 1 << 20
+    x4 = knots(n4, seed=104)
+    ms, out = run_flat(ctx, x4, 20, 3, stream, "soa")
+    per = ms / 20
+    gbs, _ = roofline(n4, per)
+    tfl = n4 * FLOPS_PER_KNOT_ARM7 / (per * 1e-3) / 1e12
+    r = res[f"urdf_model_{name}_2p20_fp32_soa"] = {
+        "knots_per_launch": n4, "ms_per_launch": per, "knots_per_s": n4 / (per * 1e-3), "hbm_gbs": gbs,
+        "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_tflops": tfl, "fp32_frac": tfl / FP32_PEAK_TFLOPS,
+        "flops_per_knot": FLOPS_PER_KNOT_ARM7}
+    CHECKS.add(r, "accuracy", out, x4, sample=8192, model=tables)
+    log(f"[model] {name} 2^20 SoA: {r}")
+    ctx.close()
     return res
 
 
@@ -305,9 +375,10 @@ def main():
     ms_step = ms1 / args.steps
     value = n1 * args.steps / (ms1 * 1e-3)
     gbs1, tfl1 = roofline(n1, ms_step)
-    acc1 = accuracy(out1, x1)
+    checked = {}
+    CHECKS.add(checked, "accuracy", out1, x1)
     kname = ctx.kernel_name(n1)
-    log(f"[c1] {kname}: {value:.3e} knots/s, {ms_step * 1e3:.2f} us/launch, acc {acc1}")
+    log(f"[c1] {kname}: {value:.3e} knots/s, {ms_step * 1e3:.2f} us/launch")
 
     workloads = {"autotune": {"group_threshold_knots": threshold,
                               "two_wave_threshold_knots": ctx.two_wave_threshold,
@@ -349,15 +420,24 @@ def main():
                     "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_tflops": tfl,
                     "fp32_frac": tfl / FP32_PEAK_TFLOPS,
                     "traffic_bytes": MEASURED_TRAFFIC_BYTES.get(
-                        (f"fdgrad_{layout}_kernel<PrecFP32>", n4)) if prec == "fp32" else None,
-                    "accuracy": accuracy(out4, x4, sample=8192)}
+                        (f"fdgrad_{layout}_kernel<PrecFP32>", n4)) if prec == "fp32" else None}
+                CHECKS.add(workloads[f"large_batch_2p20_{prec}_{layout}"], "accuracy", out4, x4, sample=8192)
                 log(f"[c4] {workloads[f'large_batch_2p20_{prec}_{layout}']}")
             if c4 is not ctx:
                 c4.close()
         workloads.update(mixed_workload(ctx, stream))
         workloads.update(served_workload(ctx))
+        workloads.update(urdf_model_workload(stream, args.steps // 4, args.warmup))
 
-    cpu = None if args.no_cpu else cpu_baseline(x1, args.cpu_seconds)
+    # ---- CPU leg: the oracle as the CPU baseline, and as the checker of the GPU outputs ----
+    if args.no_cpu:
+        cpu = None
+        CHECKS.skip("--no-cpu")
+    else:
+        cpu = cpu_baseline(x1, args.cpu_seconds)
+        log("checking the GPU outputs against the FP64 oracle")
+        CHECKS.run()
+    acc1 = checked["accuracy"]
 
     line = {
         "metric": METRIC,

@@ -22,7 +22,13 @@
 // The file is plain C++17 when RBD_HD is empty, so the same arithmetic is also
 // compiled for the host by the CPU test harness.
 #pragma once
+// The robot: the KUKA iiwa 14 unless the build names a header generated from a
+// URDF by urdf_model.py (-DRBD_MODEL_HEADER='"model_NAME.hpp"').
+#ifdef RBD_MODEL_HEADER
+#include RBD_MODEL_HEADER
+#else
 #include "iiwa_model.hpp"
+#endif
 #include <type_traits>
 
 #if defined(__HIPCC__)

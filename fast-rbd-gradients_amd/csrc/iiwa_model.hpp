@@ -12,6 +12,7 @@
 
 namespace rbd {
 
+constexpr const char* kModelName = "iiwa14";
 constexpr int NQ = 7;
 constexpr double kGravity = 9.81;
 

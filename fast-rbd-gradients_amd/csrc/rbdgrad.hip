@@ -374,6 +374,8 @@ extern "C" {
 
 int rbd_abi_version(void) { return 1; }
 
+const char* rbd_model_name(void) { return rbd::kModelName; }
+
 void rbd_config_default(rbd_config* cfg) {
   if (!cfg) return;
   std::memset(cfg, 0, sizeof(*cfg));

@@ -13,6 +13,12 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librbdgrad.so")
 
+
+def lib_path(model=None):
+    """librbdgrad.so (the iiwa 14), or librbdgrad_MODEL.so built from a URDF by
+    `make -C fast-rbd-gradients_amd MODEL=... URDF=...`."""
+    return LIB_PATH if not model else os.path.join(_HERE, f"librbdgrad_{model}.so")
+
 NQ = 7
 IN_PER_KNOT = 21
 OUT_PER_KNOT = 105
@@ -46,7 +52,7 @@ EXPORTED = [
     "rbd_server_run_poisson", "rbd_server_get_trace", "rbd_fdgrad_minv_traj_f32", "rbd_forward_dynamics_f32",
     "rbd_fdgrad_traj_fext_f32", "rbd_forward_dynamics_fext_f32",
     "rbd_dynamics_gradient_kernel_fused_f32", "rbd_dynamics_gradient_kernel_split_f32",
-    "rbd_gradient_setup_split_f32", "rbd_gradient_finish_f32",
+    "rbd_gradient_setup_split_f32", "rbd_gradient_finish_f32", "rbd_model_name",
 ]
 
 
@@ -93,18 +99,19 @@ class Config(ctypes.Structure):
                 ("lanes_per_knot", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
 
 
-_lib = None
+_libs = {}
 _hip = None
 
 
-def lib():
-    """Load librbdgrad.so (raises if it is not built)."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RbdError(f"{LIB_PATH} is not built: run __graft_entry__.build() or "
-                           f"`make -C fast-rbd-gradients_amd`")
-        L = ctypes.CDLL(LIB_PATH)
+def lib(model=None):
+    """Load librbdgrad.so, or librbdgrad_MODEL.so (raises if it is not built)."""
+    key = model or ""
+    if key not in _libs:
+        path = lib_path(model)
+        if not os.path.exists(path):
+            raise RbdError(f"{path} is not built: run __graft_entry__.build() or "
+                           f"`make -C fast-rbd-gradients_amd" + (f" MODEL={model} URDF=...`" if model else "`"))
+        L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         i64 = ctypes.c_int64
         L.rbd_config_default.argtypes = [ctypes.POINTER(Config)]
@@ -154,8 +161,9 @@ def lib():
         L.rbd_server_get_trace.argtypes = [vp, ctypes.POINTER(BatchRecord), i64, ctypes.POINTER(i64)]
         L.rbd_server_run_poisson.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(i64), ctypes.c_int,
                                              vp, i64, ctypes.c_uint64, ctypes.POINTER(ServerStats)]
-        _lib = L
-    return _lib
+        L.rbd_model_name.restype = ctypes.c_char_p
+        _libs[key] = L
+    return _libs[key]
 
 
 def hip():
@@ -194,9 +202,9 @@ def _hipcheck(err, what):
         raise HipError(f"{what}: {hip().hipGetErrorString(err).decode()} ({err})")
 
 
-def _check(status, what):
+def _check(status, what, L=None):
     if status != 0:
-        L = lib()
+        L = L or lib()
         raise RbdError(f"{what}: {L.rbd_status_string(status).decode()} -- {L.rbd_last_error().decode()}")
 
 
@@ -298,11 +306,21 @@ def _sh(stream):
     return stream.handle if stream is not None else None
 
 
-class Context:
-    """rbd_context: device, model flags, precision mode."""
+class _Bound:
+    """An object of one model library (librbdgrad.so or librbdgrad_MODEL.so)."""
+    _L = None
 
-    def __init__(self, device=0, flags=0, precision=PREC_FP32, lanes_per_knot=0):
-        L = lib()
+    def _check(self, status, what):
+        _check(status, what, self._L)
+
+
+class Context(_Bound):
+    """rbd_context: device, model flags, precision mode.  model=None is the
+    KUKA iiwa 14; model="name" loads librbdgrad_name.so, built from a URDF."""
+
+    def __init__(self, device=0, flags=0, precision=PREC_FP32, lanes_per_knot=0, model=None):
+        L = self._L = lib(model)
+        self.model = L.rbd_model_name().decode()
         cfg = Config()
         L.rbd_config_default(ctypes.byref(cfg))
         cfg.device = device
@@ -310,13 +328,13 @@ class Context:
         cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         cfg.lanes_per_knot = int(lanes_per_knot)
         self._h = ctypes.c_void_p()
-        _check(L.rbd_create(ctypes.byref(cfg), ctypes.byref(self._h)), "rbd_create")
+        self._check(L.rbd_create(ctypes.byref(cfg), ctypes.byref(self._h)), "rbd_create")
         self.flags = flags
         self.precision = cfg.precision
 
     def close(self):
         if self._h is not None and self._h.value:
-            lib().rbd_destroy(self._h)
+            self._L.rbd_destroy(self._h)
         self._h = None
 
     def __del__(self):
@@ -331,20 +349,20 @@ class Context:
 
     # --- device-pointer entry points (the drop-in calls) ---
     def fdgrad(self, n, d_in, d_out, stream=None):
-        _check(lib().rbd_fdgrad_traj_f32(self._h, int(n), d_in.handle, d_out.handle, _sh(stream)),
+        self._check(self._L.rbd_fdgrad_traj_f32(self._h, int(n), d_in.handle, d_out.handle, _sh(stream)),
                "rbd_fdgrad_traj_f32")
 
     def fdgrad_soa(self, n, d_in, ld_in, d_out, ld_out, stream=None):
-        _check(lib().rbd_fdgrad_traj_soa_f32(self._h, int(n), d_in.handle, int(ld_in), d_out.handle, int(ld_out),
+        self._check(self._L.rbd_fdgrad_traj_soa_f32(self._h, int(n), d_in.handle, int(ld_in), d_out.handle, int(ld_out),
                                              _sh(stream)), "rbd_fdgrad_traj_soa_f32")
 
     def dynamics_gradient_kernel(self, n, d_dqdd, d_vol, d_const=None, stream=None):
-        _check(lib().rbd_dynamics_gradient_kernel_f32(self._h, int(n), d_dqdd.handle, d_vol.handle,
+        self._check(self._L.rbd_dynamics_gradient_kernel_f32(self._h, int(n), d_dqdd.handle, d_vol.handle,
                                                       d_const.handle if d_const is not None else None,
                                                       _sh(stream)), "rbd_dynamics_gradient_kernel_f32")
 
     def kernel_name(self, n_knots, reference_mode=False):
-        return lib().rbd_kernel_name(self._h, int(bool(reference_mode)), int(n_knots)).decode()
+        return self._L.rbd_kernel_name(self._h, int(bool(reference_mode)), int(n_knots)).decode()
 
     def autotune(self, stream=None):
         """Time the three knot-to-lane mappings on the device; sets and returns
@@ -352,42 +370,42 @@ class Context:
         (the two-wave limit is then self.two_wave_threshold)."""
         thr = ctypes.c_int64()
         tab = (ctypes.c_float * 36)()
-        _check(lib().rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 36), "rbd_autotune")
+        self._check(self._L.rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 36), "rbd_autotune")
         table = {int(tab[i]): (tab[i + 1], tab[i + 2], tab[i + 3]) for i in range(0, 36, 4)}
         return thr.value, table
 
     @property
     def two_wave_threshold(self):
-        return lib().rbd_two_wave_threshold(self._h)
+        return self._L.rbd_two_wave_threshold(self._h)
 
     @two_wave_threshold.setter
     def two_wave_threshold(self, n):
-        _check(lib().rbd_set_two_wave_threshold(self._h, int(n)), "rbd_set_two_wave_threshold")
+        self._check(self._L.rbd_set_two_wave_threshold(self._h, int(n)), "rbd_set_two_wave_threshold")
 
     def autotune_batch(self, stream=None):
         """Time both mappings of the batched path; sets and returns its crossover
         (total knots) plus the table {total knots: (us one-lane, us lane-group)}."""
         thr = ctypes.c_int64()
         tab = (ctypes.c_float * 21)()
-        _check(lib().rbd_autotune_batch(self._h, _sh(stream), ctypes.byref(thr), tab, 21), "rbd_autotune_batch")
+        self._check(self._L.rbd_autotune_batch(self._h, _sh(stream), ctypes.byref(thr), tab, 21), "rbd_autotune_batch")
         table = {int(tab[i]): (tab[i + 1], tab[i + 2]) for i in range(0, 21, 3)}
         return thr.value, table
 
     @property
     def batch_group_threshold(self):
-        return lib().rbd_batch_group_threshold(self._h)
+        return self._L.rbd_batch_group_threshold(self._h)
 
     @batch_group_threshold.setter
     def batch_group_threshold(self, n):
-        _check(lib().rbd_set_batch_group_threshold(self._h, int(n)), "rbd_set_batch_group_threshold")
+        self._check(self._L.rbd_set_batch_group_threshold(self._h, int(n)), "rbd_set_batch_group_threshold")
 
     @property
     def group_threshold(self):
-        return lib().rbd_group_threshold(self._h)
+        return self._L.rbd_group_threshold(self._h)
 
     @group_threshold.setter
     def group_threshold(self, n):
-        _check(lib().rbd_set_group_threshold(self._h, int(n)), "rbd_set_group_threshold")
+        self._check(self._L.rbd_set_group_threshold(self._h, int(n)), "rbd_set_group_threshold")
 
     # --- host-array conveniences (upload, run, download) ---
     def fdgrad_host(self, x):
@@ -395,7 +413,7 @@ class Context:
         x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
         assert x.shape[1] == IN_PER_KNOT
         out = np.empty((x.shape[0], OUT_PER_KNOT), dtype=np.float32)
-        _check(lib().rbd_fdgrad_traj_host_f32(self._h, x.shape[0], x.ctypes.data, out.ctypes.data),
+        self._check(self._L.rbd_fdgrad_traj_host_f32(self._h, x.shape[0], x.ctypes.data, out.ctypes.data),
                "rbd_fdgrad_traj_host_f32")
         return out
 
@@ -426,7 +444,7 @@ class Context:
         d_in = DeviceBuffer.from_array(x)
         d_out = DeviceBuffer(n * OUT_PER_KNOT * 4)
         d_m = DeviceBuffer(n * 49 * 4)
-        _check(lib().rbd_fdgrad_minv_traj_f32(self._h, n, d_in.handle, d_out.handle, d_m.handle, None),
+        self._check(self._L.rbd_fdgrad_minv_traj_f32(self._h, n, d_in.handle, d_out.handle, d_m.handle, None),
                "rbd_fdgrad_minv_traj_f32")
         return d_out.download((n, OUT_PER_KNOT)), d_m.download((n, 7, 7)).transpose(0, 2, 1).copy()
 
@@ -439,11 +457,11 @@ class Context:
         d_q = DeviceBuffer(n * 7 * 4)
         d_m = DeviceBuffer(n * 49 * 4)
         if fext is None:
-            _check(lib().rbd_forward_dynamics_f32(self._h, n, d_in.handle, d_q.handle, d_m.handle, None),
+            self._check(self._L.rbd_forward_dynamics_f32(self._h, n, d_in.handle, d_q.handle, d_m.handle, None),
                    "rbd_forward_dynamics_f32")
         else:
             d_f, stride = _fext_buffer(fext, n)
-            _check(lib().rbd_forward_dynamics_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_q.handle,
+            self._check(self._L.rbd_forward_dynamics_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_q.handle,
                                                        d_m.handle, None), "rbd_forward_dynamics_fext_f32")
         return d_q.download((n, 7)), d_m.download((n, 7, 7)).transpose(0, 2, 1).copy()
 
@@ -454,7 +472,7 @@ class Context:
         d_in = DeviceBuffer.from_array(x)
         d_out = DeviceBuffer(n * OUT_PER_KNOT * 4)
         d_f, stride = _fext_buffer(fext, n)
-        _check(lib().rbd_fdgrad_traj_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_out.handle, None),
+        self._check(self._L.rbd_fdgrad_traj_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_out.handle, None),
                "rbd_fdgrad_traj_fext_f32")
         return d_out.download((n, OUT_PER_KNOT))
 
@@ -464,7 +482,7 @@ class Context:
         n = vol.shape[0]
         d_vol = DeviceBuffer.from_array(vol)
         d_cdu = DeviceBuffer(n * DQDD_PER_KNOT * 4)
-        _check(lib().rbd_dynamics_gradient_kernel_fused_f32(self._h, n, d_cdu.handle, d_vol.handle, None, None),
+        self._check(self._L.rbd_dynamics_gradient_kernel_fused_f32(self._h, n, d_cdu.handle, d_vol.handle, None, None),
                "rbd_dynamics_gradient_kernel_fused_f32")
         return d_cdu.download((n, DQDD_PER_KNOT))
 
@@ -474,7 +492,7 @@ class Context:
         n = vol.shape[0]
         d_vol = DeviceBuffer.from_array(vol)
         d_cdu = DeviceBuffer(n * DQDD_PER_KNOT * 4)
-        _check(lib().rbd_dynamics_gradient_kernel_split_f32(self._h, n, d_cdu.handle, d_vol.handle, None, None),
+        self._check(self._L.rbd_dynamics_gradient_kernel_split_f32(self._h, n, d_cdu.handle, d_vol.handle, None, None),
                "rbd_dynamics_gradient_kernel_split_f32")
         return d_cdu.download((n, DQDD_PER_KNOT))
 
@@ -485,7 +503,7 @@ class Context:
         vol = np.zeros((n, 154), np.float32)
         vol[:, :28] = vol28
         d_vol = DeviceBuffer.from_array(vol)
-        _check(lib().rbd_gradient_setup_split_f32(self._h, n, d_vol.handle, None), "rbd_gradient_setup_split_f32")
+        self._check(self._L.rbd_gradient_setup_split_f32(self._h, n, d_vol.handle, None), "rbd_gradient_setup_split_f32")
         return d_vol.download((n, 154))
 
     def gradient_finish(self, minv, cdu):
@@ -495,7 +513,7 @@ class Context:
         n = cdu.shape[0]
         d_m, d_c = DeviceBuffer.from_array(minv), DeviceBuffer.from_array(cdu)
         d_out = DeviceBuffer(n * DQDD_PER_KNOT * 4)
-        _check(lib().rbd_gradient_finish_f32(self._h, n, d_out.handle, d_c.handle, d_m.handle, None),
+        self._check(self._L.rbd_gradient_finish_f32(self._h, n, d_out.handle, d_c.handle, d_m.handle, None),
                "rbd_gradient_finish_f32")
         return d_out.download((n, DQDD_PER_KNOT))
 
@@ -523,13 +541,14 @@ def _fext_buffer(fext, n):
     return DeviceBuffer.from_array(f), FEXT_PER_SET
 
 
-class Batch:
+class Batch(_Bound):
     """rbd_batch: runs many trajectories of uneven length with one persistent-kernel launch."""
 
     def __init__(self, ctx, max_trajectories):
         self.ctx = ctx
+        self._L = ctx._L
         self._h = ctypes.c_void_p()
-        _check(lib().rbd_batch_create(ctx.handle, int(max_trajectories), ctypes.byref(self._h)),
+        self._check(self._L.rbd_batch_create(ctx.handle, int(max_trajectories), ctypes.byref(self._h)),
                "rbd_batch_create")
 
     def run(self, trajs, use_graph=False, stream=None):
@@ -538,19 +557,19 @@ class Batch:
         for i, (di, do, n) in enumerate(trajs):
             arr[i] = Traj(di.handle.value if di is not None else None, do.handle.value if do is not None else None,
                           int(n))
-        _check(lib().rbd_fdgrad_batched_f32(self._h, arr, len(trajs), int(bool(use_graph)), _sh(stream)),
+        self._check(self._L.rbd_fdgrad_batched_f32(self._h, arr, len(trajs), int(bool(use_graph)), _sh(stream)),
                "rbd_fdgrad_batched_f32")
 
     def set_grid(self, lanes_per_knot, waves):
-        _check(lib().rbd_batch_set_grid(self._h, int(lanes_per_knot), int(waves)), "rbd_batch_set_grid")
+        self._check(self._L.rbd_batch_set_grid(self._h, int(lanes_per_knot), int(waves)), "rbd_batch_set_grid")
 
     @property
     def last_lanes(self):
-        return lib().rbd_batch_last_lanes(self._h)
+        return self._L.rbd_batch_last_lanes(self._h)
 
     def close(self):
         if self._h is not None and self._h.value:
-            lib().rbd_batch_destroy(self._h)
+            self._L.rbd_batch_destroy(self._h)
         self._h = None
 
     def __del__(self):
@@ -560,66 +579,67 @@ class Batch:
             pass
 
 
-class Server:
+class Server(_Bound):
     """rbd_server: continuous batching of gradient requests (one trajectory each)."""
 
     def __init__(self, ctx, **cfg):
+        self._L = ctx._L
         c = ServerConfig()
-        lib().rbd_server_config_default(ctypes.byref(c))
+        self._L.rbd_server_config_default(ctypes.byref(c))
         for k, v in cfg.items():
             setattr(c, k, v)
         self.ctx = ctx
         self.config = c
         self._h = ctypes.c_void_p()
         self._keep = {}                      # request id -> (input, output) arrays, alive until collected
-        _check(lib().rbd_server_create(ctx.handle, ctypes.byref(c), ctypes.byref(self._h)), "rbd_server_create")
+        self._check(self._L.rbd_server_create(ctx.handle, ctypes.byref(c), ctypes.byref(self._h)), "rbd_server_create")
 
     def submit(self, x):
         """Queue one trajectory x (n, 21); returns the request id."""
         x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
         out = np.empty((x.shape[0], OUT_PER_KNOT), dtype=np.float32)
         rid = ctypes.c_uint64()
-        _check(lib().rbd_server_submit(self._h, x.ctypes.data, out.ctypes.data, x.shape[0], ctypes.byref(rid)),
+        self._check(self._L.rbd_server_submit(self._h, x.ctypes.data, out.ctypes.data, x.shape[0], ctypes.byref(rid)),
                "rbd_server_submit")
         self._keep[rid.value] = (x, out)
         return rid.value
 
     def poll(self, rid):
-        return lib().rbd_server_poll(self._h, rid)
+        return self._L.rbd_server_poll(self._h, rid)
 
     def wait(self, rid, timeout_s=60.0):
         """Block until done; returns the (n, 105) result."""
-        _check(lib().rbd_server_wait(self._h, rid, timeout_s), "rbd_server_wait")
+        self._check(self._L.rbd_server_wait(self._h, rid, timeout_s), "rbd_server_wait")
         return self._keep.pop(rid)[1]
 
     def stats(self):
         st = ServerStats()
-        _check(lib().rbd_server_get_stats(self._h, ctypes.byref(st)), "rbd_server_get_stats")
+        self._check(self._L.rbd_server_get_stats(self._h, ctypes.byref(st)), "rbd_server_get_stats")
         return st.as_dict()
 
     def trace(self, max_records=65536):
         """The most recent batch records, oldest first, as a structured array."""
         buf = (BatchRecord * int(max_records))()
         n = ctypes.c_int64()
-        _check(lib().rbd_server_get_trace(self._h, buf, int(max_records), ctypes.byref(n)), "rbd_server_get_trace")
+        self._check(self._L.rbd_server_get_trace(self._h, buf, int(max_records), ctypes.byref(n)), "rbd_server_get_trace")
         names = [f for f, _ in BatchRecord._fields_ if f != "reserved"]
         return {f: np.array([getattr(buf[i], f) for i in range(n.value)]) for f in names}
 
     def reset_stats(self):
-        _check(lib().rbd_server_reset_stats(self._h), "rbd_server_reset_stats")
+        self._check(self._L.rbd_server_reset_stats(self._h), "rbd_server_reset_stats")
 
     def run_poisson(self, rate, duration_s, lengths, pool, seed=0):
         lens = (ctypes.c_int64 * len(lengths))(*[int(v) for v in lengths])
         pool = np.ascontiguousarray(pool, dtype=np.float32)
         st = ServerStats()
-        _check(lib().rbd_server_run_poisson(self._h, float(rate), float(duration_s), lens, len(lengths),
+        self._check(self._L.rbd_server_run_poisson(self._h, float(rate), float(duration_s), lens, len(lengths),
                                             pool.ctypes.data, pool.shape[0], int(seed), ctypes.byref(st)),
                "rbd_server_run_poisson")
         return st.as_dict()
 
     def close(self):
         if self._h is not None and self._h.value:
-            lib().rbd_server_destroy(self._h)
+            self._L.rbd_server_destroy(self._h)
         self._h = None
 
     def __del__(self):

@@ -92,6 +92,12 @@ const char* rbd_status_string(rbd_status s);
 const char* rbd_last_error(void);
 /* Library ABI version (incremented on incompatible changes). */
 int rbd_abi_version(void);
+/* Name of the robot this library's kernels are compiled for: "iiwa14" for
+ * librbdgrad.so, NAME for librbdgrad_NAME.so built from a URDF
+ * (make -C fast-rbd-gradients_amd MODEL=NAME URDF=robot.urdf).
+ * Replaces: the reference's hard-coded iiwa tables (dynamicsGradient.h:123-229,
+ * dynamicsGradient.cuh:211-310); its README.md:1 points to GRiD for other robots. */
+const char* rbd_model_name(void);
 
 /* Full path for n knot points, array-of-structures (21 in / 105 out per knot),
  * one launch on `stream`.

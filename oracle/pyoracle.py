@@ -63,6 +63,7 @@ def lib():
                                        ctypes.c_int, ctypes.c_double, dp]
         L.oracle_cpu_bench.restype = ctypes.c_long
         L.oracle_set_mutation.argtypes = [ctypes.c_int]
+        L.oracle_set_model.argtypes = [dp, dp]
         _lib = L
     return _lib
 
@@ -233,6 +234,32 @@ def op_count(mode=0, skip_zeros=False):
     c = (ctypes.c_longlong * 4)()
     lib().oracle_op_count(mode, int(bool(skip_zeros)), c)
     return {"add": c[0], "mul": c[1], "div": c[2], "trig": c[3]}
+
+
+def set_model(links=None, joints=None):
+    """Compute on another 7-joint serial chain from now on (None: the iiwa 14).
+    links (7, 10): m, cx, cy, cz, ixx, ixy, ixz, iyy, iyz, izz (inertia about
+    the COM, link frame); joints (7, 12): the joint frame's fixed rotation R
+    (row-major; frame = R Rz(q)) then its offset in the parent frame."""
+    if links is None:
+        lib().oracle_set_model(None, None)
+        return
+    links = np.ascontiguousarray(links, dtype=np.float64).reshape(NQ, 10)
+    joints = np.ascontiguousarray(joints, dtype=np.float64).reshape(NQ, 12)
+    lib().oracle_set_model(_dp(links), _dp(joints))
+
+
+class using_model:
+    """with using_model(links, joints): ... -- restores the iiwa 14 afterwards."""
+
+    def __init__(self, links, joints):
+        self.links, self.joints = links, joints
+
+    def __enter__(self):
+        set_model(self.links, self.joints)
+
+    def __exit__(self, *exc):
+        set_model(None)
 
 
 def set_mutation(m):

@@ -43,6 +43,27 @@ int oracle_abi_version(void) { return 1; }
 // Plant mistake m (see g_mutation) in this thread's oracle; 0 restores it.
 void oracle_set_mutation(int m) { g_mutation = m; }
 
+// Replace the robot model for every later call (not while a call runs).
+//   links:  7 x {m, cx, cy, cz, ixx, ixy, ixz, iyy, iyz, izz}: mass, COM and the
+//           inertia about the COM, in the link frame;
+//   joints: 7 x {R00 R01 R02 R10 R11 R12 R20 R21 R22, rx, ry, rz}: the joint frame
+//           is R Rz(q) at offset r in the parent link's frame (R row-major).
+// links == nullptr restores the iiwa 14.  Entries of R within 1e-12 of an
+// integer are snapped to it, as for the iiwa's rpy angles.
+void oracle_set_model(const double* links, const double* joints) {
+  if (!links || !joints) { model() = iiwa_model(); return; }
+  Model m;
+  for (int k = 0; k < NQ; ++k) {
+    const double* l = links + 10 * k;
+    m.links[k] = LinkInertial{l[0], l[1], l[2], l[3], l[4], l[5], l[6], l[7], l[8], l[9]};
+    const double* j = joints + 12 * k;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) m.frames[k].F[a][b] = snap_unit(j[3 * b + a]);
+    for (int i = 0; i < 3; ++i) m.frames[k].r[i] = j[9 + i];
+  }
+  model() = m;
+}
+
 // 7 spatial inertias, 6x6 column-major each.
 void oracle_inertias(double* I) {
   for (int k = 0; k < NQ; ++k) spatial_inertia<double>(k, I + 36 * k);

@@ -46,6 +46,8 @@ enum Flags : int {
 // joint k is URDF iiwa_joint_{k+1}.  Inertia entries are about the COM, as in
 // the URDF; the parallel-axis shift happens in spatial_inertia().
 // (Same numbers as the reference's tables, dynamicsGradient.h:127-202, :223-229.)
+// These are the default; oracle_set_model() replaces them with another 7-joint
+// serial chain (a robot read from a URDF).
 // ---------------------------------------------------------------------------
 struct LinkInertial { double mass, cx, cy, cz, Ixx, Ixy, Ixz, Iyy, Iyz, Izz; };
 struct JointOrigin  { double x, y, z, roll, pitch, yaw; };
@@ -53,7 +55,7 @@ struct JointOrigin  { double x, y, z, roll, pitch, yaw; };
 constexpr double kPi = 3.141592653589793;
 constexpr double kHalfPi = 1.570796326794897;
 
-static const LinkInertial kLinks[NQ] = {
+static const LinkInertial kIiwaLinks[NQ] = {
   {5.76, 0.0,    -0.03,  0.12,   0.121,  0.0,      0.0,      0.116,    -0.021,   0.0175 },
   {6.35, 0.0003,  0.059, 0.042,  0.0638, 0.0001,   0.00008,  0.0416,    0.0157,  0.0331 },
   {3.5,  0.0,     0.03,  0.13,   0.0873, 0.0,      0.0,      0.083,     0.014,   0.0108 },
@@ -63,7 +65,7 @@ static const LinkInertial kLinks[NQ] = {
   {1.2,  0.0,     0.0,   0.02,   0.0055, 0.0,      0.0,      0.0055,    0.0,     0.005  },
 };
 
-static const JointOrigin kJoints[NQ] = {
+static const JointOrigin kIiwaJoints[NQ] = {
   {0.0, 0.0,    0.1575, 0.0,      0.0, 0.0 },
   {0.0, 0.0,    0.2025, kHalfPi,  0.0, kPi },
   {0.0, 0.2045, 0.0,    kHalfPi,  0.0, kPi },
@@ -73,14 +75,20 @@ static const JointOrigin kJoints[NQ] = {
   {0.0, 0.081,  0.0,   -kHalfPi,  kPi, 0.0 },
 };
 
-// Fixed part of each joint frame: F_k = R_rpy^T with R_rpy = Rz(yaw) Ry(pitch)
-// Rx(roll) (URDF convention).  For the iiwa every angle is a multiple of pi/2,
-// so the entries are rounded to exact integers -- this reproduces the exact
-// 0/+-1 pattern of the reference's hand-written table (dynamicsGradient.h:507-587).
+// Fixed part of each joint frame: F_k = R^T, where the joint frame is R Rz(q)
+// at offset r in the parent frame.  For a URDF joint R = R_rpy = Rz(yaw)
+// Ry(pitch) Rx(roll).  Entries within 1e-12 of an integer are snapped to it:
+// for the iiwa, where every angle is a multiple of pi/2, this reproduces the
+// exact 0/+-1 pattern of the reference's hand-written table
+// (dynamicsGradient.h:507-587).
 struct FixedFrame { double F[3][3]; double r[3]; };
 
-inline FixedFrame make_fixed_frame(int k) {
-  const JointOrigin& j = kJoints[k];
+inline double snap_unit(double x) {
+  const double r = std::round(x);
+  return std::fabs(x - r) < 1e-12 ? r : x;
+}
+
+inline FixedFrame make_fixed_frame(const JointOrigin& j) {
   const double cr = std::cos(j.roll), sr = std::sin(j.roll);
   const double cp = std::cos(j.pitch), sp = std::sin(j.pitch);
   const double cy = std::cos(j.yaw), sy = std::sin(j.yaw);
@@ -91,17 +99,30 @@ inline FixedFrame make_fixed_frame(int k) {
   };
   FixedFrame out;
   for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) out.F[a][b] = std::round(R[b][a]);
+    for (int b = 0; b < 3; ++b) out.F[a][b] = snap_unit(R[b][a]);
   out.r[0] = j.x; out.r[1] = j.y; out.r[2] = j.z;
   return out;
 }
-inline const FixedFrame& fixed_frame(int k) {
-  static const struct Table {
-    FixedFrame f[NQ];
-    Table() { for (int i = 0; i < NQ; ++i) f[i] = make_fixed_frame(i); }
-  } table;
-  return table.f[k];
+
+// The robot every function below computes on.
+struct Model {
+  LinkInertial links[NQ];
+  FixedFrame frames[NQ];
+};
+inline Model iiwa_model() {
+  Model m;
+  for (int k = 0; k < NQ; ++k) {
+    m.links[k] = kIiwaLinks[k];
+    m.frames[k] = make_fixed_frame(kIiwaJoints[k]);
+  }
+  return m;
 }
+// Process-wide; replaced only between computations (oracle_set_model).
+inline Model& model() {
+  static Model m = iiwa_model();
+  return m;
+}
+inline const FixedFrame& fixed_frame(int k) { return model().frames[k]; }
 
 // ---------------------------------------------------------------------------
 // Scalar helpers (work for double, float, std::complex<double> and the
@@ -115,7 +136,7 @@ template <typename T> inline T tcos(const T& x) { using std::cos; return cos(x);
 // (the parallel-axis shift of buildI6x6, dynamicsGradient.h:77-118).
 template <typename T>
 void spatial_inertia(int k, T* I) {
-  const LinkInertial& L = kLinks[k];
+  const LinkInertial& L = model().links[k];
   const double m = L.mass, cx = L.cx, cy = L.cy, cz = L.cz;
   double A[36] = {0};
   A[0]  = L.Ixx + m * (cy * cy + cz * cz);
@@ -158,8 +179,9 @@ void joint_transform(int k, const T& s, const T& c, T* X) {
         else if (i == 2 && l == 2) code = 4;
         if (code == 0) continue;
         T term = (code == 1) ? c : (code == 2) ? s : (code == 3) ? -s : T(1);
-        if (f < 0) term = -term;
-        e = any ? e + term : term;   // at most one non-zero term for these frames
+        if (f == -1.0) term = -term;
+        else if (f != 1.0) term = term * T(f);
+        e = any ? e + term : term;   // one term per entry for the iiwa's 0/+-1 frames
         any = true;
       }
       E[i][j] = e; nz[i][j] = any;

@@ -72,6 +72,8 @@ static void two_wave(long n, const float* in, float* out, int flags) {
 extern "C" {
 // The device model tables: spatial inertia of link k (6x6 column-major) and the
 // joint transform T_k(q) (6x6 column-major, built by applying T to unit vectors).
+const char* host_model_name(void) { return kModelName; }
+
 void host_model_inertia(int k, double* out) {
   float A[6][6];
   auto load = [&](auto Kc) { iload<decltype(Kc)::value>(A); };

@@ -5,6 +5,8 @@ import os
 import re
 import subprocess
 
+import pytest
+
 import common
 
 
@@ -22,23 +24,31 @@ def test_header_declares_what_the_binding_lists():
     assert declared_functions() == sorted(rbdgrad.EXPORTED)
 
 
-def test_library_exports_every_declared_symbol():
+# librbdgrad.so (the iiwa 14) and the library built from tests/golden/synthetic_arm7.urdf
+MODELS = [None, "arm7"]
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_library_exports_every_declared_symbol(model):
     import rbdgrad
-    lib = rbdgrad.lib()                       # raises if librbdgrad.so is missing
+    lib = rbdgrad.lib(model)                  # raises if the library is missing
     for name in declared_functions():
         assert hasattr(lib, name), name
-    out = subprocess.run(["nm", "-D", "--defined-only", rbdgrad.LIB_PATH], capture_output=True, text=True,
+    out = subprocess.run(["nm", "-D", "--defined-only", rbdgrad.lib_path(model)], capture_output=True, text=True,
                          check=True).stdout
     exported = set(re.findall(r"\bT (rbd_\w+)", out))
     assert set(declared_functions()) <= exported
+    assert lib.rbd_model_name().decode() == (model or "iiwa14")
 
 
-def test_library_is_built_for_gfx950():
+@pytest.mark.parametrize("model", MODELS)
+def test_library_is_built_for_gfx950(model):
     import rbdgrad
-    out = subprocess.run(["/opt/rocm/bin/roc-obj-ls", rbdgrad.LIB_PATH], capture_output=True, text=True)
+    path = rbdgrad.lib_path(model)
+    out = subprocess.run(["/opt/rocm/bin/roc-obj-ls", path], capture_output=True, text=True)
     if out.returncode != 0:
         # fall back to scanning the offload bundle's target triple
-        data = open(rbdgrad.LIB_PATH, "rb").read()
+        data = open(path, "rb").read()
         assert b"gfx950" in data
     else:
         assert "gfx950" in out.stdout

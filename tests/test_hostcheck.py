@@ -3,36 +3,17 @@ compiled for the host (tests/hostcheck) against the URDF data and the FP64
 oracle, in every precision mode, before anything runs on the GPU."""
 import ctypes
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
 import common
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-HOSTLIB = os.path.join(HERE, "hostcheck", "libknot_host.so")
 
 
 @pytest.fixture(scope="module")
 def host():
-    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "hostcheck")], check=True)
-    L = ctypes.CDLL(HOSTLIB)
-    fp = ctypes.POINTER(ctypes.c_float)
-    dp = ctypes.POINTER(ctypes.c_double)
-    L.host_fdgrad_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    L.host_fdgrad_refmode_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    L.host_model_inertia.argtypes = [ctypes.c_int, dp]
-    L.host_fd_f32.argtypes = [ctypes.c_long, fp, fp, fp, ctypes.c_int]
-    L.host_fdgrad_minv_f32.argtypes = [ctypes.c_long, fp, fp, fp, ctypes.c_int, ctypes.c_int]
-    L.host_model_transform.argtypes = [ctypes.c_int, ctypes.c_double, dp]
-    L.host_fdgrad_fext_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_long, fp, ctypes.c_int, ctypes.c_int]
-    L.host_fd_fext_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_long, fp, fp, ctypes.c_int]
-    L.host_id_gradient_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    L.host_vaf_f32.argtypes = [ctypes.c_long, fp, ctypes.c_int]
-    L.host_finish_f32.argtypes = [ctypes.c_long, fp, fp, fp]
-    L.host_fdgrad_two_wave_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int]
-    return L
+    return common.hostcheck_lib()
 
 
 def run(host, x, flags=0, prec=0, lanes=0):
