@@ -314,24 +314,25 @@ def profile_one(args, stream):
         log(f"copy: {args.steps} x {nbytes} bytes device to device")
         return
     if args.only == "c2":
-        ctx = rbdgrad.Context(device=0, precision=args.precision)
+        ctx = rbdgrad.Context(device=0, precision=args.precision, model=args.model)
         res = mixed_workload(ctx, stream, steps=args.steps)
         log(json.dumps(res))
         return
     if args.only == "c3":
-        ctx = rbdgrad.Context(device=0, precision=args.precision)
+        ctx = rbdgrad.Context(device=0, precision=args.precision, model=args.model)
         res = served_workload(ctx)
         log(json.dumps(res))
         return
     lanes = 1 if args.only in ("c1-one-lane", "c4-aos", "c4-soa") else (16 if args.only == "c1-lane-group" else 0)
-    ctx = rbdgrad.Context(device=0, precision=args.precision, lanes_per_knot=lanes)
+    ctx = rbdgrad.Context(device=0, precision=args.precision, lanes_per_knot=lanes, model=args.model)
     if args.only.startswith("c1"):
         n, x, layout = 128, knots(128, seed=101), "aos"
     else:
         n, layout = 1 << 20, args.only[3:]
         x = knots(n, seed=104, near_singular_frac=0.1)
     ms, _ = run_flat(ctx, x, args.steps, min(args.warmup, 10), stream, layout)
-    log(f"{args.only}: {ctx.kernel_name(n)} n={n}: {ms / args.steps * 1e3:.3f} us per launch over {args.steps}")
+    log(f"{args.only} ({ctx.model}): {ctx.kernel_name(n)} n={n}: {ms / args.steps * 1e3:.3f} us per launch over "
+        f"{args.steps}")
 
 
 def main():
@@ -345,6 +346,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="skip the large-batch and mixed workloads")
     ap.add_argument("--only", choices=["c1", "c1-one-lane", "c1-lane-group", "c4-aos", "c4-soa", "c2", "c3", "copy"],
                     help="run one workload only and print its timing (for profiler runs)")
+    ap.add_argument("--model", default=None,
+                    help="with --only: the robot library librbdgrad_MODEL.so (default: the iiwa 14)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
