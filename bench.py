@@ -331,8 +331,8 @@ def profile_one(args, stream):
         n, layout = 1 << 20, args.only[3:]
         x = knots(n, seed=104, near_singular_frac=0.1)
     ms, _ = run_flat(ctx, x, args.steps, min(args.warmup, 10), stream, layout)
-    log(f"{args.only} ({ctx.model}): {ctx.kernel_name(n)} n={n}: {ms / args.steps * 1e3:.3f} us per launch over "
-        f"{args.steps}")
+    log(f"{args.only} ({ctx.model}, {layout}): {ctx.kernel_name(n)} mapping, n={n}: "
+        f"{ms / args.steps * 1e3:.3f} us per launch over {args.steps}")
 
 
 def main():
