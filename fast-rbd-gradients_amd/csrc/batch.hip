@@ -23,10 +23,8 @@ struct rbd_batch {
   rbd::BatchHeader* d_hdr = nullptr;
   rbd::TrajDesc* d_desc = nullptr;
   long* d_prefix = nullptr;
-  // queue cursor of staged launches (device memory, never reset: each launch
-  // advances it by its chunk total, which counter_base tracks)
-  unsigned long long* d_counter = nullptr;
-  unsigned long long counter_base = 0;
+  // counters of the last partial round (device memory, zero between launches)
+  unsigned long long* d_tail = nullptr;
   size_t upload_bytes = 0;            // header + descriptors
   // pinned host staging ring for the descriptor upload
   static constexpr int kRing = 4;
@@ -47,25 +45,32 @@ namespace {
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 constexpr size_t kHeaderBytes = 256;   // the header's slot before the descriptors
+constexpr int64_t kMaxBatchChunks = (int64_t(1) << 31) - 1;   // the kernels index chunks with int
 
 // Where a launch finds its queue: header and descriptors (the batch's own, or
-// ones a caller staged in its device memory) and the prefix-sum scratch.
+// ones a caller staged) and the prefix-sum scratch.
 struct Queue {
   const rbd::BatchHeader* hdr;
   const rbd::TrajDesc* descs;
-  unsigned long long* cursor;
-  unsigned long long base;
   long* prefix;
+  const rbd::ArgBatch* args = nullptr;   // a small batch in the kernel arguments instead
 };
 
-// the batch's own header and descriptors, uploaded with a zeroed cursor
-Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, &b->d_hdr->cursor, 0ull, b->d_prefix}; }
+// the batch's own header and descriptors, uploaded per launch
+Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, b->d_prefix}; }
 
 template <class P, int LANES, bool SMALL>
 void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
   const unsigned grid = b->grid[LANES == 1 ? 0 : 1];
-  hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, SMALL>), dim3(grid), dim3(kBlock), 0, s, q.descs,
-                     q.hdr, q.cursor, q.base, q.prefix, b->ctx->opt);
+  if constexpr (SMALL) {
+    static const rbd::ArgBatch kNone{};
+    const rbd::ArgBatch& a = q.args ? *q.args : kNone;
+    hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, true>), dim3(grid), dim3(kBlock), 0, s,
+                       q.args ? nullptr : q.descs, q.hdr, a, b->d_tail, q.prefix, b->ctx->opt);
+  } else {
+    hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, false>), dim3(grid), dim3(kBlock), 0, s, q.descs,
+                       q.hdr, rbd::NoArgBatch{}, b->d_tail, q.prefix, b->ctx->opt);
+  }
 }
 
 template <class P>
@@ -94,7 +99,7 @@ hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hip
 // the descriptors); returns the chunk total
 long stage(char* dst, const rbd_traj* trajs, int n_traj, int lanes) {
   rbd::BatchHeader* h = reinterpret_cast<rbd::BatchHeader*>(dst);
-  std::memset(h, 0, sizeof(*h));             // cursor = 0
+  std::memset(h, 0, sizeof(*h));
   const long chunk = 64 / lanes;
   long total = 0;
   rbd::TrajDesc* d = reinterpret_cast<rbd::TrajDesc*>(dst + kHeaderBytes);
@@ -159,13 +164,14 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
   b->d_hdr = reinterpret_cast<rbd::BatchHeader*>(b->d_mem);
   b->d_desc = reinterpret_cast<rbd::TrajDesc*>(b->d_mem + hdr);
   b->d_prefix = reinterpret_cast<long*>(b->d_mem + hdr + desc);
-  e = hipMalloc(&b->d_counter, sizeof(unsigned long long));
-  if (e == hipSuccess) e = hipMemset(b->d_counter, 0, sizeof(unsigned long long));   // once, outside any graph
+  const size_t tail_bytes = sizeof(unsigned long long) * rbd::kTailCounters * rbd::kTailStride;
+  e = hipMalloc(&b->d_tail, tail_bytes);
+  if (e == hipSuccess) e = hipMemset(b->d_tail, 0, tail_bytes);   // once, outside any graph; kernels keep it zero
   if (e != hipSuccess) {
-    if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->d_tail) (void)hipFree(b->d_tail);
     (void)hipFree(b->d_mem);
     delete b;
-    return fail(RBD_ERR_OUT_OF_MEMORY, "batch queue counter");
+    return fail(RBD_ERR_OUT_OF_MEMORY, "batch queue counters");
   }
   for (int i = 0; i < rbd_batch::kRing && e == hipSuccess; ++i) {
     e = hipHostMalloc(reinterpret_cast<void**>(&b->h_stage[i]), b->upload_bytes, hipHostMallocDefault);
@@ -195,7 +201,7 @@ rbd_status rbd_batch_destroy(rbd_batch* b) {
     if (b->h_stage[i]) (void)hipHostFree(b->h_stage[i]);
   }
   if (b->d_mem) (void)hipFree(b->d_mem);
-  if (b->d_counter) (void)hipFree(b->d_counter);
+  if (b->d_tail) (void)hipFree(b->d_tail);
   delete b;
   return RBD_OK;
 }
@@ -231,19 +237,39 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   if (n_traj < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative trajectory count");
   if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
   if (n_traj > 0 && !trajs) return fail(RBD_ERR_INVALID_ARGUMENT, "null trajectory list");
-  int64_t total = 0;
+  int64_t total = 0, max_chunks = 0;
   for (int t = 0; t < n_traj; ++t) {
     if (trajs[t].n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative trajectory length");
     if (trajs[t].n_knots > 0 && (!trajs[t].d_in || !trajs[t].d_out))
       return fail(RBD_ERR_INVALID_ARGUMENT, "null trajectory buffer");
     total += trajs[t].n_knots;
+    max_chunks += (trajs[t].n_knots + 3) / 4;   // chunks of the 4-knot mapping (the larger count)
   }
+  if (max_chunks > kMaxBatchChunks) return fail(RBD_ERR_CAPACITY, "batch too large (more than 2^31 chunks)");
   b->last_total_knots = total;
   if (total == 0) return RBD_OK;
   if (rbd_status s = use_device(b->ctx)) return s;
   const hipStream_t st = (hipStream_t)stream;
   const int lanes = lanes_req ? lanes_req : lanes_for_batch(b->ctx, total);
   b->last_lanes = lanes;
+  const bool small = n_traj <= rbd::kSmallBatch;
+  if (small && !use_graph) {
+    // the descriptors go in the kernel arguments: no upload
+    rbd::ArgBatch a{};
+    const long chunk = 64 / lanes;
+    long chunks = 0;
+    for (int t = 0; t < n_traj; ++t) {
+      a.d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
+      chunks += (long(trajs[t].n_knots) + chunk - 1) / chunk;
+    }
+    a.n_traj = n_traj;
+    a.total_chunks = int(chunks);
+    Queue q{nullptr, nullptr, b->d_prefix};
+    q.args = &a;
+    hipError_t le = launch_batch(b, q, lanes, true, st);
+    if (le != hipSuccess) return hip_fail(le, "batch kernel launch");
+    return RBD_OK;
+  }
   // stage header + descriptors in pinned memory, copy to the device on `stream`
   const int r = b->ring;
   b->ring = (r + 1) % rbd_batch::kRing;
@@ -253,7 +279,6 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   e = hipMemcpyAsync(b->d_mem, b->h_stage[r], batch_stage_bytes(n_traj), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipEventRecord(b->staged[r], st);
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
-  const bool small = n_traj <= rbd::kSmallBatch;
   if (use_graph) {
     if (rbd_status s = ensure_graph(b, lanes, small)) return s;
     e = hipGraphLaunch(b->exec[graph_index(lanes, small)], st);
@@ -274,16 +299,15 @@ int64_t rbdi::batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int
 rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, int n_traj, int64_t total_knots,
                                      int64_t total_chunks, int lanes, rbd_stream_t stream) {
   if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
+  if (total_chunks > kMaxBatchChunks) return fail(RBD_ERR_CAPACITY, "batch too large (more than 2^31 chunks)");
   b->last_total_knots = total_knots;
   b->last_lanes = lanes;
   if (total_knots == 0) return RBD_OK;
   const char* base = static_cast<const char*>(d_staged);
   const Queue q{reinterpret_cast<const rbd::BatchHeader*>(base),
-                reinterpret_cast<const rbd::TrajDesc*>(base + kHeaderBytes), b->d_counter, b->counter_base,
-                b->d_prefix};
+                reinterpret_cast<const rbd::TrajDesc*>(base + kHeaderBytes), b->d_prefix};
   hipError_t e = launch_batch(b, q, lanes, n_traj <= rbd::kSmallBatch, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
-  b->counter_base += (unsigned long long)total_chunks;
   return RBD_OK;
 }
 
@@ -293,13 +317,14 @@ int rbd_batch_last_lanes(const rbd_batch* b) { return b ? b->last_lanes : 0; }
 
 // Times both mappings of the batched path on batches of 132-knot trajectories
 // (the mean length of lengths uniform over 8..256) at 8 ... 512 trajectories
-// and sets the largest total at which the lane-group mapping is faster.
+// and sets the total at which the lane-group mapping stops being faster,
+// interpolated between the two sizes that bracket it.
 rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                               int table_len) {
   if (rbd_status s = check_ctx(ctx)) return s;
   if (rbd_status s = use_device(ctx)) return s;
   constexpr int64_t kLen = 132;
-  static const int counts[] = {8, 16, 32, 64, 128, 256, 512};
+  static const int counts[] = {8, 16, 32, 48, 64, 96, 128, 256, 512};
   constexpr int kSizes = sizeof(counts) / sizeof(counts[0]);
   const int max_traj = counts[kSizes - 1];
   const size_t knots = size_t(max_traj) * kLen;
@@ -319,7 +344,6 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
   if (e == hipSuccess && rs == RBD_OK) e = hipEventCreate(&t0);
   if (e == hipSuccess && rs == RBD_OK) e = hipEventCreate(&t1);
   float best_us[2][kSizes] = {};
-  int64_t threshold = 0;
   for (int si = 0; si < kSizes && e == hipSuccess && rs == RBD_OK; ++si) {
     for (int li = 0; li < 2 && e == hipSuccess && rs == RBD_OK; ++li) {
       const int lanes = li ? rbd::kLanesPerKnotGroup : 1;
@@ -337,7 +361,18 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
       }
       best_us[li][si] = best;
     }
-    if (e == hipSuccess && rs == RBD_OK && best_us[1][si] < best_us[0][si]) threshold = counts[si] * kLen;
+  }
+  // crossover: interpolated where the lane-group mapping stops winning
+  int64_t threshold = 0;
+  for (int si = 0; si < kSizes; ++si) {
+    const double d = double(best_us[0][si]) - double(best_us[1][si]);   // > 0: lane groups faster
+    if (d > 0) { threshold = counts[si] * kLen; continue; }
+    if (si > 0 && threshold == counts[si - 1] * kLen) {
+      const double d0 = double(best_us[0][si - 1]) - double(best_us[1][si - 1]);
+      const double n0 = double(counts[si - 1] * kLen), n1 = double(counts[si] * kLen);
+      threshold = int64_t(n0 + (n1 - n0) * d0 / (d0 - d));
+    }
+    break;
   }
   (void)hipStreamSynchronize(st);
   if (t0) (void)hipEventDestroy(t0);

@@ -2,26 +2,33 @@
 // persistent kernel that drains it.
 //
 // A batch is a list of trajectories, each with its own input/output buffers
-// and length.  A chunk is the knots one wave processes per iteration: 64 in the
-// one-lane mapping, 4 in the 16-lane-group mapping.  Wave w of the persistent
-// grid takes chunk w first; after that each wave takes chunk
-// gridDim.x + atomicAdd(cursor, 1) - cursor_base, so a batch no larger than the grid
-// costs one atomic per busy wave and idle waves none (all waves contending for
-// one address was the kernel's main overhead).  Every processed chunk is
-// followed by exactly one atomic add, so a launch advances the cursor by
-// exactly its chunk total: a caller that keeps the cursor in device memory
-// passes the running sum as cursor_base and never resets it, one that uploads
-// a zeroed cursor with the header passes 0.  A chunk's trajectory comes from
-// the prefix sum of chunk counts:
-//   * up to 64 trajectories (SMALL): every wave scans the counts itself, one
-//     trajectory per lane, and finds a chunk's trajectory with a ballot and a
-//     population count -- no plan kernel, no memory traffic per chunk;
-//   * more: the plan kernel writes the prefix sum and the total first, and the
-//     waves binary-search it.
-// Every wave leaves the loop once its chunk index passes the total, which is
-// fixed before the persistent kernel starts, and the cursor only grows, so the
-// grid always drains.  The header and descriptors are only read, so they may
-// live in device memory or in pinned host memory.
+// and length.  A chunk is the knots one wave processes at a time: 64 in the
+// one-lane mapping, 4 in the 16-lane-group mapping.  The queue holds the
+// chunks in trajectory order; a chunk's trajectory comes from the prefix sum
+// of chunk counts:
+//   * up to 64 trajectories (SMALL): a wave scans the counts itself, one
+//     trajectory per lane, and finds the chunk's trajectory with a ballot and
+//     a population count -- no plan kernel, no memory traffic beyond the counts;
+//   * more: the plan kernel writes the prefix sum first, and the waves
+//     binary-search it.
+// With G waves in the grid and T chunks, the waves pull the chunks of the full
+// rounds, floor(T / G) * G of them, in a fixed order (wave w: w, w + G, ...),
+// which needs no atomic: every chunk costs the same, so a fixed order is
+// already balanced.  The last partial round, T mod G chunks, is pulled through
+// atomic counters so that waves that start or finish early take more of it.
+// Its chunks are dealt round-robin over S = min(64, G) counters, each on its
+// own 128-byte line, and wave w claims from counter w mod S: one address is
+// never contended by more than G / S waves (all G waves on one counter was the
+// kernel's main overhead).  Each wave claims until a claim fails, so a counter
+// takes exactly (its tail chunks + its waves) increments per launch, and the
+// wave whose claim is the last of them sets it back to zero: the counters
+// start every launch at zero with no host work, no upload and no memset, also
+// in graph replays.  A batch object's launches must therefore run one at a
+// time (one stream, or stream-ordered).  Every wave reaches the tail, every
+// claim either returns a chunk or ends the wave's loop, and the totals are
+// fixed before the kernel starts, so the grid always drains.  The header and
+// descriptors are only read, so they may live in device memory or in pinned
+// host memory; the counters must be device memory.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -37,13 +44,25 @@ struct TrajDesc {
 };
 
 struct BatchHeader {
-  int n_traj;                     // written by the host copy with the descriptors
+  int n_traj;                     // trajectories
   int chunk_knots;                // knots per chunk
-  long total_chunks;              // chunk total, computed by the host
-  unsigned long long cursor;      // dynamic chunks handed out, when the caller uploads it zeroed
+  long total_chunks;              // chunk total, computed by the host (below 2^31)
 };
 
+// counters of a launch's last partial round
+constexpr int kTailCounters = 64;
+constexpr int kTailStride = 16;   // 64-bit words between counters: one 128-byte line each
+
 constexpr int kSmallBatch = 64;   // trajectories handled without the plan kernel
+
+// A small batch passed by value in the kernel arguments, so that a direct
+// launch needs no descriptor upload; used when descs is null.
+struct ArgBatch {
+  int n_traj;
+  int total_chunks;
+  TrajDesc d[kSmallBatch];
+};
+struct NoArgBatch {};
 
 constexpr int kPlanThreads = 1024;
 
@@ -79,35 +98,46 @@ __global__ __launch_bounds__(kPlanThreads) void batch_plan_kernel(const TrajDesc
 
 // Persistent drain of the queue.  LANES = 1: lane i of a wave takes knot
 // k0 + i of a 64-knot chunk; LANES = 16: lane group g takes knot k0 + g of a
-// 4-knot chunk.  One wave per workgroup.
+// 4-knot chunk.  One wave per workgroup.  tail: kTailCounters counters,
+// kTailStride words apart, zero when the launch starts (and left at zero).
+// SMALL batches may come in the arguments (descs == nullptr: arg holds them).
 template <class P, int LANES, bool SMALL>
 __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
-    const TrajDesc* __restrict__ descs, const BatchHeader* __restrict__ hdr, unsigned long long* __restrict__ cursor,
-    unsigned long long cursor_base, const long* __restrict__ prefix, Options opt) {
-  const int n_traj = hdr->n_traj;
-  const int chunk = 64 / LANES;
-  const int lane = threadIdx.x;
+    const TrajDesc* __restrict__ descs, const BatchHeader* __restrict__ hdr,
+    const typename std::conditional<SMALL, ArgBatch, NoArgBatch>::type arg, unsigned long long* __restrict__ tail,
+    const long* __restrict__ prefix, Options opt) {
+  constexpr int chunk = 64 / LANES;
+  int n_traj, total;
+  if constexpr (SMALL) {
+    if (descs == nullptr) {
+      descs = arg.d;
+      n_traj = arg.n_traj;
+      total = arg.total_chunks;
+    } else {
+      n_traj = hdr->n_traj;
+      total = int(hdr->total_chunks);
+    }
+  } else {
+    n_traj = hdr->n_traj;
+    total = int(hdr->total_chunks);
+  }
+  const int G = int(gridDim.x);
+  const int w = int(blockIdx.x);
+  const int lane = int(threadIdx.x);
   const int slot = lane / LANES;            // knot slot in the chunk
   const int role = lane % LANES;
-  long total, incl = 0;                     // SMALL: inclusive prefix of chunk counts up to trajectory `lane`
-  if constexpr (SMALL) {
-    long v = lane < n_traj ? (descs[lane].n + chunk - 1) / chunk : 0;
-    for (int off = 1; off < 64; off <<= 1) {
-      const long o = __shfl_up(v, off, 64);
-      if (lane >= off) v += o;
-    }
-    incl = v;
-    total = __shfl(v, 63, 64);
-  } else {
-    total = hdr->total_chunks;
-  }
-  for (long c = blockIdx.x; c < total;) {
-    int t;
-    long first;                             // first chunk of trajectory t
+
+  auto process = [&](int c) {
+    int t, first;                           // trajectory of chunk c, and its first chunk
     if constexpr (SMALL) {
-      // trajectories that end at or before chunk c are the lanes with incl <= c
-      t = __popcll(__ballot(incl <= c));
-      first = __shfl(incl, t > 0 ? t - 1 : 0, 64);
+      int v = lane < n_traj ? int((descs[lane].n + chunk - 1) / chunk) : 0;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+      }
+      // trajectories that end at or before chunk c are the lanes with v <= c
+      t = __popcll(__ballot(v <= c));
+      first = __shfl(v, t > 0 ? t - 1 : 0, 64);
       if (t == 0) first = 0;
     } else {
       int lo = 0, hi = n_traj;               // prefix[lo] <= c < prefix[hi]
@@ -116,22 +146,46 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
         if (prefix[mid] <= c) lo = mid; else hi = mid;
       }
       t = lo;
-      first = prefix[lo];
+      first = int(prefix[lo]);
     }
     const TrajDesc d = descs[t];
-    const long k = (c - first) * chunk + slot;
+    const long k = long(c - first) * chunk + slot;
     if (k < d.n) {
       InAoS i{d.in + kInPerKnot * k};
       OutAoS o{d.out + kOutPerKnot * k};
       if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
       else fdgrad_knot_lanes<P>(i, o, opt, role);
     }
-    unsigned long long next = 0;
-    if (lane == 0) next = atomicAdd(cursor, 1ull) - cursor_base;
-    next = __shfl(next, 0, 64);
-    // a cursor out of step with cursor_base (a caller error) ends the loop
-    // instead of producing an index outside [0, total)
-    c = long(gridDim.x) + long(next < (unsigned long long)total ? next : (unsigned long long)total);
+  };
+
+  // full rounds (all chunks when they fit one round): fixed order, no atomics;
+  // the last partial round: through the counters
+  const int fixed = total <= G ? total : total - total % G;
+  const int rest = total - fixed;
+  const int S = G < kTailCounters ? G : kTailCounters;
+  const int s = w % S;
+  const int mine = s < rest ? (rest - 1 - s) / S + 1 : 0;     // tail chunks dealt to counter s
+  const int waves = (G - 1 - s) / S + 1;                      // waves that claim from counter s
+  unsigned long long* ctr = tail + s * kTailStride;
+  int next = w;
+  for (;;) {
+    int c;
+    if (next < fixed) {
+      c = next;
+      next += G;
+    } else {
+      if (rest == 0) break;
+      int j = 0;
+      if (lane == 0) {
+        j = int(atomicAdd(ctr, 1ull));
+        // the launch's last claim on this counter: back to zero for the next launch
+        if (j == mine + waves - 1) atomicExch(ctr, 0ull);
+      }
+      j = __builtin_amdgcn_readfirstlane(j);
+      if (j >= mine) break;
+      c = fixed + s + j * S;
+    }
+    process(c);
   }
 }
 

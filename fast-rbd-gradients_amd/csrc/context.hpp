@@ -28,7 +28,7 @@ constexpr int64_t kDefaultGroupThreshold = 16384;
 constexpr int kTwoWaveLanes = 32;   // lanes_per_knot value of the two-wave mapping (16 lanes in each of 2 waves)
 // The same crossover for batches (rbd_batch_*, total knots of the batch),
 // measured by rbd_autotune_batch on batches of 132-knot trajectories.
-constexpr int64_t kDefaultBatchGroupThreshold = 4096;
+constexpr int64_t kDefaultBatchGroupThreshold = 16384;
 // Batches of up to this many trajectories find a chunk's trajectory inside the
 // wave, larger ones through the plan kernel (rbd::kSmallBatch, batch_kernels.hpp).
 constexpr int kSmallBatchTrajectories = 64;
@@ -100,8 +100,9 @@ rbd_status batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_gr
 // trajectory pointers as given) to host memory and returns the chunk total;
 // batch_launch_staged runs the batch from d_staged, a device-visible address
 // of that block (8-byte aligned, read only: device memory the caller copied it
-// to on `stream`, or the pinned host memory itself), with the queue cursor in
-// the batch object's device memory.
+// to on `stream`, or the pinned host memory itself), with the queue's tail
+// counters in the batch object's device memory.  Launches on one batch object
+// must run one at a time (the server gives each in-flight slot its own).
 size_t batch_stage_bytes(int n_traj);
 int64_t batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int lanes);
 rbd_status batch_launch_staged(rbd_batch* b, const void* d_staged, int n_traj, int64_t total_knots,

@@ -386,9 +386,9 @@ class Context(_Bound):
         """Time both mappings of the batched path; sets and returns its crossover
         (total knots) plus the table {total knots: (us one-lane, us lane-group)}."""
         thr = ctypes.c_int64()
-        tab = (ctypes.c_float * 21)()
-        self._check(self._L.rbd_autotune_batch(self._h, _sh(stream), ctypes.byref(thr), tab, 21), "rbd_autotune_batch")
-        table = {int(tab[i]): (tab[i + 1], tab[i + 2]) for i in range(0, 21, 3)}
+        tab = (ctypes.c_float * 27)()
+        self._check(self._L.rbd_autotune_batch(self._h, _sh(stream), ctypes.byref(thr), tab, 27), "rbd_autotune_batch")
+        table = {int(tab[i]): (tab[i + 1], tab[i + 2]) for i in range(0, 27, 3)}
         return thr.value, table
 
     @property
@@ -541,6 +541,18 @@ def _fext_buffer(fext, n):
     return DeviceBuffer.from_array(f), FEXT_PER_SET
 
 
+class TrajList:
+    """An rbd_traj array built once from [(d_in, d_out, n_knots), ...]; keeps the buffers alive."""
+
+    def __init__(self, trajs):
+        self.n = len(trajs)
+        self.arr = (Traj * max(1, self.n))()
+        for i, (di, do, n) in enumerate(trajs):
+            self.arr[i] = Traj(di.handle.value if di is not None else None,
+                               do.handle.value if do is not None else None, int(n))
+        self._keep = list(trajs)
+
+
 class Batch(_Bound):
     """rbd_batch: runs many trajectories of uneven length with one persistent-kernel launch."""
 
@@ -552,13 +564,12 @@ class Batch(_Bound):
                "rbd_batch_create")
 
     def run(self, trajs, use_graph=False, stream=None):
-        """trajs: list of (d_in DeviceBuffer, d_out DeviceBuffer, n_knots)."""
-        arr = (Traj * max(1, len(trajs)))()
-        for i, (di, do, n) in enumerate(trajs):
-            arr[i] = Traj(di.handle.value if di is not None else None, do.handle.value if do is not None else None,
-                          int(n))
-        self._check(self._L.rbd_fdgrad_batched_f32(self._h, arr, len(trajs), int(bool(use_graph)), _sh(stream)),
-               "rbd_fdgrad_batched_f32")
+        """trajs: list of (d_in DeviceBuffer, d_out DeviceBuffer, n_knots), or a
+        TrajList built from one (no per-call Python work on the list)."""
+        if not isinstance(trajs, TrajList):
+            trajs = TrajList(trajs)
+        self._check(self._L.rbd_fdgrad_batched_f32(self._h, trajs.arr, trajs.n, int(bool(use_graph)), _sh(stream)),
+                    "rbd_fdgrad_batched_f32")
 
     def set_grid(self, lanes_per_knot, waves):
         self._check(self._L.rbd_batch_set_grid(self._h, int(lanes_per_knot), int(waves)), "rbd_batch_set_grid")

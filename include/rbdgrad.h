@@ -229,11 +229,11 @@ int64_t rbd_two_wave_threshold(rbd_context* ctx);
 rbd_status rbd_set_two_wave_threshold(rbd_context* ctx, int64_t n_knots);
 
 /* The batched path (rbd_fdgrad_batched_f32) has its own crossover on a
- * batch's total knots: its persistent kernel pays for the queue, so the
- * one-lane mapping wins earlier.  rbd_autotune_batch times both mappings on
- * batches of 8 ... 512 trajectories of 132 knots and sets it; the table holds
- * (total knots, us one-lane, us lane-group) triples.  Same conditions as
- * rbd_autotune.  No reference counterpart. */
+ * batch's total knots.  rbd_autotune_batch times both mappings on batches of
+ * 8, 16, 32, 48, 64, 96, 128, 256 and 512 trajectories of 132 knots and sets
+ * it where the lane-group mapping stops being faster (interpolated); the table
+ * holds up to nine (total knots, us one-lane, us lane-group) triples.  Same
+ * conditions as rbd_autotune.  No reference counterpart. */
 rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                               int table_len);
 int64_t rbd_batch_group_threshold(rbd_context* ctx);
@@ -262,7 +262,8 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
 rbd_status rbd_batch_destroy(rbd_batch* b);
 
 /* Run the forward-dynamics gradient over n_traj trajectories on `stream`.
- * The descriptors are copied to the device asynchronously on `stream`; the
+ * Up to 64 trajectories travel in the kernel's arguments (no upload, unless
+ * use_graph); more are copied to the device asynchronously on `stream`.  The
  * caller's trajs array may be reused on return.  The knot-to-lane mapping is
  * chosen from the total knot count against the context's batch crossover
  * (rbd_autotune_batch), unless the context fixes lanes_per_knot.

@@ -136,6 +136,56 @@ def test_alternating_small_and_large_trajectory_counts(rbd, ctx, use_graph):
     b.close()
 
 
+@pytest.mark.parametrize("lanes,waves", [(16, 100), (1, 70), (16, 2048), (1, 5)])
+def test_last_round_counters_return_to_zero(rbd, lanes, waves):
+    """Chunk totals above the grid and not a multiple of it, so every launch
+    ends with a partial round pulled through the tail counters (64 of them,
+    or one per wave for grids under 64); back-to-back launches on one batch
+    object with no synchronisation between them each cover every knot, which
+    holds only if each launch leaves its counters at zero."""
+    c = rbd.Context(precision="fp32")
+    c.batch_group_threshold = 1 << 40 if lanes == 16 else 0
+    b = rbd.Batch(c, 128)
+    b.set_grid(lanes, waves)
+    rng = np.random.default_rng(40 + waves)
+    chunk = 64 // lanes
+    runs = []
+    for rep in range(5):
+        n_traj = int(rng.integers(1, 128))
+        lengths = rng.integers(1, 3 * chunk * waves // n_traj + 2, n_traj)
+        xs, trajs = make_batch(rbd, lengths, seed=400 + rep)
+        total = int(sum((int(n) + chunk - 1) // chunk for n in lengths))
+        b.run(trajs)
+        assert b.last_lanes == lanes
+        runs.append((xs, trajs, lengths, total))
+    rbd.hip().hipDeviceSynchronize()
+    flat = rbd.Context(precision="fp32", lanes_per_knot=lanes)
+    for xs, trajs, lengths, total in runs:
+        for x, g in zip(xs, download(trajs)):
+            np.testing.assert_array_equal(g, flat.run_aos(x))
+        say(f"lanes {lanes} grid {waves}: {len(lengths)} trajectories, {total} chunks "
+            f"({total // waves} full rounds + {total % waves}) ok")
+    flat.close()
+    b.close()
+    c.close()
+
+
+def test_batch_autotune(rbd):
+    c = rbd.Context(precision="fp32")
+    thr, table = c.autotune_batch()
+    sizes = sorted(table)
+    assert len(sizes) == 9 and sizes[0] == 8 * 132 and sizes[-1] == 512 * 132
+    assert all(a > 0 and g > 0 for a, g in table.values())
+    assert c.batch_group_threshold == thr
+    faster = [n for n in sizes if table[n][1] < table[n][0]]
+    if faster and faster[0] == sizes[0]:
+        assert thr >= sizes[0]
+    else:
+        assert thr == 0
+    say(f"batch crossover {thr} knots; table {table}")
+    c.close()
+
+
 def test_batch_errors(rbd, ctx):
     b = rbd.Batch(ctx, 2)
     xs, trajs = make_batch(rbd, [5, 5, 5], seed=5)
