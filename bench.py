@@ -227,12 +227,14 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000), duration_s=1.
     """Config 3: Poisson request arrivals at each rate; each request one
     trajectory with length uniform over {16, 32, 64, 128, 256}; device-resident
     constants, admission by a fixed memory budget, continuous batching over
-    4 in-flight slots.  Latency = submit -> result in the caller's host buffer."""
+    4 in-flight slots, a polling dispatcher.  Latency = submit -> result in the
+    caller's host buffer."""
     pool = knots(1 << 16, seed=103)
     res = {}
     for zero_copy in (0, 1):
         mode = "zero_copy" if zero_copy else "device_buffers"
-        srv = rbdgrad.Server(ctx, slots=4, lanes_per_knot=16, memory_budget_bytes=256 << 20, zero_copy=zero_copy)
+        srv = rbdgrad.Server(ctx, slots=4, lanes_per_knot=16, memory_budget_bytes=256 << 20, zero_copy=zero_copy,
+                             poll_dispatch=1)
         for rate in rates:
             log(f"[c3] served ({mode}): Poisson {rate} req/s for {duration_s} s")
             st = srv.run_poisson(rate, duration_s, [16, 32, 64, 128, 256], pool, seed=rate)

@@ -296,17 +296,26 @@ int64_t rbdi::batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int
   return stage(static_cast<char*>(host_dst), trajs, n_traj, lanes);
 }
 
-rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, int n_traj, int64_t total_knots,
-                                     int64_t total_chunks, int lanes, rbd_stream_t stream) {
+rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, const rbd_traj* trajs, int n_traj,
+                                     int64_t total_knots, int64_t total_chunks, int lanes, rbd_stream_t stream) {
   if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
   if (total_chunks > kMaxBatchChunks) return fail(RBD_ERR_CAPACITY, "batch too large (more than 2^31 chunks)");
   b->last_total_knots = total_knots;
   b->last_lanes = lanes;
   if (total_knots == 0) return RBD_OK;
   const char* base = static_cast<const char*>(d_staged);
-  const Queue q{reinterpret_cast<const rbd::BatchHeader*>(base),
-                reinterpret_cast<const rbd::TrajDesc*>(base + kHeaderBytes), b->d_prefix};
-  hipError_t e = launch_batch(b, q, lanes, n_traj <= rbd::kSmallBatch, (hipStream_t)stream);
+  Queue q{reinterpret_cast<const rbd::BatchHeader*>(base),
+          reinterpret_cast<const rbd::TrajDesc*>(base + kHeaderBytes), b->d_prefix};
+  const bool small = n_traj <= rbd::kSmallBatch;
+  rbd::ArgBatch a{};
+  if (small && trajs) {              // the descriptors in the kernel arguments: no reads of the staged block
+    for (int t = 0; t < n_traj; ++t)
+      a.d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
+    a.n_traj = n_traj;
+    a.total_chunks = int(total_chunks);
+    q.args = &a;
+  }
+  hipError_t e = launch_batch(b, q, lanes, small, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   return RBD_OK;
 }

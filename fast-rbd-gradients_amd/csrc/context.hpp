@@ -101,12 +101,14 @@ rbd_status batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_gr
 // batch_launch_staged runs the batch from d_staged, a device-visible address
 // of that block (8-byte aligned, read only: device memory the caller copied it
 // to on `stream`, or the pinned host memory itself), with the queue's tail
-// counters in the batch object's device memory.  Launches on one batch object
+// counters in the batch object's device memory; with trajs (the same list,
+// device-visible pointers) a batch of up to 64 trajectories takes its
+// descriptors from the kernel arguments instead.  Launches on one batch object
 // must run one at a time (the server gives each in-flight slot its own).
 size_t batch_stage_bytes(int n_traj);
 int64_t batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int lanes);
-rbd_status batch_launch_staged(rbd_batch* b, const void* d_staged, int n_traj, int64_t total_knots,
-                               int64_t total_chunks, int lanes, rbd_stream_t stream);
+rbd_status batch_launch_staged(rbd_batch* b, const void* d_staged, const rbd_traj* trajs, int n_traj,
+                               int64_t total_knots, int64_t total_chunks, int lanes, rbd_stream_t stream);
 
 inline rbd_status check_ctx(const rbd_context* ctx) {
   if (!ctx) return fail(RBD_ERR_INVALID_ARGUMENT, "null context");

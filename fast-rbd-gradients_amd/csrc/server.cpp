@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -144,6 +145,7 @@ struct rbd_server {
 
   std::mutex mu;
   std::condition_variable cv_dispatch;  // new request, returned memory, freed slot, stop
+  std::atomic<uint64_t> wake{0};        // bumped with every cv_dispatch notification (polling dispatcher)
   std::condition_variable cv_complete;  // a slot was launched, stop (one completion thread per slot)
   std::condition_variable cv_done;      // a request finished
   std::deque<std::shared_ptr<Request>> queue;                    // submitted, not admitted
@@ -175,14 +177,26 @@ double since(const rbd_server* s, Clock::time_point t) {
 void dispatch_loop(rbd_server* s) {
   if (use_device(s->ctx) != RBD_OK) return;
   std::unique_lock<std::mutex> lk(s->mu);
+  auto ready = [s] {
+    if (s->stop) return true;
+    if (s->queue.empty()) return false;
+    for (const Slot& sl : s->slots)
+      if (!sl.busy) return true;
+    return false;
+  };
   for (;;) {
-    s->cv_dispatch.wait(lk, [s] {
-      if (s->stop) return true;
-      if (s->queue.empty()) return false;
-      for (const Slot& sl : s->slots)
-        if (!sl.busy) return true;
-      return false;
-    });
+    if (s->cfg.poll_dispatch) {
+      // poll instead of sleeping: a condition-variable wake-up costs about
+      // 10 us, more than the batch's kernel at low load
+      while (!ready()) {
+        const uint64_t w = s->wake.load(std::memory_order_acquire);
+        lk.unlock();
+        while (s->wake.load(std::memory_order_acquire) == w) std::this_thread::yield();
+        lk.lock();
+      }
+    } else {
+      s->cv_dispatch.wait(lk, ready);
+    }
     if (s->stop) return;
     // optional batching window: let requests right behind the head join
     if (s->cfg.batch_window_us > 0) {
@@ -268,7 +282,8 @@ void dispatch_loop(rbd_server* s) {
     if (copy)
       e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, stage_bytes + in_bytes, hipMemcpyHostToDevice, sl.stream);
     rbd_status st = RBD_OK;
-    if (e == hipSuccess) st = batch_launch_staged(sl.batch, work + off, n_traj, knots, chunks, lanes, sl.stream);
+    if (e == hipSuccess)
+      st = batch_launch_staged(sl.batch, work + off, trajs.data(), n_traj, knots, chunks, lanes, sl.stream);
     if (copy && e == hipSuccess && st == RBD_OK)
       e = hipMemcpyAsync(s->h_pool + out0, s->d_pool + out0, out_bytes, hipMemcpyDeviceToHost, sl.stream);
     if (e == hipSuccess && st == RBD_OK) e = hipEventRecord(sl.done, sl.stream);
@@ -328,6 +343,7 @@ void complete_loop(rbd_server* s, int slot) {
     sl.busy = false;
     s->mem_gen++;
     s->cv_done.notify_all();
+    s->wake.fetch_add(1, std::memory_order_release);
     s->cv_dispatch.notify_all();
   }
 }
@@ -371,7 +387,8 @@ rbd_status warm_up(rbd_server* s) {
       if (copy && e == hipSuccess)
         e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, stage + in_sz, hipMemcpyHostToDevice, sl.stream);
       if (e != hipSuccess) return hip_fail(e, "server warm-up upload");
-      if (rbd_status st = batch_launch_staged(sl.batch, work + off, n, n, chunks, lanes, sl.stream)) return st;
+      if (rbd_status st = batch_launch_staged(sl.batch, work + off, trajs.data(), n, n, chunks, lanes, sl.stream))
+        return st;
       if (copy)
         e = hipMemcpyAsync(s->h_pool + off + stage + in_sz, s->d_pool + off + stage + in_sz,
                            size_t(n) * rbd::kOutPerKnot * sizeof(float), hipMemcpyDeviceToHost, sl.stream);
@@ -477,6 +494,7 @@ rbd_status rbd_server_destroy(rbd_server* s) {
     s->queue.clear();
     s->stop = true;
   }
+  s->wake.fetch_add(1, std::memory_order_release);
   s->cv_dispatch.notify_all();
   if (s->dispatcher.joinable()) s->dispatcher.join();
   s->cv_complete.notify_all();
@@ -522,6 +540,7 @@ rbd_status rbd_server_submit(rbd_server* s, const float* h_in, float* h_out, int
     s->live[r->id] = r;
     s->queue.push_back(r);
   }
+  s->wake.fetch_add(1, std::memory_order_release);
   s->cv_dispatch.notify_all();
   return RBD_OK;
 }

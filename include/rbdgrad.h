@@ -312,7 +312,9 @@ typedef struct rbd_server_config {
   int zero_copy;                  /* 0: requests' buffers in device memory, one upload and one download per batch;
                                      1: the kernels read inputs from and write results to the pinned staging
                                      directly (no copies: lower latency, PCIe traffic from the kernels) */
-  int reserved[2];
+  int poll_dispatch;              /* 1: the dispatcher thread polls for requests instead of sleeping (one busy
+                                     core; saves the ~10 us wake-up of a sleeping thread per batch) */
+  int reserved[1];
 } rbd_server_config;
 
 typedef struct rbd_server_stats {

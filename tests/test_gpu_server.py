@@ -19,10 +19,10 @@ def ctx(rbd):
     c.close()
 
 
-@pytest.mark.parametrize("zero_copy", [0, 1])
+@pytest.mark.parametrize("zero_copy,poll", [(0, 0), (1, 0), (1, 1)])
 @pytest.mark.parametrize("lanes", [1, 16])
-def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_copy):
-    srv = rbd.Server(ctx, lanes_per_knot=lanes, slots=4, zero_copy=zero_copy)
+def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_copy, poll):
+    srv = rbd.Server(ctx, lanes_per_knot=lanes, slots=4, zero_copy=zero_copy, poll_dispatch=poll)
     rng = np.random.default_rng(21)
     probe = common.knots(128, seed=500)
     alone = srv.wait(srv.submit(probe))
@@ -77,10 +77,11 @@ def test_request_larger_than_budget_is_rejected(rbd, ctx):
     srv.close()
 
 
+@pytest.mark.parametrize("poll", [0, 1])
 @pytest.mark.parametrize("zero_copy", [0, 1])
 @pytest.mark.parametrize("slots", [1, 4])
-def test_poisson_load(rbd, ctx, slots, zero_copy):
-    srv = rbd.Server(ctx, slots=slots, zero_copy=zero_copy)
+def test_poisson_load(rbd, ctx, slots, zero_copy, poll):
+    srv = rbd.Server(ctx, slots=slots, zero_copy=zero_copy, poll_dispatch=poll)
     pool = common.knots(8192, seed=800)
     st = srv.run_poisson(rate=20000, duration_s=0.5, lengths=[16, 32, 64, 128, 256], pool=pool, seed=1)
     say("poisson 20k/s", st)
