@@ -126,6 +126,7 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
   const int lane = int(threadIdx.x);
   const int slot = lane / LANES;            // knot slot in the chunk
   const int role = lane % LANES;
+  __shared__ typename P::Minv mi_sh[64 / LANES][kMinvPacked];   // Minv per lane group (LANES = 16)
 
   auto process = [&](int c) {
     int t, first;                           // trajectory of chunk c, and its first chunk
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
       InAoS i{d.in + kInPerKnot * k};
       OutAoS o{d.out + kOutPerKnot * k};
       if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
-      else fdgrad_knot_lanes<P>(i, o, opt, role);
+      else fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[slot]);
     }
   };
 

@@ -332,6 +332,72 @@ RBD_HD RBD_INLINE void compute_minv(const CS<S>* t, S Mi[NQ][NQ]) {
   });
 }
 
+// Column j of Minv, for a lane that computes one column: col[k] = Minv[k][j]
+// for k <= j (entries k > j are left unspecified).  The articulated inertias
+// (U_k, D_k, the IA transports) are computed in full, the F and Minv updates
+// for column j only -- the recursion of compute_minv restricted to one column.
+// Through the backward pass column j's F stays exactly zero for every link
+// k > j (its Minv entry there is -D_k^-1 * 0), so one code path serves every j.
+template <typename S, typename SP>
+RBD_HD RBD_INLINE void compute_minv_column(const CS<S>* t, int j, S col[NQ]) {
+  S IA[6][6];
+  SP IA22;
+  iload<NQ - 1>(IA);
+  IA22 = SP(IA[2][2]);
+  S F[6] = {S(0), S(0), S(0), S(0), S(0), S(0)};
+  S U[NQ][6], Dinv[NQ];
+  static_for_down<NQ, 0>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    for (int r = 0; r < 6; ++r) U[k][r] = IA[r][2];
+    const SP dinv = SP(1) / IA22;
+    Dinv[k] = S(dinv);
+    col[k] = (j == k) ? Dinv[k] : -Dinv[k] * F[2];          // (21), (12)
+    for (int r = 0; r < 6; ++r) F[r] += U[k][r] * col[k];   // (14), (19)
+    if constexpr (k > 0) {
+      S y[6];
+      xforce_t<k>(t[k], F, y);                               // (15)
+      for (int r = 0; r < 6; ++r) F[r] = y[r];
+      S Ia[6][6];
+      S UD[6];
+      for (int r = 0; r < 6; ++r) UD[r] = U[k][r] * Dinv[k];
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c)
+          Ia[r][c] = (r == 2 || c == 2) ? S(0) : IA[r][c] - U[k][r] * UD[c];
+      S P[6][6];
+      transport<k>(t[k], Ia, P);
+      iload<k - 1>(IA);
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) IA[r][c] += P[r][c];
+      if constexpr (sizeof(SP) > sizeof(S)) {
+        SP IaW[6][6], PW[6][6];
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c < 6; ++c) IaW[r][c] = SP(Ia[r][c]);
+        const CS<SP> tw{SP(t[k].c), SP(t[k].s)};
+        transport<k>(tw, IaW, PW);
+        IA22 = SP(inertia_entry(k - 1, 2, 2)) + PW[2][2];
+      } else {
+        IA22 = SP(IA[2][2]);
+      }
+    }
+  });
+  // forward pass: F_k = S Minv[k, j] + T F_{k-1}
+  static_for<0, NQ>([&](auto Kc) {
+    constexpr int k = decltype(Kc)::value;
+    if constexpr (k == 0) {
+      for (int r = 0; r < 6; ++r) F[r] = S(0);
+      F[2] = col[0];
+    } else {
+      S w[6];
+      xmotion<k>(t[k], F, w);
+      const S uw = U[k][0] * w[0] + U[k][1] * w[1] + U[k][2] * w[2] +
+                   U[k][3] * w[3] + U[k][4] * w[4] + U[k][5] * w[5];
+      col[k] -= Dinv[k] * uw;                                 // (30)
+      w[2] += col[k];                                         // (32) + (34)
+      for (int r = 0; r < 6; ++r) F[r] = w[r];
+    }
+  });
+}
+
 // ---------------------------------------------------------------------------
 // RNEA forward pass (RNEA_fp, dynamicsGradient.h:619-659): v, a, f per link.
 // The gravity trick sets the base acceleration to +g along z.
@@ -620,25 +686,34 @@ struct Options {
   float gravity = float(kGravity);    // 0 in MPC_MODE (dynamicsGradient.h:635-637)
 };
 
-// Steps 1-5 of the full path: sin/cos, Minv, bias RNEA, qdd = Minv (u - c), RNEA
-// with qdd; fills the phase-C inputs and qdd.
+// sin/cos type of the full path: computed once in the widest arithmetic type
+template <class P>
+using KnotTrig = Wider<typename P::Minv, Wider<typename P::Rnea, typename P::Col>>;
+
 template <class P, class In>
-RBD_HD RBD_INLINE void phase_a(const In& in, const Options& opt, PhaseC<typename P::Col>& PC,
-                               typename P::Rnea qdd[NQ]) {
-  using SM = typename P::Minv; using SP = typename P::Pivot;
-  using SR = typename P::Rnea; using SC = typename P::Col;
-  using ST = Wider<Wider<SM, SR>, SC>;     // sin/cos computed once in the widest type
-  CS<SM> tm[NQ]; CS<SR> tr[NQ]; CS<SC> tc[NQ];
-  SR qd[NQ], u[NQ];
+RBD_HD RBD_INLINE void knot_trig(const In& in, CS<KnotTrig<P>> tw[NQ]) {
+  using ST = KnotTrig<P>;
   for (int k = 0; k < NQ; ++k) {
     ST s, c;
     sincos_s(ST(in.q(k)), &s, &c);
-    tm[k] = CS<SM>{SM(c), SM(s)}; tr[k] = CS<SR>{SR(c), SR(s)}; tc[k] = CS<SC>{SC(c), SC(s)};
+    tw[k] = CS<ST>{c, s};
+  }
+}
+
+// Steps 2-5 of the full path, given sin/cos and Minv: bias RNEA, qdd = Minv
+// (u - c), RNEA with qdd; fills the phase-C inputs and qdd.
+template <class P, class In>
+RBD_HD RBD_INLINE void phase_a_after_minv(const In& in, const Options& opt, const CS<KnotTrig<P>> tw[NQ],
+                                          const typename P::Minv Mi[NQ][NQ], PhaseC<typename P::Col>& PC,
+                                          typename P::Rnea qdd[NQ]) {
+  using SR = typename P::Rnea; using SC = typename P::Col;
+  CS<SR> tr[NQ]; CS<SC> tc[NQ];
+  SR qd[NQ], u[NQ];
+  for (int k = 0; k < NQ; ++k) {
+    tr[k] = CS<SR>{SR(tw[k].c), SR(tw[k].s)}; tc[k] = CS<SC>{SC(tw[k].c), SC(tw[k].s)};
     qd[k] = SR(in.qd(k));
     u[k] = SR(in.u(k));
   }
-  SM Mi[NQ][NQ];
-  compute_minv<SM, SP>(tm, Mi);
   // bias: RNEA with qdd = 0
   SR v[NQ][6], a[NQ][6], f[NQ][6];
   const SR zero[NQ] = {SR(0), SR(0), SR(0), SR(0), SR(0), SR(0), SR(0)};
@@ -668,6 +743,21 @@ RBD_HD RBD_INLINE void phase_a(const In& in, const Options& opt, PhaseC<typename
       for (int i = 0; i < 6; ++i) f[k][i] += dF[k][i];
   }
   make_phase_c(PC, tc, qd, v, a, f, Mi);
+}
+
+// Steps 1-5 of the full path: sin/cos, Minv, bias RNEA, qdd = Minv (u - c), RNEA
+// with qdd; fills the phase-C inputs and qdd.
+template <class P, class In>
+RBD_HD RBD_INLINE void phase_a(const In& in, const Options& opt, PhaseC<typename P::Col>& PC,
+                               typename P::Rnea qdd[NQ]) {
+  using SM = typename P::Minv; using SP = typename P::Pivot;
+  CS<KnotTrig<P>> tw[NQ];
+  knot_trig<P>(in, tw);
+  CS<SM> tm[NQ];
+  for (int k = 0; k < NQ; ++k) tm[k] = CS<SM>{SM(tw[k].c), SM(tw[k].s)};
+  SM Mi[NQ][NQ];
+  compute_minv<SM, SP>(tm, Mi);
+  phase_a_after_minv<P>(in, opt, tw, Mi, PC, qdd);
 }
 
 // Steps 1 and 5 of the reference-compatible modes: qdd given, and with
@@ -762,14 +852,64 @@ RBD_HD RBD_INLINE void fd_knot(const In& in, Out& out, const Options& opt) {
     for (int c = 0; c < NQ; ++c) out.minv(r, c, float(r <= c ? Mi[r][c] : Mi[c][r]));
 }
 
-// Full path for one knot spread over kLanesPerKnotGroup lanes: every lane of
-// the group runs steps 1-5 (identical values), then lane `role` computes one
+// Minv of a knot shared by its lane group: the packed upper triangle
+// (upper_index), in LDS in the kernels.
+constexpr int kMinvPacked = NQ * (NQ + 1) / 2;
+
+// index of (r, c), r <= c, in the packed upper triangle
+RBD_HD constexpr int upper_index(int r, int c) { return r * NQ - r * (r - 1) / 2 + (c - r); }
+
+// Lanes of one wave see each other's LDS writes made before this point.  The
+// fences order the LDS accesses of the wave (they cost no instruction); in the
+// host build, which runs the lane roles of a stage one after another, there
+// is nothing to do.
+RBD_HD RBD_INLINE void wave_lds_sync() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+#endif
+}
+
+// Lane-group stage 1: sin/cos (kept in tw for stage 2) and lane role's column
+// of Minv (roles 0..6), its upper entries stored into the group's mi.
+template <class P, class In>
+RBD_HD RBD_INLINE void lanes_minv_stage(const In& in, int role, CS<KnotTrig<P>> tw[NQ], typename P::Minv* mi) {
+  using SM = typename P::Minv; using SP = typename P::Pivot;
+  knot_trig<P>(in, tw);
+  CS<SM> tm[NQ];
+  for (int k = 0; k < NQ; ++k) tm[k] = CS<SM>{SM(tw[k].c), SM(tw[k].s)};
+  const int j = role < NQ ? role : NQ - 1;
+  SM col[NQ];
+  compute_minv_column<SM, SP>(tm, j, col);
+  if (role < NQ)
+    for (int k = 0; k < NQ; ++k)
+      if (k <= j) mi[upper_index(k, j)] = col[k];
+}
+
+// Minv (upper triangle) from a group's packed mi
+template <typename S>
+RBD_HD RBD_INLINE void load_packed_minv(const S* mi, S Mi[NQ][NQ]) {
+  static_for<0, NQ>([&](auto Rc) {
+    constexpr int r = decltype(Rc)::value;
+    static_for<r, NQ>([&](auto Cc) {
+      constexpr int c = decltype(Cc)::value;
+      Mi[r][c] = mi[upper_index(r, c)];
+    });
+  });
+}
+
+// Lane-group stage 2 (after every lane's stage 1): steps 2-5 with the shared
+// Minv (identical values in every lane), then lane `role` computes one
 // derivative column (steps 6-7) and stores it.
-template <class P = PrecFP32, class In, class Out>
-RBD_HD RBD_INLINE void fdgrad_knot_lanes(const In& in, Out& out, const Options& opt, int role) {
+template <class P, class In, class Out>
+RBD_HD RBD_INLINE void lanes_finish_stage(const In& in, Out& out, const Options& opt, int role,
+                                          const CS<KnotTrig<P>> tw[NQ], const typename P::Minv* mi) {
+  typename P::Minv Mi[NQ][NQ];
+  load_packed_minv(mi, Mi);
   PhaseC<typename P::Col> PC;
   typename P::Rnea qdd[NQ];
-  phase_a<P>(in, opt, PC, qdd);
+  phase_a_after_minv<P>(in, opt, tw, Mi, PC, qdd);
   if (role == kRoleQdd) {
     for (int r = 0; r < NQ; ++r) out.qdd(r, float(qdd[r]));
     for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
@@ -779,11 +919,26 @@ RBD_HD RBD_INLINE void fdgrad_knot_lanes(const In& in, Out& out, const Options& 
   gradient_column_lane(PC, opt.damping, role, out);
 }
 
+// Full path for one knot spread over kLanesPerKnotGroup lanes of one wave:
+// lanes 0..6 each compute one column of Minv and share it through the group's
+// LDS slot mi (kMinvPacked values), then every lane runs steps 2-5 and one
+// derivative column.  Device code only: the host build runs the two stages
+// for every lane role in turn.
+template <class P = PrecFP32, class In, class Out>
+RBD_HD RBD_INLINE void fdgrad_knot_lanes(const In& in, Out& out, const Options& opt, int role,
+                                         typename P::Minv* mi) {
+  CS<KnotTrig<P>> tw[NQ];
+  lanes_minv_stage<P>(in, role, tw, mi);
+  wave_lds_sync();
+  lanes_finish_stage<P>(in, out, opt, role, tw, mi);
+}
+
 // ---------------------------------------------------------------------------
 // Two-wave latency mapping.  For one knot, one wave (M) runs the Minv
-// recursion while a second wave (R) runs the RNEA at qdd = 0 and all thirteen
-// derivative columns with the bias accelerations and forces a_b, f_b; both
-// are ~1.7k instructions, so they overlap instead of adding up.  The columns
+// recursion, lane j of the knot's group computing column j, while a second
+// wave (R) runs the RNEA at qdd = 0 and all thirteen derivative columns with
+// the bias accelerations and forces a_b, f_b; the two overlap instead of
+// adding up.  The columns
 // need a, f at the true qdd only in the d/dq seeds (a_c x S at link c) and the
 // backward extra term (f_c at link c), in which a d/dq column is linear.  So
 // with da = a - a_b and df = f - f_b, both linear in qdd:
@@ -793,21 +948,19 @@ RBD_HD RBD_INLINE void fdgrad_knot_lanes(const In& in, Out& out, const Options& 
 // the chain da_k = T_k da_{k-1} + S qdd_k, dF_k = I_k da_k, df = RNEA
 // backward pass of dF, and each d/dq lane's correction column'; R then adds
 // the corrections and forms -Minv dtau.  Stage functions are per (knot, lane
-// role); wave M runs M1, waits for R1a's bias, runs M2 and signals; wave R
-// runs R1a, signals, runs R1b, waits for M2, runs R2.  The critical path is
-// M1 + M2 + R2 (~2.7k instructions against ~3.5k for the one-wave lane group).
+// role); wave M runs M1a (its Minv column into LDS), M1b (the whole Minv back
+// from LDS), waits for R1a's bias, runs M2 and signals; wave R runs R1a,
+// signals, runs R1b, waits for M2, runs R2.  The critical path is the longer
+// of M1 + M2 and R1, plus R2.
 // ---------------------------------------------------------------------------
 
 // Exchange between the waves for one knot (LDS in the kernel).
 template <class P>
 struct TwoWaveShared {
   typename P::Rnea bias[NQ];                        // u - c - damping qd (R1 -> M2)
-  typename P::Minv mi[NQ * (NQ + 1) / 2];           // upper Minv, row-major packed (M2 -> R2)
+  typename P::Minv mi[kMinvPacked];                 // upper Minv, packed (M1a -> M1b, R2)
   typename P::Col corr[kLanesPerKnotGroup][NQ];     // d/dq corrections by lane role (M2 -> R2)
 };
-
-// index of (r, c), r <= c, in the packed upper triangle
-RBD_HD constexpr int upper_index(int r, int c) { return r * NQ - r * (r - 1) / 2 + (c - r); }
 
 // column'(da_c x S, df_c) for lane role's d/dq column c, uniform over lanes
 // like dtau_column_uniform: dax[k] = (da_k x S) as (a1, -a0, a4, -a3),
@@ -897,24 +1050,23 @@ RBD_HD RBD_INLINE void two_wave_r1b(const Options& opt, int role, const TwoWaveR
 
 // sin/cos type of wave M (shared by its two stages)
 template <class P>
-using TwoWaveTrig = Wider<typename P::Minv, Wider<typename P::Rnea, typename P::Col>>;
+using TwoWaveTrig = KnotTrig<P>;
 
-// Stage M1 (wave M): sin/cos (kept for M2) and Minv.
+// Stage M1a (wave M): sin/cos (kept for M2) and lane role's column of Minv
+// (roles 0..6), its upper entries stored into sh.mi.
 template <class P, class In>
-RBD_HD RBD_INLINE void two_wave_m1(const In& in, CS<TwoWaveTrig<P>> tw[NQ], typename P::Minv Mi[NQ][NQ]) {
-  using SM = typename P::Minv; using SP = typename P::Pivot; using ST = TwoWaveTrig<P>;
-  CS<SM> tm[NQ];
-  for (int k = 0; k < NQ; ++k) {
-    ST s, c;
-    sincos_s(ST(in.q(k)), &s, &c);
-    tw[k] = CS<ST>{c, s};
-    tm[k] = CS<SM>{SM(c), SM(s)};
-  }
-  compute_minv<SM, SP>(tm, Mi);
+RBD_HD RBD_INLINE void two_wave_m1a(const In& in, int role, CS<TwoWaveTrig<P>> tw[NQ], TwoWaveShared<P>& sh) {
+  lanes_minv_stage<P>(in, role, tw, sh.mi);
+}
+
+// Stage M1b (wave M, after every lane's M1a): the whole upper Minv from sh.mi.
+template <class P>
+RBD_HD RBD_INLINE void two_wave_m1b(const TwoWaveShared<P>& sh, typename P::Minv Mi[NQ][NQ]) {
+  load_packed_minv(sh.mi, Mi);
 }
 
 // Stage M2 (wave M, after R1): qdd (stored by lane role kRoleQdd), the
-// qdd-dependent chain, lane role's d/dq correction and Minv into sh.
+// qdd-dependent chain and lane role's d/dq correction into sh.
 template <class P, class Out>
 RBD_HD RBD_INLINE void two_wave_m2(int role, const CS<TwoWaveTrig<P>> tw[NQ], const typename P::Minv Mi[NQ][NQ],
                                    TwoWaveShared<P>& sh, Out& out) {
@@ -950,15 +1102,6 @@ RBD_HD RBD_INLINE void two_wave_m2(int role, const CS<TwoWaveTrig<P>> tw[NQ], co
   SC corr[NQ];
   dq_correction_uniform(tc, dax, ndfx, role - (NQ - 1), corr);
   for (int r = 0; r < NQ; ++r) sh.corr[role][r] = corr[r];
-  // the packed upper Minv, two entries per lane role
-  static_for<0, NQ>([&](auto Rc) {
-    constexpr int r = decltype(Rc)::value;
-    static_for<r, NQ>([&](auto Cc) {
-      constexpr int c = decltype(Cc)::value;
-      constexpr int idx = upper_index(r, c);
-      if (role == idx % kLanesPerKnotGroup) sh.mi[idx] = Mi[r][c];
-    });
-  });
 }
 
 // Stage R2 (wave R, after M2): corrections added, -Minv dtau, stores.

@@ -87,11 +87,12 @@ template <class P, int W>
 __global__ __launch_bounds__(kBlock, W) void fdgrad_aos_lanes_kernel(long n, const float* __restrict__ in,
                                                                   float* __restrict__ out, Options opt) {
   const int role = threadIdx.x % kLanesPerKnotGroup;
+  __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
   const long groups = long(gridDim.x) * (blockDim.x / kLanesPerKnotGroup);
   for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / kLanesPerKnotGroup; k < n; k += groups) {
     InAoS i{in + kInPerKnot * k};
     OutAoS o{out + kOutPerKnot * k};
-    fdgrad_knot_lanes<P>(i, o, opt, role);
+    fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
   }
 }
 
@@ -99,11 +100,12 @@ template <class P, int W>
 __global__ __launch_bounds__(kBlock, W) void fdgrad_soa_lanes_kernel(long n, const float* __restrict__ in, long ld_in,
                                                                   float* __restrict__ out, long ld_out, Options opt) {
   const int role = threadIdx.x % kLanesPerKnotGroup;
+  __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
   const long groups = long(gridDim.x) * (blockDim.x / kLanesPerKnotGroup);
   for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / kLanesPerKnotGroup; k < n; k += groups) {
     InSoA i{in + k, ld_in};
     OutSoA o{out + k, ld_out};
-    fdgrad_knot_lanes<P>(i, o, opt, role);
+    fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
   }
 }
 
@@ -175,7 +177,9 @@ __global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, 
   if (wave == 0) {
     CS<TwoWaveTrig<P>> tw[NQ];
     typename P::Minv Mi[NQ][NQ];
-    two_wave_m1<P>(i, tw, Mi);
+    two_wave_m1a<P>(i, role, tw, sh[g]);
+    wave_lds_sync();
+    two_wave_m1b<P>(sh[g], Mi);
     for (int r = 0; r < NQ; ++r) {
       pin(tw[r].c);
       pin(tw[r].s);
@@ -202,12 +206,13 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_minv_kernel(l
                                                              float* __restrict__ out, float* __restrict__ minv,
                                                              Options opt) {
   const int role = threadIdx.x % LANES;
+  __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
   const long groups = long(gridDim.x) * (blockDim.x / LANES);
   for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
     InAoS i{in + kInPerKnot * k};
     OutAoSMinv o{out + kOutPerKnot * k, minv + NQ * NQ * k};
     if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
-    else fdgrad_knot_lanes<P>(i, o, opt, role);
+    else fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
   }
 }
 
@@ -218,12 +223,13 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_fext_kernel(l
                                                              const float* __restrict__ fext, long fext_stride,
                                                              float* __restrict__ out, Options opt) {
   const int role = threadIdx.x % LANES;
+  __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
   const long groups = long(gridDim.x) * (blockDim.x / LANES);
   for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
     InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
     OutAoS o{out + kOutPerKnot * k};
     if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
-    else fdgrad_knot_lanes<P>(i, o, opt, role);
+    else fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
   }
 }
 

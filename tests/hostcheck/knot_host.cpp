@@ -5,6 +5,8 @@
 // GPU run; it is not a product path and nothing outside tests/ loads it.
 #include "../../fast-rbd-gradients_amd/csrc/fdgrad_knot.hpp"
 
+#include <cmath>
+
 using namespace rbd;
 
 static Options options(int flags) {
@@ -12,6 +14,16 @@ static Options options(int flags) {
   o.damping = (flags & 1) ? 0.5f : 0.0f;
   o.gravity = (flags & 2) ? 0.0f : float(kGravity);
   return o;
+}
+
+// the lane-group mapping, emulated: stage 1 (Minv columns into the group's
+// shared block) for every lane role, then stage 2 for every role
+template <class P, class In, class Out>
+static void lane_group(const In& i, Out& o, const Options& opt) {
+  typename P::Minv mi[kMinvPacked];
+  CS<KnotTrig<P>> tw[kLanesPerKnotGroup][NQ];
+  for (int role = 0; role < kLanesPerKnotGroup; ++role) lanes_minv_stage<P>(i, role, tw[role], mi);
+  for (int role = 0; role < kLanesPerKnotGroup; ++role) lanes_finish_stage<P>(i, o, opt, role, tw[role], mi);
 }
 
 // lanes != 0: emulate the lane-group mapping, running every lane role of a
@@ -31,7 +43,7 @@ static void run(bool ref, int flags, long n, const float* in, float* out, int la
       InAoS i{in + kInPerKnot * k};
       OutAoS o{out + kOutPerKnot * k};
       if (lanes)
-        for (int role = 0; role < kLanesPerKnotGroup; ++role) fdgrad_knot_lanes<P>(i, o, opt, role);
+        lane_group<P>(i, o, opt);
       else
         fdgrad_knot<P>(i, o, opt);
     }
@@ -48,9 +60,9 @@ static void by_precision(int prec, bool ref, int flags, long n, const float* in,
   }
 }
 
-// two-wave latency mapping, emulated: for each knot, stage M1, R1a and R1b
-// for every lane role, M2 for every role, R2 for every role (an order the
-// kernel's flags allow)
+// two-wave latency mapping, emulated: for each knot, stage M1a for every lane
+// role, M1b, R1a and R1b for every role, M2 for every role, R2 for every role
+// (an order the kernel's flags allow)
 template <class P>
 static void two_wave(long n, const float* in, float* out, int flags) {
   const Options opt = options(flags);
@@ -62,7 +74,8 @@ static void two_wave(long n, const float* in, float* out, int flags) {
     typename P::Minv Mi[NQ][NQ];
     typename P::Col tau[kLanesPerKnotGroup][NQ];
     static TwoWaveR<P> st[kLanesPerKnotGroup];
-    two_wave_m1<P>(i, tw, Mi);
+    for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_m1a<P>(i, role, tw, sh);
+    two_wave_m1b<P>(sh, Mi);
     for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_r1a<P>(i, opt, role, sh, st[role]);
     for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_r1b<P>(opt, role, st[role], tau[role]);
     for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_m2<P>(role, tw, Mi, sh, o);
@@ -73,6 +86,25 @@ extern "C" {
 // The device model tables: spatial inertia of link k (6x6 column-major) and the
 // joint transform T_k(q) (6x6 column-major, built by applying T to unit vectors).
 const char* host_model_name(void) { return kModelName; }
+
+// Minv at n poses (q: n x 7), by the whole recursion (full, 49 per pose,
+// Minv[r][c] at r * 7 + c, upper triangle) and by the one-column recursion for
+// each column j (cols, the same layout, entries r <= c only).
+void host_minv_columns(long n, const double* q, double* full, double* cols) {
+  for (long p = 0; p < n; ++p) {
+    CS<float> t[NQ];
+    for (int k = 0; k < NQ; ++k) t[k] = CS<float>{float(std::cos(q[NQ * p + k])), float(std::sin(q[NQ * p + k]))};
+    float Mi[NQ][NQ];
+    compute_minv<float, float>(t, Mi);
+    for (int r = 0; r < NQ; ++r)
+      for (int c = r; c < NQ; ++c) full[NQ * NQ * p + NQ * r + c] = Mi[r][c];
+    for (int j = 0; j < NQ; ++j) {
+      float col[NQ];
+      compute_minv_column<float, float>(t, j, col);
+      for (int r = 0; r <= j; ++r) cols[NQ * NQ * p + NQ * r + j] = col[r];
+    }
+  }
+}
 
 void host_model_inertia(int k, double* out) {
   float A[6][6];
@@ -118,7 +150,7 @@ void host_fdgrad_minv_f32(long n, const float* in, float* out, float* minv, int 
     InAoS i{in + kInPerKnot * k};
     OutAoSMinv o{out + kOutPerKnot * k, minv + NQ * NQ * k};
     if (lanes)
-      for (int role = 0; role < kLanesPerKnotGroup; ++role) fdgrad_knot_lanes<PrecFP32>(i, o, opt, role);
+      lane_group<PrecFP32>(i, o, opt);
     else
       fdgrad_knot<PrecFP32>(i, o, opt);
   }
@@ -133,7 +165,7 @@ void host_fdgrad_fext_f32(long n, const float* in, const float* fext, long fext_
     InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
     OutAoS o{out + kOutPerKnot * k};
     if (lanes)
-      for (int role = 0; role < kLanesPerKnotGroup; ++role) fdgrad_knot_lanes<PrecFP32>(i, o, opt, role);
+      lane_group<PrecFP32>(i, o, opt);
     else
       fdgrad_knot<PrecFP32>(i, o, opt);
   }

@@ -198,3 +198,20 @@ def test_two_wave_mapping(host, oracle, flags, prec):
     # the same values as the one-knot-per-lane-group mapping up to rounding
     ref = run(host, x, flags, prec, lanes=1)
     assert np.abs(out - ref).max() <= 1e-3 * np.abs(ref).max()
+
+
+def test_minv_by_columns_equals_the_whole_recursion(host):
+    """compute_minv_column (one column per lane, the two-wave mapping's Minv)
+    gives each upper entry exactly as compute_minv does."""
+    rng = np.random.default_rng(290)
+    q = rng.uniform(-np.pi, np.pi, (200, 7))
+    q[::4, [1, 3, 5]] = rng.uniform(-1e-3, 1e-3, (50, 3))
+    full = np.full(200 * 49, np.nan)
+    cols = np.full(200 * 49, np.nan)
+    dp = ctypes.POINTER(ctypes.c_double)
+    host.host_minv_columns(200, np.ascontiguousarray(q).ctypes.data_as(dp), full.ctypes.data_as(dp),
+                           cols.ctypes.data_as(dp))
+    upper = np.triu(np.ones((7, 7), bool)).ravel()
+    full, cols = full.reshape(200, 49)[:, upper], cols.reshape(200, 49)[:, upper]
+    assert np.all(np.isfinite(full))
+    np.testing.assert_array_equal(cols, full)
