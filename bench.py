@@ -88,7 +88,9 @@ def time_launches(launch, steps, warmup, stream):
 
 class OracleChecks:
     """GPU outputs kept for the CPU leg, which checks them against the FP64
-    oracle (max relative error per block on the same FP32 inputs).  A sample of
+    oracle on the same FP32 inputs: per block and knot max|d| / max(max|ref|,
+    1e-6) (max and p99.9 over knots), and the elementwise max of
+    |d| / (|ref| + 1e-3 max|ref|).  A sample of
     rows is taken when the output is recorded, so large outputs are not held;
     the result is written into target[key] when the CPU leg runs."""
 
@@ -105,16 +107,22 @@ class OracleChecks:
         from oracle import pyoracle
         threads = min(16, os.cpu_count() or 1)
         for target, key, o, x, model in self.items:
-            if model is not None:
+            if x.shape[1] == rbdgrad.VOL_PER_KNOT:          # reference-compatible mode: 98 out, no qdd
+                ref = np.concatenate([x[:, 21:28], pyoracle.fdgrad_refmode(x)], axis=1)
+                o = np.concatenate([x[:, 21:28], o], axis=1)
+            elif model is not None:
                 with pyoracle.using_model(*model):
                     ref = pyoracle.fdgrad(x, threads=threads)
             else:
                 ref = pyoracle.fdgrad(x, threads=threads)
             res = {}
             for name, sl in (("qdd", slice(0, 7)), ("dqdd_dq", slice(7, 56)), ("dqdd_dqd", slice(56, 105))):
-                e = np.max(np.abs(o[:, sl] - ref[:, sl]), axis=1) / np.maximum(np.max(np.abs(ref[:, sl]), axis=1),
-                                                                               1e-6)
-                res[name] = {"max": float(e.max()), "p99.9": float(np.percentile(e, 99.9))}
+                d, r = np.abs(o[:, sl] - ref[:, sl]), np.abs(ref[:, sl])
+                rmax = np.max(r, axis=1)
+                e = np.max(d, axis=1) / np.maximum(rmax, 1e-6)               # normwise per block and knot
+                el = d / (r + 1e-3 * rmax[:, None] + 1e-30)                  # elementwise
+                res[name] = {"max": float(e.max()), "p99.9": float(np.percentile(e, 99.9)),
+                             "elementwise_max": float(el.max())}
             res["knots_checked"] = int(len(x))
             target[key] = res
 
@@ -154,26 +162,46 @@ def run_flat(ctx, x, steps, warmup, stream, layout="aos"):
     return ms, out
 
 
-def cpu_baseline(x, seconds):
+def cpu_baseline(x, seconds, vol=None):
     """The oracle's FP32 restatement of the reference's ReusableThreads loop on
     this host: 4 threads (the reference's CPU_THREADS_GLOBAL) and all usable
-    cores (capped at 16, the GPU host's CPU share); the faster is reported."""
+    cores (capped at 16, the GPU host's CPU share); the faster is reported.
+    With vol (77 floats per knot) the reference's own timed configuration
+    (QDD_MINV_PASSED_IN: qdd and Minv given) is timed too, at the same thread
+    counts, and reported under "reference_mode"."""
     from oracle import pyoracle
     n = len(x)
     cores = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
-    results = []
-    for threads in sorted({4, cores}):
-        times, _ = pyoracle.cpu_bench(x, threads, iters=10 ** 7, mode=0, time_cap_s=seconds)
+
+    def leg(inp, mode, threads, secs):
+        times, _ = pyoracle.cpu_bench(inp, threads, iters=10 ** 7, mode=mode, time_cap_s=secs)
         times = times[len(times) // 10:]          # drop the first 10% (pool warm-up)
         mean_us = float(np.mean(times))
-        results.append({"value": n / (mean_us * 1e-6), "unit": "knots/s", "cores": threads, "kind": "port",
-                        "sample": f"{len(times)} calls x {n} knots (config 1 inputs), FP32, ReusableThreads-style "
-                                  f"pool, {threads} threads, ~{seconds:.0f} s",
-                        "p50_us": float(np.percentile(times, 50)), "p99_us": float(np.percentile(times, 99))})
-        log(f"cpu baseline {threads} threads: {results[-1]['value']:.3e} knots/s")
-    best = max(results, key=lambda r: r["value"])
-    best = dict(best)
-    best["other"] = [r for r in results if r is not best and r["cores"] != best["cores"]]
+        what = "full path (q, qd, u in)" if mode == 0 else "qdd and Minv given (QDD_MINV_PASSED_IN)"
+        r = {"value": n / (mean_us * 1e-6), "unit": "knots/s", "cores": threads, "kind": "port",
+             "sample": f"{len(times)} calls x {n} knots (config 1 inputs), {what}, FP32, ReusableThreads-style "
+                       f"pool, {threads} threads, ~{secs:.0f} s",
+             "mean_us": mean_us, "std_us": float(np.std(times)),
+             "p50_us": float(np.percentile(times, 50)), "p99_us": float(np.percentile(times, 99))}
+        log(f"cpu baseline mode {mode} {threads} threads: {r['value']:.3e} knots/s")
+        return r
+
+    results = [leg(x, 0, t, seconds) for t in sorted({4, cores})]
+    best = dict(max(results, key=lambda r: r["value"]))
+    best["other"] = [r for r in results if r["cores"] != best["cores"]]
+    if vol is not None:
+        ref = [leg(vol, 1, t, seconds / 2) for t in sorted({4, cores})]
+        best["reference_mode"] = ref
+    # the reference harness's trajectory lengths, 4 threads, about 1 s each
+    sweep = {}
+    for m in N_SET:
+        times, _ = pyoracle.cpu_bench(knots(m, seed=110 + m), 4, iters=10 ** 7, mode=0, time_cap_s=1.0)
+        times = times[len(times) // 10:]
+        sweep[str(m)] = {"mean_us": float(np.mean(times)), "std_us": float(np.std(times)),
+                         "p50_us": float(np.percentile(times, 50)), "p99_us": float(np.percentile(times, 99)),
+                         "knots_per_s": m / (float(np.mean(times)) * 1e-6), "calls": int(len(times))}
+    best["over_n_4_threads"] = sweep
+    log(f"cpu baseline over N (4 threads, us per call): { {k: round(v['mean_us'], 1) for k, v in sweep.items()} }")
     return best
 
 
@@ -239,10 +267,69 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000), duration_s=1.
             log(f"[c3] served ({mode}): Poisson {rate} req/s for {duration_s} s")
             st = srv.run_poisson(rate, duration_s, [16, 32, 64, 128, 256], pool, seed=rate)
             st["batch_phases_us"] = batch_phases(srv.trace())
+            st["hbm_frac"] = st["knots_per_s"] * BYTES_PER_KNOT / 1e9 / HBM_PEAK_GBS
+            st["fp32_frac"] = st["knots_per_s"] * FLOPS_PER_KNOT / 1e12 / FP32_PEAK_TFLOPS
             res[f"served_{mode}_{rate}_req_per_s"] = st
             log(f"[c3] {st}")
         srv.close()
     return res
+
+
+def reference_mode_workload(ctx, stream, steps, warmup):
+    """The reference's own timed configuration (dynamicsGradientKernel,
+    time_GPU.cu:279-316): q, qd, u, qdd and the dense Minv in (77 floats per
+    knot, qdd and Minv from forward dynamics, outside the timed region), the
+    98 derivative values out; one launch of 128 knots per call."""
+    n = 128
+    x = knots(n, seed=101)
+    qdd, minv = ctx.forward_dynamics(x)
+    vol = np.ascontiguousarray(np.concatenate([x, qdd, minv.transpose(0, 2, 1).reshape(n, 49)], axis=1),
+                               dtype=np.float32)                  # Minv column-major, as the reference packs it
+    d_vol = rbdgrad.DeviceBuffer.from_array(vol)
+    d_out = rbdgrad.DeviceBuffer(n * rbdgrad.DQDD_PER_KNOT * 4)
+    ms = time_launches(lambda: ctx.dynamics_gradient_kernel(n, d_out, d_vol, stream=stream), steps, warmup, stream)
+    out = d_out.download((n, rbdgrad.DQDD_PER_KNOT))
+    per = ms / steps
+    r = {"kernel": ctx.kernel_name(n, reference_mode=True), "us_per_launch": per * 1e3,
+         "knots_per_s": n / (per * 1e-3), "bytes_per_knot": 700,
+         "hbm_gbs": n * 700 / (per * 1e-3) / 1e9}
+    CHECKS.add(r, "accuracy", out, vol)
+    d_vol.free()
+    d_out.free()
+    log(f"[c1] reference-compatible mode: {r}")
+    return {"per_call_n128_reference_mode": r}, vol
+
+
+N_SET = (1, 10, 16, 32, 64, 128, 256)   # trajectory lengths of the reference's harness (time_CPU.cpp:149-156)
+
+
+def n_sweep(ctx, stream, steps, warmup):
+    """Per-call latency over the reference harness's trajectory lengths (GPU;
+    the CPU leg adds the same lengths on the CPU)."""
+    res = {}
+    for n in N_SET:
+        x = knots(n, seed=110 + n)
+        ms, _ = run_flat(ctx, x, steps, warmup, stream)
+        res[str(n)] = {"kernel": ctx.kernel_name(n), "us_per_launch": ms / steps * 1e3,
+                       "knots_per_s": n * steps / (ms * 1e-3)}
+    log(f"[sweep] per call over N: { {k: round(v['us_per_launch'], 2) for k, v in res.items()} }")
+    return res
+
+
+def copy_bandwidth(stream, nbytes=(1 << 20) * 420, reps=20):
+    """Achievable HBM bandwidth: a device-to-device copy of nbytes (read + write counted)."""
+    a, b = rbdgrad.DeviceBuffer(nbytes), rbdgrad.DeviceBuffer(nbytes)
+    a.fill_zero()
+    H = rbdgrad.hip()
+
+    def launch():
+        rbdgrad._hipcheck(H.hipMemcpyAsync(b.handle, a.handle, nbytes, rbdgrad.D2D, stream.handle), "hipMemcpyAsync")
+    ms = time_launches(launch, reps, 3, stream) / reps
+    a.free()
+    b.free()
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    log(f"[copy] {nbytes} bytes device to device: {ms * 1e3:.1f} us, {gbs:.0f} GB/s (read + write)")
+    return {"bytes": nbytes, "us": ms * 1e3, "gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
 
 
 def urdf_model_workload(stream, steps, warmup, name="arm7",
@@ -344,7 +431,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000, help="timed launches of the per-call workload")
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--precision", default="fp32", choices=list(rbdgrad.PRECISIONS))
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline time per thread count")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="CPU baseline time per thread count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the large-batch and mixed workloads")
     ap.add_argument("--only", choices=["c1", "c1-one-lane", "c1-lane-group", "c4-aos", "c4-soa", "c2", "c3", "copy"],
@@ -409,6 +496,9 @@ def main():
         "knots_per_s": n1 * (args.steps // 4) / (ms1c * 1e-3)}
     log(f"[c1] one-wave lane group: {workloads['per_call_n128_lane_group']}")
     grp.close()
+    ref_mode, vol1 = reference_mode_workload(ctx, stream, args.steps // 4, args.warmup)
+    workloads.update(ref_mode)
+    workloads["per_call_over_n"] = n_sweep(ctx, stream, args.steps // 4, args.warmup)
     if not args.quick:
         # ---- large batch: 2^20 knots, 10% stretched-arm poses ----
         n4 = 1 << 20
@@ -434,13 +524,14 @@ def main():
         workloads.update(mixed_workload(ctx, stream))
         workloads.update(served_workload(ctx))
         workloads.update(urdf_model_workload(stream, args.steps // 4, args.warmup))
+        workloads["hbm_copy"] = copy_bandwidth(stream)
 
     # ---- CPU leg: the oracle as the CPU baseline, and as the checker of the GPU outputs ----
     if args.no_cpu:
         cpu = None
         CHECKS.skip("--no-cpu")
     else:
-        cpu = cpu_baseline(x1, args.cpu_seconds)
+        cpu = cpu_baseline(x1, args.cpu_seconds, vol1)
         log("checking the GPU outputs against the FP64 oracle")
         CHECKS.run()
     acc1 = checked["accuracy"]
