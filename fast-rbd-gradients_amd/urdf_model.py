@@ -220,18 +220,18 @@ def build(joints, links):
 
 
 def axis_frame(axis):
-    """R_a with R_a z = axis/|axis| (the smallest rotation; a half-turn about x for -z)."""
-    n = math.sqrt(sum(float(a) ** 2 for a in axis))
-    a = [float(x) / n for x in axis]
-    if a[2] > 1.0 - 1e-15:
-        return [row[:] for row in _EYE]
-    if a[2] < -1.0 + 1e-15:
-        return [[1.0, 0.0, 0.0], [0.0, -1.0, 0.0], [0.0, 0.0, -1.0]]
-    v = [-a[1], a[0], 0.0]                 # z x a
-    K = [[0.0, -v[2], v[1]], [v[2], 0.0, -v[0]], [-v[1], v[0], 0.0]]
-    K2 = _mm(K, K)
-    f = 1.0 / (1.0 + a[2])
-    return [[(1.0 if i == j else 0.0) + K[i][j] + f * K2[i][j] for j in range(3)] for i in range(3)]
+    """A rotation R_a with R_a z = axis/|axis|: its columns are x', y' = a x x'
+    and a, with x' the unit part of e_x (e_y for axes near x) orthogonal to a.
+    R_a = 1 exactly for +z; exact to rounding for every other axis."""
+    n = math.sqrt(sum(float(v) ** 2 for v in axis))
+    a = [float(v) / n for v in axis]
+    e = [1.0, 0.0, 0.0] if abs(a[0]) < 0.9 else [0.0, 1.0, 0.0]
+    d = sum(e[i] * a[i] for i in range(3))
+    x = [e[i] - d * a[i] for i in range(3)]
+    xn = math.sqrt(sum(v * v for v in x))
+    x = [v / xn for v in x]
+    y = [a[1] * x[2] - a[2] * x[1], a[2] * x[0] - a[0] * x[2], a[0] * x[1] - a[1] * x[0]]
+    return [[x[i], y[i], a[i]] for i in range(3)]
 
 
 def tables(model):

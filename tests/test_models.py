@@ -213,3 +213,23 @@ def test_op_count_on_urdf_model(on_arm7):
     import bench
     c = on_arm7.op_count(0)
     assert c["add"] + c["mul"] + c["div"] == bench.FLOPS_PER_KNOT_ARM7
+
+
+@pytest.mark.parametrize("axis", [(0, 0, 1), (0, 0, -1), (1, 0, 0), (0, 1, 0), (0.6, 0, 0.8), (1, 2, -3),
+                                  (0, 0, 5), (1e-9, 0, -1)])
+def test_axis_frame_takes_z_to_the_axis(axis):
+    import urdf_model
+    R = np.array(urdf_model.axis_frame(axis))
+    a = np.array(axis, float) / np.linalg.norm(axis)
+    np.testing.assert_allclose(R @ [0, 0, 1], a, atol=1e-12)
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+    assert abs(np.linalg.det(R) - 1) < 1e-12
+
+
+def test_rpy_matrix_is_the_urdf_convention():
+    """R = Rz(yaw) Ry(pitch) Rx(roll), against the textbook model's own rotation."""
+    import urdf_model
+    rng = np.random.default_rng(640)
+    for _ in range(20):
+        r, p, y = rng.uniform(-np.pi, np.pi, 3)
+        np.testing.assert_allclose(np.array(urdf_model.rpy_matrix(r, p, y)), common._rpy(r, p, y), atol=1e-14)
