@@ -496,6 +496,16 @@ def main():
         "knots_per_s": n1 * (args.steps // 4) / (ms1c * 1e-3)}
     log(f"[c1] one-wave lane group: {workloads['per_call_n128_lane_group']}")
     grp.close()
+    # ... and with the hardware sin/cos precision mode
+    hw = rbdgrad.Context(device=0, precision="fp32_hw_trig")
+    hw.autotune(stream)
+    ms1d, out1d = run_flat(hw, x1, args.steps // 4, args.warmup, stream)
+    r = workloads["per_call_n128_fp32_hw_trig"] = {
+        "kernel": hw.kernel_name(n1), "us_per_launch": ms1d / (args.steps // 4) * 1e3,
+        "knots_per_s": n1 * (args.steps // 4) / (ms1d * 1e-3)}
+    CHECKS.add(r, "accuracy", out1d, x1)
+    log(f"[c1] hardware sin/cos: {r}")
+    hw.close()
     ref_mode, vol1 = reference_mode_workload(ctx, stream, args.steps // 4, args.warmup)
     workloads.update(ref_mode)
     workloads["per_call_over_n"] = n_sweep(ctx, stream, args.steps // 4, args.warmup)
@@ -504,9 +514,9 @@ def main():
         n4 = 1 << 20
         x4 = knots(n4, seed=104, near_singular_frac=0.1)
         steps4 = 20
-        for prec in ("fp32", "fp64_columns"):
+        for prec in ("fp32", "fp64_columns", "fp32_hw_trig"):
             c4 = ctx if prec == args.precision else rbdgrad.Context(device=0, precision=prec)
-            for layout in ("aos", "soa"):
+            for layout in (("aos", "soa") if prec != "fp32_hw_trig" else ("soa",)):
                 log(f"[c4] 2^20 knots prec={prec} layout={layout}: 3 warmup + {steps4} timed")
                 ms4, out4 = run_flat(c4, x4, steps4, 3, stream, layout)
                 per = ms4 / steps4

@@ -89,6 +89,7 @@ hipError_t launch_batch_p(rbd_batch* b, const Queue& q, int lanes, bool small, h
 hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hipStream_t s) {
   switch (b->ctx->precision) {
     case RBD_PREC_FP32: return launch_batch_p<rbd::PrecFP32>(b, q, lanes, small, s);
+    case RBD_PREC_FP32_HW_TRIG: return launch_batch_p<rbd::PrecFP32HwTrig>(b, q, lanes, small, s);
     case RBD_PREC_FP64_PIVOT: return launch_batch_p<rbd::PrecFP64Pivot>(b, q, lanes, small, s);
     case RBD_PREC_FP64_COLUMNS: return launch_batch_p<rbd::PrecFP64Col>(b, q, lanes, small, s);
     default: return launch_batch_p<rbd::PrecFP64>(b, q, lanes, small, s);
@@ -183,7 +184,8 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
   }
   // persistent grid: as many waves as the SIMDs (4 per CU) keep resident --
   // two per SIMD with FP32 columns, one with FP64 columns
-  const int waves = (ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT) ? 2 : 1;
+  const int waves = (ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT ||
+                     ctx->precision == RBD_PREC_FP32_HW_TRIG) ? 2 : 1;
   b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4 * waves);
   *out = b;
   return RBD_OK;

@@ -232,11 +232,14 @@ RBD_HD RBD_INLINE void transport(const CS<S>& t, const S Ia[6][6], S P[6][6]) {
 //   Rnea  : bias RNEA, qdd and the RNEA with qdd
 //   Col   : the derivative columns dtau/d(q,qd) and -Minv dtau
 // ---------------------------------------------------------------------------
-template <typename SMinv, typename SPivot, typename SRnea, typename SCol>
+//   HW_TRIG: FP32 sin/cos from the hardware's v_sin_f32/v_cos_f32 (knot_sincos)
+template <typename SMinv, typename SPivot, typename SRnea, typename SCol, bool HW_TRIG = false>
 struct Precision {
   using Minv = SMinv; using Pivot = SPivot; using Rnea = SRnea; using Col = SCol;
+  static constexpr bool kHwTrig = HW_TRIG;
 };
 using PrecFP32      = Precision<float, float, float, float>;     // all FP32
+using PrecFP32HwTrig = Precision<float, float, float, float, true>;  // all FP32, hardware sin/cos
 using PrecFP64Pivot = Precision<float, double, float, float>;    // FP64 pivots only
 using PrecFP64Col   = Precision<float, float, float, double>;    // FP64 derivative columns
 using PrecFP64      = Precision<double, double, double, double>; // all FP64
@@ -247,6 +250,32 @@ using PrecFP64      = Precision<double, double, double, double>; // all FP64
 // wave at 2^20 knots); with FP64 columns it needs about 500 and runs one.
 template <class P>
 constexpr int kWavesPerSimd = (sizeof(typename P::Col) == 4 && sizeof(typename P::Rnea) == 4) ? 2 : 1;
+
+// sin/cos of a joint angle under precision P: the math library's sincos, or,
+// with P::kHwTrig and an FP32 angle, the hardware's v_sin_f32/v_cos_f32 (whose
+// argument is in revolutions).  The angle is first reduced to [-pi, pi] as
+// x - k 2pi with 2pi split into two FP32 parts (exact to about 2^-40 |x|), so
+// small and large angles keep their precision.  Measured on MI355X: about 10%
+// less per-call latency, and the largest relative output error on random and
+// stretched-arm poses 3-4e-5 instead of 1-3e-5.  The host build uses the
+// math library's.
+template <class P, typename S>
+RBD_HD RBD_INLINE void knot_sincos(S x, S* s, S* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (P::kHwTrig && std::is_same<S, float>::value) {
+    constexpr double kTwoPi = 6.283185307179586476925286766559;
+    constexpr float kHi = float(kTwoPi);
+    constexpr float kLo = float(kTwoPi - double(kHi));
+    constexpr float kInv = float(1.0 / kTwoPi);
+    const float k = __builtin_rintf(x * kInv);
+    const float a = __builtin_fmaf(-k, kLo, __builtin_fmaf(-k, kHi, x));
+    *s = __builtin_amdgcn_sinf(a * kInv);
+    *c = __builtin_amdgcn_cosf(a * kInv);
+    return;
+  }
+#endif
+  sincos_s(x, s, c);
+}
 
 // ---------------------------------------------------------------------------
 // Minv = M^-1 (upper triangle, Mi[r][c] for r <= c) by the articulated-body
@@ -695,7 +724,7 @@ RBD_HD RBD_INLINE void knot_trig(const In& in, CS<KnotTrig<P>> tw[NQ]) {
   using ST = KnotTrig<P>;
   for (int k = 0; k < NQ; ++k) {
     ST s, c;
-    sincos_s(ST(in.q(k)), &s, &c);
+    knot_sincos<P>(ST(in.q(k)), &s, &c);
     tw[k] = CS<ST>{c, s};
   }
 }
@@ -770,7 +799,7 @@ RBD_HD RBD_INLINE void phase_a_given_qdd_minv(const In& in, const Options& opt, 
   SR qd[NQ], qdd[NQ];
   for (int k = 0; k < NQ; ++k) {
     ST s, c;
-    sincos_s(ST(in.q(k)), &s, &c);
+    knot_sincos<P>(ST(in.q(k)), &s, &c);
     tr[k] = CS<SR>{SR(c), SR(s)}; tc[k] = CS<SC>{SC(c), SC(s)};
     qd[k] = SR(in.qd(k));
     qdd[k] = SR(in.qdd(k));
@@ -832,7 +861,7 @@ RBD_HD RBD_INLINE void fd_knot(const In& in, Out& out, const Options& opt) {
   SR qd[NQ], u[NQ];
   for (int k = 0; k < NQ; ++k) {
     ST s, c;
-    sincos_s(ST(in.q(k)), &s, &c);
+    knot_sincos<P>(ST(in.q(k)), &s, &c);
     tm[k] = CS<SM>{SM(c), SM(s)}; tr[k] = CS<SR>{SR(c), SR(s)};
     qd[k] = SR(in.qd(k));
     u[k] = SR(in.u(k));
@@ -1016,7 +1045,7 @@ RBD_HD RBD_INLINE void two_wave_r1a(const In& in, const Options& opt, int role, 
   SR u[NQ];
   for (int k = 0; k < NQ; ++k) {
     ST s, c;
-    sincos_s(ST(in.q(k)), &s, &c);
+    knot_sincos<P>(ST(in.q(k)), &s, &c);
     tr[k] = CS<SR>{SR(c), SR(s)}; st.tc[k] = CS<SC>{SC(c), SC(s)};
     st.qd[k] = SR(in.qd(k));
     u[k] = SR(in.u(k));
@@ -1182,7 +1211,7 @@ RBD_HD RBD_INLINE void phase_split(const In& in, PhaseC<typename P::Col>& PC) {
   SC qd[NQ], v[NQ][6], a[NQ][6], f[NQ][6], Mi[NQ][NQ];
   for (int k = 0; k < NQ; ++k) {
     SC s, c;
-    sincos_s(SC(in.q(k)), &s, &c);
+    knot_sincos<P>(SC(in.q(k)), &s, &c);
     tc[k] = CS<SC>{c, s};
     qd[k] = SC(in.qd(k));
     for (int i = 0; i < 6; ++i) { v[k][i] = SC(in.v(k, i)); a[k][i] = SC(in.a(k, i)); f[k][i] = SC(in.f(k, i)); }
@@ -1215,7 +1244,7 @@ RBD_HD RBD_INLINE void vaf_knot(const In& in, Out& out, const Options& opt) {
   SR qd[NQ], qdd[NQ];
   for (int k = 0; k < NQ; ++k) {
     SR s, c;
-    sincos_s(SR(in.q(k)), &s, &c);
+    knot_sincos<P>(SR(in.q(k)), &s, &c);
     tr[k] = CS<SR>{c, s};
     qd[k] = SR(in.qd(k));
     qdd[k] = SR(in.qdd(k));

@@ -362,6 +362,7 @@ hipError_t launch_flat_p(const FlatCall& c) {
 hipError_t launch_flat(int precision, const FlatCall& c) {
   switch (precision) {
     case RBD_PREC_FP32: return launch_flat_p<rbd::PrecFP32>(c);
+    case RBD_PREC_FP32_HW_TRIG: return launch_flat_p<rbd::PrecFP32HwTrig>(c);
     case RBD_PREC_FP64_PIVOT: return launch_flat_p<rbd::PrecFP64Pivot>(c);
     case RBD_PREC_FP64_COLUMNS: return launch_flat_p<rbd::PrecFP64Col>(c);
     default: return launch_flat_p<rbd::PrecFP64>(c);
@@ -410,7 +411,7 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   *out = nullptr;
   rbd_config c;
   if (cfg) c = *cfg; else rbd_config_default(&c);
-  if (c.precision < RBD_PREC_FP32 || c.precision > RBD_PREC_FP64)
+  if (c.precision < RBD_PREC_FP32 || c.precision > RBD_PREC_FP32_HW_TRIG)
     return fail(RBD_ERR_INVALID_ARGUMENT, "unknown precision mode");
   if (c.flags & ~(RBD_FLAG_VEL_DAMPING | RBD_FLAG_NO_GRAVITY))
     return fail(RBD_ERR_INVALID_ARGUMENT, "unknown flag bits");
@@ -524,18 +525,22 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
 
 const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knots) {
   if (!ctx) return "";
-  static const char* names[2][3][4] = {
+  static const char* names[2][3][5] = {
       {{"fdgrad_aos_kernel<PrecFP32>", "fdgrad_aos_kernel<PrecFP64Pivot>", "fdgrad_aos_kernel<PrecFP64Col>",
-        "fdgrad_aos_kernel<PrecFP64>"},
+        "fdgrad_aos_kernel<PrecFP64>", "fdgrad_aos_kernel<PrecFP32HwTrig>"},
        {"fdgrad_aos_lanes_kernel<PrecFP32>", "fdgrad_aos_lanes_kernel<PrecFP64Pivot>",
-        "fdgrad_aos_lanes_kernel<PrecFP64Col>", "fdgrad_aos_lanes_kernel<PrecFP64>"},
+        "fdgrad_aos_lanes_kernel<PrecFP64Col>", "fdgrad_aos_lanes_kernel<PrecFP64>",
+        "fdgrad_aos_lanes_kernel<PrecFP32HwTrig>"},
        {"fdgrad_two_wave_kernel<PrecFP32>", "fdgrad_two_wave_kernel<PrecFP64Pivot>",
-        "fdgrad_two_wave_kernel<PrecFP64Col>", "fdgrad_two_wave_kernel<PrecFP64>"}},
+        "fdgrad_two_wave_kernel<PrecFP64Col>", "fdgrad_two_wave_kernel<PrecFP64>",
+        "fdgrad_two_wave_kernel<PrecFP32HwTrig>"}},
       {{"fdgrad_refmode_kernel<PrecFP32>", "fdgrad_refmode_kernel<PrecFP64Pivot>",
-        "fdgrad_refmode_kernel<PrecFP64Col>", "fdgrad_refmode_kernel<PrecFP64>"},
+        "fdgrad_refmode_kernel<PrecFP64Col>", "fdgrad_refmode_kernel<PrecFP64>",
+        "fdgrad_refmode_kernel<PrecFP32HwTrig>"},
        {"fdgrad_refmode_lanes_kernel<PrecFP32>", "fdgrad_refmode_lanes_kernel<PrecFP64Pivot>",
-        "fdgrad_refmode_lanes_kernel<PrecFP64Col>", "fdgrad_refmode_lanes_kernel<PrecFP64>"},
-       {"", "", "", ""}}};
+        "fdgrad_refmode_lanes_kernel<PrecFP64Col>", "fdgrad_refmode_lanes_kernel<PrecFP64>",
+        "fdgrad_refmode_lanes_kernel<PrecFP32HwTrig>"},
+       {"", "", "", "", ""}}};
   const int lanes = lanes_for(ctx, n_knots, !reference_mode);
   const int mapping = lanes == 1 ? 0 : (lanes == kTwoWaveLanes ? 2 : 1);
   return names[reference_mode ? 1 : 0][mapping][ctx->precision];
@@ -561,6 +566,7 @@ rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n, const float* d_
   };
   switch (ctx->precision) {
     case RBD_PREC_FP32: go(rbd::PrecFP32{}); break;
+    case RBD_PREC_FP32_HW_TRIG: go(rbd::PrecFP32HwTrig{}); break;
     case RBD_PREC_FP64_PIVOT: go(rbd::PrecFP64Pivot{}); break;
     case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP64Col{}); break;
     default: go(rbd::PrecFP64{}); break;
@@ -600,6 +606,7 @@ rbd_status forward_dynamics(rbd_context* ctx, int64_t n, const float* d_in, cons
   switch (ctx->precision) {
     case RBD_PREC_FP32:
     case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP32{}); break;
+    case RBD_PREC_FP32_HW_TRIG: go(rbd::PrecFP32HwTrig{}); break;
     case RBD_PREC_FP64_PIVOT: go(rbd::PrecFP64Pivot{}); break;
     default: go(rbd::PrecFP64{}); break;
   }
@@ -642,6 +649,7 @@ rbd_status rbd_fdgrad_traj_fext_f32(rbd_context* ctx, int64_t n, const float* d_
   };
   switch (ctx->precision) {
     case RBD_PREC_FP32: go(rbd::PrecFP32{}); break;
+    case RBD_PREC_FP32_HW_TRIG: go(rbd::PrecFP32HwTrig{}); break;
     case RBD_PREC_FP64_PIVOT: go(rbd::PrecFP64Pivot{}); break;
     case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP64Col{}); break;
     default: go(rbd::PrecFP64{}); break;
@@ -673,6 +681,7 @@ rbd_status id_gradient(rbd_context* ctx, int64_t n, float* d_cdu, const float* d
   };
   switch (ctx->precision) {
     case RBD_PREC_FP32: go(rbd::PrecFP32{}); break;
+    case RBD_PREC_FP32_HW_TRIG: go(rbd::PrecFP32HwTrig{}); break;
     case RBD_PREC_FP64_PIVOT: go(rbd::PrecFP64Pivot{}); break;   // no pivots here: same as FP32
     case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP64Col{}); break;
     default: go(rbd::PrecFP64{}); break;

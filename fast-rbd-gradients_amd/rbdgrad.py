@@ -34,8 +34,9 @@ PREC_FP32 = 0
 PREC_FP64_PIVOT = 1
 PREC_FP64_COLUMNS = 2
 PREC_FP64 = 3
+PREC_FP32_HW_TRIG = 4
 PRECISIONS = {"fp32": PREC_FP32, "fp64_pivot": PREC_FP64_PIVOT, "fp64_columns": PREC_FP64_COLUMNS,
-              "fp64": PREC_FP64}
+              "fp64": PREC_FP64, "fp32_hw_trig": PREC_FP32_HW_TRIG}
 
 # every function the header declares, for the export check
 EXPORTED = [

@@ -18,9 +18,10 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = np.load(os.path.join(common.GOLDEN, "fdgrad_golden.npz"))
 SETS = sorted({k.split("__")[0] for k in GOLDEN.files if not k.startswith("refmode")})
-PRECISIONS = ["fp32", "fp64_pivot", "fp64_columns", "fp64"]
+PRECISIONS = ["fp32", "fp64_pivot", "fp64_columns", "fp64", "fp32_hw_trig"]
 # fp64 keeps only the FP32 rounding of inputs/outputs
-BOUND = {"fp32": common.REL_TOL, "fp64_pivot": common.REL_TOL, "fp64_columns": 1e-5, "fp64": 1e-6}
+BOUND = {"fp32": common.REL_TOL, "fp64_pivot": common.REL_TOL, "fp64_columns": 1e-5, "fp64": 1e-6,
+         "fp32_hw_trig": common.REL_TOL}
 
 
 def say(*a):
@@ -184,9 +185,26 @@ def test_fp32_error_statistics_near_singular(contexts, oracle):
     x = common.knots(16384, seed=305, near_singular_frac=1.0)
     ref = oracle.fdgrad(x.astype(np.float64), threads=8)
     res = {}
-    for p in ("fp32", "fp64_pivot", "fp64_columns"):
+    for p in ("fp32", "fp64_pivot", "fp64_columns", "fp32_hw_trig"):
         e = common.block_errors(contexts[(0, p)].run_aos(x), ref)
         res[p] = {k: (float(v.max()), float(np.percentile(v, 99.9))) for k, v in e.items()}
         say(p, res[p])
     assert max(v[0] for v in res["fp64_columns"].values()) < 1e-5
     assert max(v[1] for v in res["fp32"].values()) < common.REL_TOL      # p99.9 holds in FP32
+    assert max(v[1] for v in res["fp32_hw_trig"].values()) < common.REL_TOL
+
+
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_hardware_trig_mode_holds_the_bar(contexts, oracle, flags):
+    """RBD_PREC_FP32_HW_TRIG (hardware sin/cos) against the FP64 oracle on
+    random poses, stretched-arm poses and angles far outside [-pi, pi]."""
+    x = np.concatenate([common.knots(8192, seed=310 + flags), common.knots(8192, seed=320 + flags, near_singular_frac=1.0)])
+    big = common.knots(2048, seed=330 + flags)
+    big[:, :7] += np.float32(2 * np.pi) * np.random.default_rng(331).integers(-50, 50, (2048, 7)).astype(np.float32)
+    x = np.ascontiguousarray(np.concatenate([x, big]), dtype=np.float32)
+    ref = oracle.fdgrad(x.astype(np.float64), flags, threads=8)
+    for lanes in (1, 16, 32):
+        e = common.block_errors(contexts[(flags, "fp32_hw_trig", lanes)].run_aos(x), ref)
+        err = {k: float(v.max()) for k, v in e.items()}
+        say(f"hw trig flags={flags} lanes={lanes}", err)
+        assert max(err.values()) < common.REL_TOL, err
