@@ -729,6 +729,24 @@ RBD_HD RBD_INLINE void knot_trig(const In& in, CS<KnotTrig<P>> tw[NQ]) {
   }
 }
 
+// The same for a knot's lane group: lane role j < 7 computes the sin/cos of
+// q_j alone and the group's lanes read all seven from each other by
+// shuffles, instead of every lane computing all seven (a sincos is about 100
+// instructions).  The values are those of knot_trig; the host build, which
+// runs the lane roles one after another, calls knot_trig.
+template <class P, class In>
+RBD_HD RBD_INLINE void knot_trig_group(const In& in, int role, CS<KnotTrig<P>> tw[NQ]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  using ST = KnotTrig<P>;
+  ST s, c;
+  knot_sincos<P>(ST(in.q(role < NQ ? role : 0)), &s, &c);
+  for (int k = 0; k < NQ; ++k) tw[k] = CS<ST>{__shfl(c, k, kLanesPerKnotGroup), __shfl(s, k, kLanesPerKnotGroup)};
+#else
+  (void)role;
+  knot_trig<P>(in, tw);
+#endif
+}
+
 // Steps 2-5 of the full path, given sin/cos and Minv: bias RNEA, qdd = Minv
 // (u - c), RNEA with qdd; fills the phase-C inputs and qdd.
 template <class P, class In>
@@ -905,7 +923,7 @@ RBD_HD RBD_INLINE void wave_lds_sync() {
 template <class P, class In>
 RBD_HD RBD_INLINE void lanes_minv_stage(const In& in, int role, CS<KnotTrig<P>> tw[NQ], typename P::Minv* mi) {
   using SM = typename P::Minv; using SP = typename P::Pivot;
-  knot_trig<P>(in, tw);
+  knot_trig_group<P>(in, role, tw);
   CS<SM> tm[NQ];
   for (int k = 0; k < NQ; ++k) tm[k] = CS<SM>{SM(tw[k].c), SM(tw[k].s)};
   const int j = role < NQ ? role : NQ - 1;
@@ -1040,13 +1058,12 @@ template <class P, class In>
 RBD_HD RBD_INLINE void two_wave_r1a(const In& in, const Options& opt, int role, TwoWaveShared<P>& sh,
                                     TwoWaveR<P>& st) {
   using SR = typename P::Rnea; using SC = typename P::Col;
-  using ST = Wider<SR, SC>;
+  CS<KnotTrig<P>> tw[NQ];
+  knot_trig_group<P>(in, role, tw);
   CS<SR> tr[NQ];
   SR u[NQ];
   for (int k = 0; k < NQ; ++k) {
-    ST s, c;
-    knot_sincos<P>(ST(in.q(k)), &s, &c);
-    tr[k] = CS<SR>{SR(c), SR(s)}; st.tc[k] = CS<SC>{SC(c), SC(s)};
+    tr[k] = CS<SR>{SR(tw[k].c), SR(tw[k].s)}; st.tc[k] = CS<SC>{SC(tw[k].c), SC(tw[k].s)};
     st.qd[k] = SR(in.qd(k));
     u[k] = SR(in.u(k));
   }

@@ -79,11 +79,12 @@ def test_lane_group_matches_one_lane(contexts, prec):
     a different instruction schedule) and every output value is written."""
     x = common.knots(300, seed=306)
     a = contexts[(0, prec, 1)].run_aos(x)
+    bound = 2e-5 if prec == "fp32_hw_trig" else 1e-5     # its larger errors differ more between schedules
     for lanes in (16, 32):
         b = contexts[(0, prec, lanes)].run_aos(x)
         assert np.all(np.isfinite(b))
         err = common.max_block_errors(b, a.astype(np.float64))
-        assert max(err.values()) < 1e-5, (lanes, err)
+        assert max(err.values()) < bound, (lanes, err)
 
 
 def test_autotune_sets_a_crossover(rbd):
