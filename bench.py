@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
 # profiles/fdgrad_per_call_n128_and_large_batch_2p20.md.  Keyed by (kernel,
 # knots per launch); other configurations report null.
 MEASURED_TRAFFIC_BYTES = {
-    ("fdgrad_two_wave_kernel<PrecFP32>", 128): int((250.5 * 2.000 + 53.0 * 1.000) * 1024),
+    ("fdgrad_two_wave_kernel<PrecFP32>", 128): int((181.0 * 2.000 + 53.0 * 1.000) * 1024),
     ("fdgrad_soa_kernel<PrecFP32>", 1 << 20): int((41188.1 * 2.000 + 430080.0) * 1024),
     ("fdgrad_aos_kernel<PrecFP32>", 1 << 20): int((61694.9 * 2.000 + 1066783.0) * 1024),
 }
