@@ -127,6 +127,9 @@ struct Slot {
 
 constexpr size_t kTraceRecords = 65536;
 constexpr int kWarmRounds = 512;       // warm-up batches per slot at creation
+// zero_copy = 2: batches of more knots than this are copied (the kernels' PCIe
+// traffic caps zero-copy batches at about 3e7 knots/s; copied ones reach 1e8)
+constexpr int64_t kZeroCopyMaxKnots = 4096;
 
 }  // namespace rbdi
 
@@ -134,12 +137,18 @@ struct rbd_server {
   rbd_context* ctx = nullptr;          // the caller's context (device, flags, precision)
   rbd_context* run_ctx = nullptr;      // same, with the knot-to-lane mapping fixed
   rbd_server_config cfg{};
-  char* d_pool = nullptr;              // device memory of the budget (copy mode)
+  char* d_pool = nullptr;              // device memory of the budget (copied batches)
   char* h_pool = nullptr;              // pinned staging of the same size
-  char* dh_pool = nullptr;             // device address of h_pool (zero-copy mode)
+  char* dh_pool = nullptr;             // device address of h_pool (zero-copy batches)
   // where the kernels find a batch's queue, inputs and outputs: the device
   // pool (copy mode) or the pinned pool itself (zero-copy mode)
-  char* work_pool() const { return cfg.zero_copy ? dh_pool : d_pool; }
+  char* work_pool(bool copy) const { return copy ? d_pool : dh_pool; }
+  // whether a batch of this many knots goes through device memory (copies) or
+  // is read and written in the pinned pool (zero-copy)
+  bool copied(int64_t knots) const {
+    return cfg.zero_copy == 0 || (cfg.zero_copy == 2 && knots > zero_copy_max_knots);
+  }
+  int64_t zero_copy_max_knots = 0;
   std::unique_ptr<RangeAllocator> alloc;
   std::vector<Slot> slots;
 
@@ -265,8 +274,8 @@ void dispatch_loop(rbd_server* s) {
     lk.unlock();
     // stage descriptors and inputs; copy mode: one H2D copy, the launch, one
     // D2H copy; zero-copy mode: the launch alone; then the event
-    char* const work = s->work_pool();
-    const bool copy = !s->cfg.zero_copy;
+    const bool copy = s->copied(knots);
+    char* const work = s->work_pool(copy);
     std::vector<rbd_traj> trajs;
     trajs.reserve(sl.reqs.size());
     for (auto& r : sl.reqs) {
@@ -355,10 +364,8 @@ rbd_status warm_up(rbd_server* s) {
   // touch the whole staging and device pools once, copies both ways included,
   // so no page of either is first used by a request
   const size_t pool = s->alloc->size();
-  const bool copy = !s->cfg.zero_copy;
-  char* const work = s->work_pool();
   std::memset(s->h_pool, 0, pool);
-  if (copy) {
+  if (s->d_pool) {
     hipError_t e0 = hipMemcpy(s->d_pool, s->h_pool, pool, hipMemcpyHostToDevice);
     if (e0 == hipSuccess) e0 = hipMemcpy(s->h_pool, s->d_pool, pool, hipMemcpyDeviceToHost);
     if (e0 != hipSuccess) return hip_fail(e0, "server warm-up copies");
@@ -376,12 +383,15 @@ rbd_status warm_up(rbd_server* s) {
     for (size_t si = 0; si < s->slots.size(); ++si) {
       Slot& sl = s->slots[si];
       const size_t off = si * block;
+      const int n = (round & 1) ? n_big : 1;
+      // both paths of a server that copies large batches only
+      const bool copy = s->cfg.zero_copy == 2 ? (round & 2) != 0 : s->copied(n);
+      char* const work = s->work_pool(copy);
       std::vector<rbd_traj> trajs(static_cast<size_t>(n_big));
       for (int t = 0; t < n_big; ++t)
         trajs[size_t(t)] =
             rbd_traj{reinterpret_cast<const float*>(work + off + stage) + size_t(t) * rbd::kInPerKnot,
                      reinterpret_cast<float*>(work + off + stage + in_sz) + size_t(t) * rbd::kOutPerKnot, 1};
-      const int n = (round & 1) ? n_big : 1;
       hipError_t e = hipEventSynchronize(sl.done);        // this slot's previous round is done with its block
       const int64_t chunks = batch_stage(s->h_pool + off, trajs.data(), n, lanes);
       if (copy && e == hipSuccess)
@@ -424,6 +434,7 @@ void rbd_server_config_default(rbd_server_config* cfg) {
   cfg->max_batch_knots = 1 << 15;
   cfg->batch_window_us = 0.0;
   cfg->use_graph = 0;
+  cfg->zero_copy = 2;                         // zero-copy for batches up to 4096 knots, copies above
   cfg->lanes_per_knot = rbd::kLanesPerKnotGroup;
 }
 
@@ -440,6 +451,8 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
   if (c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup)
     return fail(RBD_ERR_INVALID_ARGUMENT, "server lanes_per_knot must be 1 or 16");
   if (!(c.batch_window_us >= 0)) return fail(RBD_ERR_INVALID_ARGUMENT, "negative batch window");
+  if (c.zero_copy < 0 || c.zero_copy > 2) return fail(RBD_ERR_INVALID_ARGUMENT, "zero_copy must be 0, 1 or 2");
+  if (c.zero_copy_max_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative zero_copy_max_knots");
   if (rbd_status st = use_device(ctx)) return st;
   auto* s = new (std::nothrow) rbd_server();
   if (!s) return fail(RBD_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -455,9 +468,10 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
   rbd_status st = rbd_create(&rc, &s->run_ctx);
   const size_t budget = round_up(size_t(c.memory_budget_bytes));
   hipError_t e = hipSuccess;
-  if (st == RBD_OK && !c.zero_copy) e = hipMalloc(&s->d_pool, budget);
-  // zero-copy: the kernels read and write the pinned pool directly, so it is
-  // mapped and fine-grained (coherent with the host without cache flushes)
+  s->zero_copy_max_knots = c.zero_copy_max_knots > 0 ? c.zero_copy_max_knots : kZeroCopyMaxKnots;
+  if (st == RBD_OK && c.zero_copy != 1) e = hipMalloc(&s->d_pool, budget);
+  // zero-copy batches: the kernels read and write the pinned pool directly, so
+  // it is mapped and fine-grained (coherent with the host without cache flushes)
   if (st == RBD_OK && e == hipSuccess)
     e = hipHostMalloc(reinterpret_cast<void**>(&s->h_pool), budget,
                       c.zero_copy ? (hipHostMallocMapped | hipHostMallocCoherent) : hipHostMallocDefault);

@@ -73,7 +73,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("lanes_per_knot", ctypes.c_int), ("memory_budget_bytes", ctypes.c_int64),
                 ("max_batch_trajectories", ctypes.c_int64), ("max_batch_knots", ctypes.c_int64),
                 ("batch_window_us", ctypes.c_double), ("use_graph", ctypes.c_int), ("zero_copy", ctypes.c_int),
-                ("poll_dispatch", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("poll_dispatch", ctypes.c_int), ("zero_copy_max_knots", ctypes.c_int)]
 
 
 class BatchRecord(ctypes.Structure):

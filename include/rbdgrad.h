@@ -313,10 +313,13 @@ typedef struct rbd_server_config {
   int use_graph;                  /* ignored: each batch's queue is uploaded with its inputs and launched directly */
   int zero_copy;                  /* 0: requests' buffers in device memory, one upload and one download per batch;
                                      1: the kernels read inputs from and write results to the pinned staging
-                                     directly (no copies: lower latency, PCIe traffic from the kernels) */
+                                     directly (no copies: lower latency, PCIe traffic from the kernels, which caps
+                                     the server at about 3e7 knots/s);
+                                     2 (the default): per batch -- zero-copy up to zero_copy_max_knots knots,
+                                     copies above */
   int poll_dispatch;              /* 1: the dispatcher thread polls for requests instead of sleeping (one busy
                                      core; saves the ~10 us wake-up of a sleeping thread per batch) */
-  int reserved[1];
+  int zero_copy_max_knots;        /* zero_copy = 2: the largest zero-copy batch (0: 4096 knots) */
 } rbd_server_config;
 
 typedef struct rbd_server_stats {

@@ -19,10 +19,13 @@ def ctx(rbd):
     c.close()
 
 
-@pytest.mark.parametrize("zero_copy,poll", [(0, 0), (1, 0), (1, 1)])
+@pytest.mark.parametrize("zero_copy,poll", [(0, 0), (1, 0), (1, 1), (2, 1)])
 @pytest.mark.parametrize("lanes", [1, 16])
 def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_copy, poll):
-    srv = rbd.Server(ctx, lanes_per_knot=lanes, slots=4, zero_copy=zero_copy, poll_dispatch=poll)
+    # zero_copy = 2 with a 300-knot limit: the probe alone is a zero-copy batch,
+    # in the mix most batches are copied
+    srv = rbd.Server(ctx, lanes_per_knot=lanes, slots=4, zero_copy=zero_copy, poll_dispatch=poll,
+                     zero_copy_max_knots=300)
     rng = np.random.default_rng(21)
     probe = common.knots(128, seed=500)
     alone = srv.wait(srv.submit(probe))
@@ -48,7 +51,7 @@ def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_c
     srv.close()
 
 
-@pytest.mark.parametrize("zero_copy", [0, 1])
+@pytest.mark.parametrize("zero_copy", [0, 1, 2])
 def test_memory_budget_is_respected(rbd, ctx, zero_copy):
     """A budget that holds only a few requests: everything still completes, in
     several batches, and the pool never exceeds the budget."""
@@ -78,7 +81,7 @@ def test_request_larger_than_budget_is_rejected(rbd, ctx):
 
 
 @pytest.mark.parametrize("poll", [0, 1])
-@pytest.mark.parametrize("zero_copy", [0, 1])
+@pytest.mark.parametrize("zero_copy", [0, 1, 2])
 @pytest.mark.parametrize("slots", [1, 4])
 def test_poisson_load(rbd, ctx, slots, zero_copy, poll):
     srv = rbd.Server(ctx, slots=slots, zero_copy=zero_copy, poll_dispatch=poll)
@@ -92,7 +95,7 @@ def test_poisson_load(rbd, ctx, slots, zero_copy, poll):
     srv.close()
 
 
-@pytest.mark.parametrize("zero_copy", [0, 1])
+@pytest.mark.parametrize("zero_copy", [0, 1, 2])
 def test_batches_overlap_on_the_device(rbd, ctx, zero_copy):
     """Requests arriving while batches are in flight are admitted into free
     slots: under a 100k req/s load several batches are on the device at once."""
@@ -110,3 +113,9 @@ def test_batches_overlap_on_the_device(rbd, ctx, zero_copy):
     assert np.all(tr["delivered_us"] >= tr["done_us"])
     assert int(tr["requests"].sum()) == st["requests_completed"]
     srv.close()
+
+
+def test_server_config_errors(rbd, ctx):
+    for bad in ({"zero_copy": 3}, {"zero_copy": -1}, {"zero_copy_max_knots": -5}):
+        with pytest.raises(rbd.RbdError):
+            rbd.Server(ctx, **bad)
