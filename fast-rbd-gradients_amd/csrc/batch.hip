@@ -54,6 +54,7 @@ struct Queue {
   const rbd::TrajDesc* descs;
   long* prefix;
   const rbd::ArgBatch* args = nullptr;   // a small batch in the kernel arguments instead
+  int64_t chunks = 0;                     // the batch's chunk total when known: no more waves than chunks
 };
 
 // the batch's own header and descriptors, uploaded per launch
@@ -61,7 +62,11 @@ Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, b->d_prefix}; 
 
 template <class P, int LANES, bool SMALL>
 void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
-  const unsigned grid = b->grid[LANES == 1 ? 0 : 1];
+  // a batch of fewer chunks than the persistent grid launches one wave per
+  // chunk (fewer workgroups to dispatch); graphs keep the full grid, whose
+  // captured launch serves every batch
+  unsigned grid = b->grid[LANES == 1 ? 0 : 1];
+  if (q.chunks > 0 && q.chunks < int64_t(grid)) grid = unsigned(q.chunks);
   if constexpr (SMALL) {
     static const rbd::ArgBatch kNone{};
     const rbd::ArgBatch& a = q.args ? *q.args : kNone;
@@ -268,6 +273,7 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
     a.total_chunks = int(chunks);
     Queue q{nullptr, nullptr, b->d_prefix};
     q.args = &a;
+    q.chunks = chunks;
     hipError_t le = launch_batch(b, q, lanes, true, st);
     if (le != hipSuccess) return hip_fail(le, "batch kernel launch");
     return RBD_OK;
@@ -277,7 +283,7 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   b->ring = (r + 1) % rbd_batch::kRing;
   hipError_t e = hipEventSynchronize(b->staged[r]);     // previous copy out of this slot is done
   if (e != hipSuccess) return hip_fail(e, "staging wait");
-  stage(b->h_stage[r], trajs, n_traj, lanes);
+  const int64_t chunks = stage(b->h_stage[r], trajs, n_traj, lanes);
   e = hipMemcpyAsync(b->d_mem, b->h_stage[r], batch_stage_bytes(n_traj), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipEventRecord(b->staged[r], st);
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
@@ -286,7 +292,9 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
     e = hipGraphLaunch(b->exec[graph_index(lanes, small)], st);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   } else {
-    e = launch_batch(b, own_queue(b), lanes, small, st);
+    Queue q = own_queue(b);
+    q.chunks = chunks;
+    e = launch_batch(b, q, lanes, small, st);
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   }
   return RBD_OK;
@@ -308,6 +316,7 @@ rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, const r
   const char* base = static_cast<const char*>(d_staged);
   Queue q{reinterpret_cast<const rbd::BatchHeader*>(base),
           reinterpret_cast<const rbd::TrajDesc*>(base + kHeaderBytes), b->d_prefix};
+  q.chunks = total_chunks;
   const bool small = n_traj <= rbd::kSmallBatch;
   rbd::ArgBatch a{};
   if (small && trajs) {              // the descriptors in the kernel arguments: no reads of the staged block
