@@ -277,6 +277,101 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000, 300000), durat
     return res
 
 
+def reference_mode_workload(ctx, stream, steps, warmup):
+    """The reference's own timed configuration (dynamicsGradientKernel,
+    time_GPU.cu:279-316): q, qd, u, qdd and the dense Minv in (77 floats per
+    knot, qdd and Minv from forward dynamics, outside the timed region), the
+    98 derivative values out; one launch of 128 knots per call."""
+    n = 128
+    x = knots(n, seed=101)
+    qdd, minv = ctx.forward_dynamics(x)
+    vol = np.ascontiguousarray(np.concatenate([x, qdd, minv.transpose(0, 2, 1).reshape(n, 49)], axis=1),
+                               dtype=np.float32)                  # Minv column-major, as the reference packs it
+    d_vol = rbdgrad.DeviceBuffer.from_array(vol)
+    d_out = rbdgrad.DeviceBuffer(n * rbdgrad.DQDD_PER_KNOT * 4)
+    ms = time_launches(lambda: ctx.dynamics_gradient_kernel(n, d_out, d_vol, stream=stream), steps, warmup, stream)
+    out = d_out.download((n, rbdgrad.DQDD_PER_KNOT))
+    per = ms / steps
+    r = {"kernel": ctx.kernel_name(n, reference_mode=True), "us_per_launch": per * 1e3,
+         "knots_per_s": n / (per * 1e-3), "bytes_per_knot": 700,
+         "hbm_gbs": n * 700 / (per * 1e-3) / 1e9}
+    CHECKS.add(r, "accuracy", out, vol)
+    d_vol.free()
+    d_out.free()
+    log(f"[c1] reference-compatible mode: {r}")
+    return {"per_call_n128_reference_mode": r}, vol
+
+
+N_SET = (1, 10, 16, 32, 64, 128, 256)   # trajectory lengths of the reference's harness (time_CPU.cpp:149-156)
+
+
+def n_sweep(ctx, stream, steps, warmup):
+    """Per-call latency over the reference harness's trajectory lengths (GPU;
+    the CPU leg adds the same lengths on the CPU)."""
+    res = {}
+    for n in N_SET:
+        x = knots(n, seed=110 + n)
+        ms, _ = run_flat(ctx, x, steps, warmup, stream)
+        res[str(n)] = {"kernel": ctx.kernel_name(n), "us_per_launch": ms / steps * 1e3,
+                       "knots_per_s": n * steps / (ms * 1e-3)}
+    log(f"[sweep] per call over N: { {k: round(v['us_per_launch'], 2) for k, v in res.items()} }")
+    return res
+
+
+def copy_bandwidth(stream, nbytes=(1 << 20) * 420, reps=20):
+    """Achievable HBM bandwidth: a device-to-device copy of nbytes (read + write counted)."""
+    a, b = rbdgrad.DeviceBuffer(nbytes), rbdgrad.DeviceBuffer(nbytes)
+    a.fill_zero()
+    H = rbdgrad.hip()
+
+    def launch():
+        rbdgrad._hipcheck(H.hipMemcpyAsync(b.handle, a.handle, nbytes, rbdgrad.D2D, stream.handle), "hipMemcpyAsync")
+    ms = time_launches(launch, reps, 3, stream) / reps
+    a.free()
+    b.free()
+    gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+    log(f"[copy] {nbytes} bytes device to device: {ms * 1e3:.1f} us, {gbs:.0f} GB/s (read + write)")
+    return {"bytes": nbytes, "us": ms * 1e3, "gbs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS}
+
+
+def urdf_model_workload(stream, steps, warmup, name="arm7",
+                        urdf=os.path.join(ROOT, "tests", "golden", "synthetic_arm7.urdf")):
+    """A robot read from a URDF: librbdgrad_arm7.so, compiled on the tables that
+    urdf_model.py generates from the synthetic 7-joint arm (joint frames at
+    general angles, so the transforms have no structural zeros).  The same
+    per-call and 2^20-knot SoA workloads as for the iiwa."""
+    if not os.path.exists(rbdgrad.lib_path(name)):
+        log(f"[model] {rbdgrad.lib_path(name)} not built: skipped")
+        return {}
+    import urdf_model
+    tab = urdf_model.tables(urdf_model.read_urdf(urdf))
+    tables = (np.array(tab["links"]),
+              np.concatenate([np.array(tab["rot"]).reshape(7, 9), np.array(tab["offset"])], axis=1))
+    ctx = rbdgrad.Context(device=0, model=name)
+    ctx.autotune(stream)
+    res = {}
+    x1 = knots(128, seed=101)
+    ms, out = run_flat(ctx, x1, steps, warmup, stream)
+    r = res[f"urdf_model_{name}_per_call_n128"] = {
+        "kernel": ctx.kernel_name(128), "us_per_launch": ms / steps * 1e3, "knots_per_s": 128 * steps / (ms * 1e-3)}
+    CHECKS.add(r, "accuracy", out, x1, model=tables)
+    log(f"[model] {name} per call: {r}")
+    n4 = 1 << 20
+    x4 = knots(n4, seed=104)
+    ms, out = run_flat(ctx, x4, 20, 3, stream, "soa")
+    per = ms / 20
+    gbs, _ = roofline(n4, per)
+    tfl = n4 * FLOPS_PER_KNOT_ARM7 / (per * 1e-3) / 1e12
+    r = res[f"urdf_model_{name}_2p20_fp32_soa"] = {
+        "knots_per_launch": n4, "ms_per_launch": per, "knots_per_s": n4 / (per * 1e-3), "hbm_gbs": gbs,
+        "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_tflops": tfl, "fp32_frac": tfl / FP32_PEAK_TFLOPS,
+        "flops_per_knot": FLOPS_PER_KNOT_ARM7}
+    CHECKS.add(r, "accuracy", out, x4, sample=8192, model=tables)
+    log(f"[model] {name} 2^20 SoA: {r}")
+    ctx.close()
+    return res
+
+
 def batch_phases(tr):
     """Where a served batch's time goes, from the server's per-batch trace:
     queued (oldest request's submit -> admission), enqueue (admission -> all

@@ -3,7 +3,9 @@ metric, and independent textbook spatial-algebra models used to pin the
 oracle: of the iiwa, built from the URDF data in tests/golden/iiwa_14_urdf.json,
 and of any URDF tree (UrdfTree), read straight from the XML.
 """
+import contextlib
 import ctypes
+import fcntl
 import json
 import os
 import subprocess
@@ -317,17 +319,31 @@ def model_tables(urdf):
     return np.array(tab["links"]), joints
 
 
+@contextlib.contextmanager
+def _build_lock():
+    """One test process at a time writes generated headers and test libraries
+    (pytest-xdist workers would otherwise race on the same files)."""
+    with open(os.path.join(HOSTCHECK, ".build.lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        try:
+            yield
+        finally:
+            fcntl.flock(lk, fcntl.LOCK_UN)
+
+
 def generate_model(urdf, name):
     """Write fast-rbd-gradients_amd/csrc/model_NAME.hpp; returns model_tables(urdf)."""
     import urdf_model
-    urdf_model.main([urdf, name])
+    with _build_lock():
+        urdf_model.main([urdf, name])
     return model_tables(urdf)
 
 
 def hostcheck_lib(model=None):
     """Build and load tests/hostcheck's libknot_host[_MODEL].so."""
     args = ["make", "-s", "-C", HOSTCHECK] + ([f"MODEL={model}"] if model else [])
-    subprocess.run(args, check=True)
+    with _build_lock():
+        subprocess.run(args, check=True)
     L = ctypes.CDLL(os.path.join(HOSTCHECK, f"libknot_host_{model}.so" if model else "libknot_host.so"))
     fp = ctypes.POINTER(ctypes.c_float)
     dp = ctypes.POINTER(ctypes.c_double)
