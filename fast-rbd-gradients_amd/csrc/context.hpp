@@ -17,9 +17,9 @@ constexpr int kBlock = 64;   // one wave per workgroup
 
 // Up to this many knots a call uses the 16-lane-group mapping (when the
 // context's lanes_per_knot is 0): the crossover measured on MI355X by
-// rbd_autotune (one-lane 22.8 us vs lane-group 12.5 us at 8192 knots,
-// 23.3 vs 22.0 us at 16384, 24.9 vs 39.2 us at 32768).
-constexpr int64_t kDefaultGroupThreshold = 16384;
+// rbd_autotune (one-lane 15.5 us vs lane-group 10.7 us at 8192 knots,
+// 16.0 vs 18.4 us at 16384, 17.6 vs 32.6 us at 32768).
+constexpr int64_t kDefaultGroupThreshold = 8192;
 // The two-wave mapping (Minv and the RNEA + columns of a knot on two waves)
 // serves calls up to the context's two_wave_threshold knots: by default
 // 16 x compute units (4,096 knots on 256 CUs, the crossover rbd_autotune
@@ -27,8 +27,9 @@ constexpr int64_t kDefaultGroupThreshold = 16384;
 // knots, 18.4 against 12.6 us at 8,192); rbd_autotune measures it.
 constexpr int kTwoWaveLanes = 32;   // lanes_per_knot value of the two-wave mapping (16 lanes in each of 2 waves)
 // The same crossover for batches (rbd_batch_*, total knots of the batch),
-// measured by rbd_autotune_batch on batches of 132-knot trajectories.
-constexpr int64_t kDefaultBatchGroupThreshold = 16384;
+// measured by rbd_autotune_batch on batches of 132-knot trajectories
+// (7,957 knots on MI355X: one-lane 17.0 us vs lane-group 18.4 us at 8,448).
+constexpr int64_t kDefaultBatchGroupThreshold = 8192;
 // Batches of up to this many trajectories find a chunk's trajectory inside the
 // wave, larger ones through the plan kernel (rbd::kSmallBatch, batch_kernels.hpp).
 constexpr int kSmallBatchTrajectories = 64;

@@ -659,21 +659,21 @@ template <bool MINV = true, typename SC, class Out>
 RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out& out) {
   // d/dq column 0 is identically zero (rotation about the gravity axis)
   for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
-#pragma clang loop unroll(disable)
-  for (int c = 0; c < NQ; ++c) {
+  static_for<0, NQ>([&](auto Cc) {
+    constexpr int c = decltype(Cc)::value;
     SC tau[NQ], col[NQ];
     dtau_column<false>(P, c, tau);
     add_on_diagonal(tau, c, SC(damping));
     if constexpr (MINV) minv_times<true>(P.Mi, tau, col);
     else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
     for (int r = 0; r < NQ; ++r) out.dqd(r, c, float(col[r]));
-    if (c > 0) {
+    if constexpr (c > 0) {
       dtau_column<true>(P, c, tau);
       if constexpr (MINV) minv_times<true>(P.Mi, tau, col);
       else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
       for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
     }
-  }
+  });
 }
 
 // Lane roles when one knot is spread over a group of kLanesPerKnotGroup lanes:
