@@ -55,6 +55,7 @@ struct Queue {
   long* prefix;
   const rbd::ArgBatch* args = nullptr;   // a small batch in the kernel arguments instead
   int64_t chunks = 0;                     // the batch's chunk total when known: no more waves than chunks
+  rbd::Completion done{nullptr, 0};       // completion word, if the caller polls one
 };
 
 // the batch's own header and descriptors, uploaded per launch
@@ -71,10 +72,10 @@ void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
     static const rbd::ArgBatch kNone{};
     const rbd::ArgBatch& a = q.args ? *q.args : kNone;
     hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, true>), dim3(grid), dim3(kBlock), 0, s,
-                       q.args ? nullptr : q.descs, q.hdr, a, b->d_tail, q.prefix, b->ctx->opt);
+                       q.args ? nullptr : q.descs, q.hdr, a, b->d_tail, q.prefix, b->ctx->opt, q.done);
   } else {
     hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, false>), dim3(grid), dim3(kBlock), 0, s, q.descs,
-                       q.hdr, rbd::NoArgBatch{}, b->d_tail, q.prefix, b->ctx->opt);
+                       q.hdr, rbd::NoArgBatch{}, b->d_tail, q.prefix, b->ctx->opt, q.done);
   }
 }
 
@@ -170,7 +171,7 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
   b->d_hdr = reinterpret_cast<rbd::BatchHeader*>(b->d_mem);
   b->d_desc = reinterpret_cast<rbd::TrajDesc*>(b->d_mem + hdr);
   b->d_prefix = reinterpret_cast<long*>(b->d_mem + hdr + desc);
-  const size_t tail_bytes = sizeof(unsigned long long) * rbd::kTailCounters * rbd::kTailStride;
+  const size_t tail_bytes = sizeof(unsigned long long) * rbd::kTailWords;
   e = hipMalloc(&b->d_tail, tail_bytes);
   if (e == hipSuccess) e = hipMemset(b->d_tail, 0, tail_bytes);   // once, outside any graph; kernels keep it zero
   if (e != hipSuccess) {
@@ -307,7 +308,8 @@ int64_t rbdi::batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int
 }
 
 rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, const rbd_traj* trajs, int n_traj,
-                                     int64_t total_knots, int64_t total_chunks, int lanes, rbd_stream_t stream) {
+                                     int64_t total_knots, int64_t total_chunks, int lanes, rbd_stream_t stream,
+                                     unsigned long long* done_flag, unsigned long long done_seq) {
   if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
   if (total_chunks > kMaxBatchChunks) return fail(RBD_ERR_CAPACITY, "batch too large (more than 2^31 chunks)");
   b->last_total_knots = total_knots;
@@ -317,6 +319,7 @@ rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, const r
   Queue q{reinterpret_cast<const rbd::BatchHeader*>(base),
           reinterpret_cast<const rbd::TrajDesc*>(base + kHeaderBytes), b->d_prefix};
   q.chunks = total_chunks;
+  q.done = rbd::Completion{done_flag, done_seq};
   const bool small = n_traj <= rbd::kSmallBatch;
   rbd::ArgBatch a{};
   if (small && trajs) {              // the descriptors in the kernel arguments: no reads of the staged block

@@ -52,6 +52,17 @@ struct BatchHeader {
 // counters of a launch's last partial round
 constexpr int kTailCounters = 64;
 constexpr int kTailStride = 16;   // 64-bit words between counters: one 128-byte line each
+// after them, on its own line: the count of waves that finished (completion word)
+constexpr int kDoneCounter = kTailCounters * kTailStride;
+constexpr int kTailWords = kDoneCounter + kTailStride;
+
+// Completion word for a host that polls memory instead of an event: with flag
+// set (host-mapped, coherent memory), the launch's last wave to finish writes
+// seq to it once every wave's stores are visible to the system.
+struct Completion {
+  unsigned long long* flag;
+  unsigned long long seq;
+};
 
 constexpr int kSmallBatch = 64;   // trajectories handled without the plan kernel
 
@@ -98,14 +109,15 @@ __global__ __launch_bounds__(kPlanThreads) void batch_plan_kernel(const TrajDesc
 
 // Persistent drain of the queue.  LANES = 1: lane i of a wave takes knot
 // k0 + i of a 64-knot chunk; LANES = 16: lane group g takes knot k0 + g of a
-// 4-knot chunk.  One wave per workgroup.  tail: kTailCounters counters,
-// kTailStride words apart, zero when the launch starts (and left at zero).
-// SMALL batches may come in the arguments (descs == nullptr: arg holds them).
+// 4-knot chunk.  One wave per workgroup.  tail: kTailWords words (the
+// kTailCounters counters kTailStride words apart, then the finished-wave
+// count), zero when the launch starts (and left at zero).  SMALL batches may
+// come in the arguments (descs == nullptr: arg holds them).
 template <class P, int LANES, bool SMALL>
 __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
     const TrajDesc* __restrict__ descs, const BatchHeader* __restrict__ hdr,
     const typename std::conditional<SMALL, ArgBatch, NoArgBatch>::type arg, unsigned long long* __restrict__ tail,
-    const long* __restrict__ prefix, Options opt) {
+    const long* __restrict__ prefix, Options opt, Completion done) {
   constexpr int chunk = 64 / LANES;
   int n_traj, total;
   if constexpr (SMALL) {
@@ -187,6 +199,22 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
       c = fixed + s + j * S;
     }
     process(c);
+  }
+  if (done.flag != nullptr) {
+    // each wave's stores (zero-copy outputs in host memory included) are
+    // written back and acknowledged at system scope before the wave is counted
+    // (a release fence: no cache invalidation, which would evict the data of
+    // other launches running beside this one); the last wave counted has seen
+    // every other count, so every output is visible when it resets the count
+    // for the next launch and publishes seq with a release store
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    if (lane == 0) {
+      unsigned long long* fin = tail + kDoneCounter;
+      if (atomicAdd(fin, 1ull) == static_cast<unsigned long long>(G - 1)) {
+        atomicExch(fin, 0ull);
+        __hip_atomic_store(done.flag, done.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
   }
 }
 

@@ -105,11 +105,16 @@ rbd_status batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_gr
 // counters in the batch object's device memory; with trajs (the same list,
 // device-visible pointers) a batch of up to 64 trajectories takes its
 // descriptors from the kernel arguments instead.  Launches on one batch object
-// must run one at a time (the server gives each in-flight slot its own).
+// must run one at a time (the server gives each in-flight slot its own).  With
+// done_flag (a device address of host-mapped coherent memory), the kernel's
+// last wave writes done_seq there once all outputs are visible to the host,
+// so the caller can poll memory instead of an event.  A batch of zero knots
+// launches nothing and writes nothing.
 size_t batch_stage_bytes(int n_traj);
 int64_t batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int lanes);
 rbd_status batch_launch_staged(rbd_batch* b, const void* d_staged, const rbd_traj* trajs, int n_traj,
-                               int64_t total_knots, int64_t total_chunks, int lanes, rbd_stream_t stream);
+                               int64_t total_knots, int64_t total_chunks, int lanes, rbd_stream_t stream,
+                               unsigned long long* done_flag = nullptr, unsigned long long done_seq = 0);
 
 inline rbd_status check_ctx(const rbd_context* ctx) {
   if (!ctx) return fail(RBD_ERR_INVALID_ARGUMENT, "null context");
