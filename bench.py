@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
 MEASURED_TRAFFIC_BYTES = {
     ("fdgrad_two_wave_kernel<PrecFP32>", 128): int((181.0 * 2.000 + 53.0 * 1.000) * 1024),
     ("fdgrad_soa_kernel<PrecFP32>", 1 << 20): int((41344.7 * 2.000 + 430080.0) * 1024),
-    ("fdgrad_aos_kernel<PrecFP32>", 1 << 20): int((68882.5 * 2.000 + 1072550.0) * 1024),
+    ("fdgrad_aos_kernel<PrecFP32>", 1 << 20): int((60402.1 * 2.000 + 881842.3) * 1024),
 }
 FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 (packed vector / matrix) peak
 

@@ -659,6 +659,17 @@ template <bool MINV = true, typename SC, class Out>
 RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out& out) {
   // d/dq column 0 is identically zero (rotation about the gravity axis)
   for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
+  // the d/dq columns, then the d/dqd columns: the order of the outputs in
+  // memory, so that each knot's AoS record fills front to back and few of its
+  // cache lines are partly written at a time
+  static_for<1, NQ>([&](auto Cc) {
+    constexpr int c = decltype(Cc)::value;
+    SC tau[NQ], col[NQ];
+    dtau_column<true>(P, c, tau);
+    if constexpr (MINV) minv_times<true>(P.Mi, tau, col);
+    else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
+    for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
+  });
   static_for<0, NQ>([&](auto Cc) {
     constexpr int c = decltype(Cc)::value;
     SC tau[NQ], col[NQ];
@@ -667,12 +678,6 @@ RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out&
     if constexpr (MINV) minv_times<true>(P.Mi, tau, col);
     else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
     for (int r = 0; r < NQ; ++r) out.dqd(r, c, float(col[r]));
-    if constexpr (c > 0) {
-      dtau_column<true>(P, c, tau);
-      if constexpr (MINV) minv_times<true>(P.Mi, tau, col);
-      else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
-      for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
-    }
   });
 }
 
