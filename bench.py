@@ -259,13 +259,18 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000, 300000), durat
     caller's host buffer.  Three serving modes: requests' buffers in device
     memory (copies), zero-copy, and zero-copy for batches up to 4096 knots with
     copies above ("auto"); the two that copy are also run at 1M req/s, past
-    what zero-copy batches sustain."""
+    what zero-copy batches sustain.  The automatic mode is run again with
+    two-wave batches."""
     pool = knots(1 << 16, seed=103)
     res = {}
-    for mode, zero_copy in (("device_buffers", 0), ("zero_copy", 1), ("auto", 2)):
-        srv = rbdgrad.Server(ctx, slots=4, lanes_per_knot=16, memory_budget_bytes=256 << 20, zero_copy=zero_copy,
+    # the first three with lane-group batches (16 lanes per knot); the last the
+    # automatic mode with two-wave batches (32), lower latency while batches
+    # are small
+    for mode, zero_copy, lanes in (("device_buffers", 0, 16), ("zero_copy", 1, 16), ("auto", 2, 16),
+                                   ("auto_two_wave", 2, 32)):
+        srv = rbdgrad.Server(ctx, slots=4, lanes_per_knot=lanes, memory_budget_bytes=256 << 20, zero_copy=zero_copy,
                              poll_dispatch=1)
-        for rate in rates + ((1000000,) if zero_copy != 1 else ()):
+        for rate in rates + ((1000000,) if zero_copy != 1 and lanes == 16 else ()):
             log(f"[c3] served ({mode}): Poisson {rate} req/s for {duration_s} s")
             st = srv.run_poisson(rate, duration_s, [16, 32, 64, 128, 256], pool, seed=rate)
             st["batch_phases_us"] = batch_phases(srv.trace())

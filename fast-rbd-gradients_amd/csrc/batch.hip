@@ -31,11 +31,11 @@ struct rbd_batch {
   char* h_stage[kRing] = {};
   hipEvent_t staged[kRing] = {};
   int ring = 0;
-  // persistent grid (waves) per mapping, and the captured graphs per
-  // (mapping, small batch): index 2 * (lanes == 16) + small
-  unsigned grid[2] = {0, 0};          // [0]: one lane per knot, [1]: lane groups
-  hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  // persistent grid per mapping, and the captured graphs per (mapping, small
+  // batch): index 2 * mapping + small
+  unsigned grid[3] = {0, 0, 0};       // waves: [0] one lane per knot, [1] lane groups; [2] two-wave workgroups
+  hipGraphExec_t exec[6] = {};
+  hipGraph_t graph[6] = {};
   int64_t last_total_knots = 0;
   int last_lanes = 0;
 };
@@ -61,14 +61,27 @@ struct Queue {
 // the batch's own header and descriptors, uploaded per launch
 Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, b->d_prefix}; }
 
+// mapping index (0 one lane, 1 lane groups, 2 two waves) and knots per chunk
+int mapping_of(int lanes) { return lanes == 1 ? 0 : (lanes == rbd::kLanesPerKnotGroup ? 1 : 2); }
+long chunk_knots(int lanes) { return lanes == 1 ? 64 : rbd::kTwoWaveKnots; }
+
 template <class P, int LANES, bool SMALL>
 void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
-  // a batch of fewer chunks than the persistent grid launches one wave per
-  // chunk (fewer workgroups to dispatch); graphs keep the full grid, whose
-  // captured launch serves every batch
-  unsigned grid = b->grid[LANES == 1 ? 0 : 1];
+  // a batch of fewer chunks than the persistent grid launches one workgroup
+  // per chunk (fewer to dispatch); graphs keep the full grid, whose captured
+  // launch serves every batch
+  unsigned grid = b->grid[mapping_of(LANES)];
   if (q.chunks > 0 && q.chunks < int64_t(grid)) grid = unsigned(q.chunks);
-  if constexpr (SMALL) {
+  if constexpr (LANES == kTwoWaveLanes) {
+    static const rbd::ArgBatch kNone{};
+    const rbd::ArgBatch& a = q.args ? *q.args : kNone;
+    if constexpr (SMALL)
+      hipLaunchKernelGGL((rbd::batch_two_wave_kernel<P, true>), dim3(grid), dim3(2 * kBlock), 0, s,
+                         q.args ? nullptr : q.descs, q.hdr, a, b->d_tail, q.prefix, b->ctx->opt, q.done);
+    else
+      hipLaunchKernelGGL((rbd::batch_two_wave_kernel<P, false>), dim3(grid), dim3(2 * kBlock), 0, s, q.descs,
+                         q.hdr, rbd::NoArgBatch{}, b->d_tail, q.prefix, b->ctx->opt, q.done);
+  } else if constexpr (SMALL) {
     static const rbd::ArgBatch kNone{};
     const rbd::ArgBatch& a = q.args ? *q.args : kNone;
     hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, true>), dim3(grid), dim3(kBlock), 0, s,
@@ -85,6 +98,9 @@ hipError_t launch_batch_p(rbd_batch* b, const Queue& q, int lanes, bool small, h
     hipLaunchKernelGGL(rbd::batch_plan_kernel, dim3(1), dim3(rbd::kPlanThreads), 0, s, q.descs, q.hdr, q.prefix);
   if (lanes == 1) {
     if (small) launch_persistent<P, 1, true>(b, q, s); else launch_persistent<P, 1, false>(b, q, s);
+  } else if (lanes == kTwoWaveLanes) {
+    if (small) launch_persistent<P, kTwoWaveLanes, true>(b, q, s);
+    else launch_persistent<P, kTwoWaveLanes, false>(b, q, s);
   } else {
     if (small) launch_persistent<P, rbd::kLanesPerKnotGroup, true>(b, q, s);
     else launch_persistent<P, rbd::kLanesPerKnotGroup, false>(b, q, s);
@@ -107,7 +123,7 @@ hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hip
 long stage(char* dst, const rbd_traj* trajs, int n_traj, int lanes) {
   rbd::BatchHeader* h = reinterpret_cast<rbd::BatchHeader*>(dst);
   std::memset(h, 0, sizeof(*h));
-  const long chunk = 64 / lanes;
+  const long chunk = chunk_knots(lanes);
   long total = 0;
   rbd::TrajDesc* d = reinterpret_cast<rbd::TrajDesc*>(dst + kHeaderBytes);
   for (int t = 0; t < n_traj; ++t) {
@@ -120,7 +136,7 @@ long stage(char* dst, const rbd_traj* trajs, int n_traj, int lanes) {
   return total;
 }
 
-int graph_index(int lanes, bool small) { return 2 * (lanes == 1 ? 0 : 1) + (small ? 1 : 0); }
+int graph_index(int lanes, bool small) { return 2 * mapping_of(lanes) + (small ? 1 : 0); }
 
 // capture the batch's kernels for one (mapping, small) pair into a graph (once)
 rbd_status ensure_graph(rbd_batch* b, int lanes, bool small) {
@@ -193,6 +209,7 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
   const int waves = (ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT ||
                      ctx->precision == RBD_PREC_FP32_HW_TRIG) ? 2 : 1;
   b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4 * waves);
+  b->grid[2] = unsigned(ctx->compute_units * 2);   // two-wave workgroups (256 registers: one wave per SIMD)
   *out = b;
   return RBD_OK;
 }
@@ -200,7 +217,7 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
 rbd_status rbd_batch_destroy(rbd_batch* b) {
   if (!b) return RBD_OK;
   (void)use_device(b->ctx);
-  for (int gi = 0; gi < 4; ++gi) {
+  for (int gi = 0; gi < 6; ++gi) {
     if (b->exec[gi]) (void)hipGraphExecDestroy(b->exec[gi]);
     if (b->graph[gi]) (void)hipGraphDestroy(b->graph[gi]);
   }
@@ -264,7 +281,7 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   if (small && !use_graph) {
     // the descriptors go in the kernel arguments: no upload
     rbd::ArgBatch a{};
-    const long chunk = 64 / lanes;
+    const long chunk = chunk_knots(lanes);
     long chunks = 0;
     for (int t = 0; t < n_traj; ++t) {
       a.d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};

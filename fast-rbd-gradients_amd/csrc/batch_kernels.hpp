@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fdgrad_knot.hpp"
+#include "two_wave.hpp"
 
 namespace rbd {
 
@@ -107,6 +108,53 @@ __global__ __launch_bounds__(kPlanThreads) void batch_plan_kernel(const TrajDesc
   if (threadIdx.x == kPlanThreads - 1) prefix[n] = partial[kPlanThreads - 1];
 }
 
+// The trajectory t of chunk c and t's first chunk.  SMALL: the counts are
+// scanned in the calling wave, one trajectory per lane (every lane of the wave
+// takes part); otherwise a binary search of the plan kernel's prefix sum.
+template <bool SMALL>
+__device__ inline void locate_chunk(const TrajDesc* descs, const long* prefix, int n_traj, int chunk, int c,
+                                    int lane, int& t, int& first) {
+  if constexpr (SMALL) {
+    int v = lane < n_traj ? int((descs[lane].n + chunk - 1) / chunk) : 0;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(v, off, 64);
+      if (lane >= off) v += o;
+    }
+    // trajectories that end at or before chunk c are the lanes with v <= c
+    t = __popcll(__ballot(v <= c));
+    first = __shfl(v, t > 0 ? t - 1 : 0, 64);
+    if (t == 0) first = 0;
+  } else {
+    int lo = 0, hi = n_traj;               // prefix[lo] <= c < prefix[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= c) lo = mid; else hi = mid;
+    }
+    t = lo;
+    first = int(prefix[lo]);
+  }
+}
+
+// End of a launch that publishes a completion word (done.flag set): each
+// wave's stores (zero-copy outputs in host memory included) are written back
+// and acknowledged at system scope before the wave is counted (a release
+// fence: no cache invalidation, which would evict the data of other launches
+// running beside this one); the last of the launch's `waves` waves to be
+// counted has seen every other count, so every output is visible when it
+// resets the count for the next launch and publishes seq with a release store.
+__device__ inline void publish_completion(const Completion& done, unsigned long long* tail, unsigned waves,
+                                          int lane) {
+  if (done.flag == nullptr) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+  if (lane == 0) {
+    unsigned long long* fin = tail + kDoneCounter;
+    if (atomicAdd(fin, 1ull) == static_cast<unsigned long long>(waves - 1)) {
+      atomicExch(fin, 0ull);
+      __hip_atomic_store(done.flag, done.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
+
 // Persistent drain of the queue.  LANES = 1: lane i of a wave takes knot
 // k0 + i of a 64-knot chunk; LANES = 16: lane group g takes knot k0 + g of a
 // 4-knot chunk.  One wave per workgroup.  tail: kTailWords words (the
@@ -142,25 +190,7 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
 
   auto process = [&](int c) {
     int t, first;                           // trajectory of chunk c, and its first chunk
-    if constexpr (SMALL) {
-      int v = lane < n_traj ? int((descs[lane].n + chunk - 1) / chunk) : 0;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(v, off, 64);
-        if (lane >= off) v += o;
-      }
-      // trajectories that end at or before chunk c are the lanes with v <= c
-      t = __popcll(__ballot(v <= c));
-      first = __shfl(v, t > 0 ? t - 1 : 0, 64);
-      if (t == 0) first = 0;
-    } else {
-      int lo = 0, hi = n_traj;               // prefix[lo] <= c < prefix[hi]
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= c) lo = mid; else hi = mid;
-      }
-      t = lo;
-      first = int(prefix[lo]);
-    }
+    locate_chunk<SMALL>(descs, prefix, n_traj, chunk, c, lane, t, first);
     const TrajDesc d = descs[t];
     const long k = long(c - first) * chunk + slot;
     if (k < d.n) {
@@ -200,22 +230,52 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
     }
     process(c);
   }
-  if (done.flag != nullptr) {
-    // each wave's stores (zero-copy outputs in host memory included) are
-    // written back and acknowledged at system scope before the wave is counted
-    // (a release fence: no cache invalidation, which would evict the data of
-    // other launches running beside this one); the last wave counted has seen
-    // every other count, so every output is visible when it resets the count
-    // for the next launch and publishes seq with a release store
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-    if (lane == 0) {
-      unsigned long long* fin = tail + kDoneCounter;
-      if (atomicAdd(fin, 1ull) == static_cast<unsigned long long>(G - 1)) {
-        atomicExch(fin, 0ull);
-        __hip_atomic_store(done.flag, done.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-      }
+  publish_completion(done, tail, unsigned(G), lane);
+}
+
+// Batches on the two-wave mapping: each workgroup of two waves takes
+// kTwoWaveKnots-knot chunks w, w + G, ... (G workgroups) in a fixed order --
+// both waves need the same chunk, and every chunk costs the same -- and runs
+// each as two_wave_group.  Chunk lookup, descriptors in the arguments and the
+// completion word as in batch_persistent_kernel.
+template <class P, bool SMALL>
+__global__ __launch_bounds__(128, 1) void batch_two_wave_kernel(
+    const TrajDesc* __restrict__ descs, const BatchHeader* __restrict__ hdr,
+    const typename std::conditional<SMALL, ArgBatch, NoArgBatch>::type arg, unsigned long long* __restrict__ tail,
+    const long* __restrict__ prefix, Options opt, Completion done) {
+  int n_traj, total;
+  if constexpr (SMALL) {
+    if (descs == nullptr) {
+      descs = arg.d;
+      n_traj = arg.n_traj;
+      total = arg.total_chunks;
+    } else {
+      n_traj = hdr->n_traj;
+      total = int(hdr->total_chunks);
     }
+  } else {
+    n_traj = hdr->n_traj;
+    total = int(hdr->total_chunks);
   }
+  __shared__ TwoWaveShared<P> sh[kTwoWaveKnots];
+  __shared__ int flags[2];                 // [0]: bias ready (R -> M), [1]: Minv and corrections ready (M -> R)
+  const int lane = int(threadIdx.x % 64);
+  const int g = lane / kLanesPerKnotGroup;
+  for (int c = int(blockIdx.x); c < total; c += int(gridDim.x)) {
+    __syncthreads();                       // the previous chunk is done with sh and flags
+    if (threadIdx.x == 0) { flags[0] = 0; flags[1] = 0; }
+    __syncthreads();
+    int t, first;
+    locate_chunk<SMALL>(descs, prefix, n_traj, kTwoWaveKnots, c, lane, t, first);
+    const TrajDesc d = descs[t];           // d.n >= 1: chunk c lies in t
+    const long k = long(c - first) * kTwoWaveKnots + g;
+    const bool valid = k < d.n;
+    const long kk = valid ? k : d.n - 1;   // lanes past the trajectory's end compute a copy, store nothing
+    InAoS i{d.in + kInPerKnot * kk};
+    OutMasked<OutAoS> o{OutAoS{d.out + kOutPerKnot * kk}, valid};
+    two_wave_group<P>(i, o, opt, sh[g], flags);
+  }
+  publish_completion(done, tail, 2u * gridDim.x, lane);
 }
 
 }  // namespace rbd

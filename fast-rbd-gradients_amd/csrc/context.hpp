@@ -86,7 +86,6 @@ inline int lanes_for(const rbd_context* ctx, int64_t n, bool two_wave = false) {
 }
 
 inline int lanes_for_batch(const rbd_context* ctx, int64_t total_knots) {
-  if (ctx->lanes_per_knot == kTwoWaveLanes) return rbd::kLanesPerKnotGroup;
   if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
   return total_knots <= ctx->batch_group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
 }

@@ -20,6 +20,7 @@
 #include <string>
 
 #include "context.hpp"
+#include "two_wave.hpp"
 
 namespace rbdi {
 
@@ -121,37 +122,8 @@ __global__ __launch_bounds__(kBlock, W) void fdgrad_refmode_lanes_kernel(long n,
   }
 }
 
-// Two-wave latency mapping: a workgroup of two waves takes 4 knots, 16 lanes
-// per knot in each wave; wave 0 runs Minv, wave 1 the RNEA and the columns
-// (two_wave_* in fdgrad_knot.hpp).  The waves hand data over in LDS and
-// signal with one LDS flag per direction: each wave sets its flag
-// unconditionally before it waits for the other's, so every wait ends (and
-// gives up after ~2^24 sleeps in any case).  Lanes past the last knot compute
-// a copy of it and store nothing.
-constexpr int kTwoWaveKnots = 4;
-
-// The fences cover LDS only: a release that also waited for the wave's
-// outstanding global stores would hold the flag back by microseconds.  The
-// scheduling barriers keep the flag store and the polling loop where they are
-// in the instruction stream.
-__device__ inline void lds_signal(int* flag) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  if (threadIdx.x % kBlock == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  __builtin_amdgcn_sched_barrier(0);
-}
-__device__ inline void lds_wait(int* flag) {
-  __builtin_amdgcn_sched_barrier(0);
-  for (int spin = 0; spin < (1 << 24); ++spin) {
-    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0) break;
-    __builtin_amdgcn_s_sleep(1);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-// Materialise a value before a wait: the compiler would otherwise sink work
-// whose results are only used after the wait below it, and the two waves
-// would run one after the other instead of side by side.
-template <typename S>
-__device__ inline void pin(S x) { asm volatile("" ::"v"(x)); }
+// Two-wave latency mapping: one group of kTwoWaveKnots knots per workgroup
+// (two_wave_group, two_wave.hpp).
 template <class P, bool SOA>
 __global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, const float* __restrict__ in,
                                                                        long ld_in, float* __restrict__ out,
@@ -160,10 +132,7 @@ __global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, 
   __shared__ int flags[2];                 // [0]: bias ready (R -> M), [1]: Minv and corrections ready (M -> R)
   if (threadIdx.x == 0) { flags[0] = 0; flags[1] = 0; }
   __syncthreads();
-  const int wave = threadIdx.x / kBlock;
-  const int lane = threadIdx.x % kBlock;
-  const int g = lane / kLanesPerKnotGroup;
-  const int role = lane % kLanesPerKnotGroup;
+  const int g = (threadIdx.x % kBlock) / kLanesPerKnotGroup;
   const long k = long(blockIdx.x) * kTwoWaveKnots + g;
   const bool valid = k < n;
   const long kk = valid ? k : n - 1;
@@ -174,30 +143,7 @@ __global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, 
   if constexpr (SOA) { i = InSoA{in + kk, ld_in}; o0 = OutSoA{out + kk, ld_out}; }
   else { i = InAoS{in + kInPerKnot * kk}; o0 = OutAoS{out + kOutPerKnot * kk}; }
   OutMasked<O> o{o0, valid};
-  if (wave == 0) {
-    CS<TwoWaveTrig<P>> tw[NQ];
-    typename P::Minv Mi[NQ][NQ];
-    two_wave_m1a<P>(i, role, tw, sh[g]);
-    wave_lds_sync();
-    two_wave_m1b<P>(sh[g], Mi);
-    for (int r = 0; r < NQ; ++r) {
-      pin(tw[r].c);
-      pin(tw[r].s);
-      for (int c = r; c < NQ; ++c) pin(Mi[r][c]);
-    }
-    lds_wait(&flags[0]);
-    two_wave_m2<P>(role, tw, Mi, sh[g], o);
-    lds_signal(&flags[1]);
-  } else {
-    TwoWaveR<P> st;
-    typename P::Col tau_col[NQ];
-    two_wave_r1a<P>(i, opt, role, sh[g], st);
-    lds_signal(&flags[0]);
-    two_wave_r1b<P>(opt, role, st, tau_col);
-    for (int r = 0; r < NQ; ++r) pin(tau_col[r]);
-    lds_wait(&flags[1]);
-    two_wave_r2<P>(role, sh[g], tau_col, o);
-  }
+  two_wave_group<P>(i, o, opt, sh[g], flags);
 }
 
 // Full path plus dqdd/du = Minv (49 per knot, dense), both mappings.

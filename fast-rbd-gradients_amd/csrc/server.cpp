@@ -500,8 +500,8 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
   if (c.memory_budget_bytes < int64_t(kAlign)) return fail(RBD_ERR_INVALID_ARGUMENT, "memory budget too small");
   if (c.max_batch_trajectories < 1 || c.max_batch_knots < 1)
     return fail(RBD_ERR_INVALID_ARGUMENT, "batch limits must be positive");
-  if (c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup)
-    return fail(RBD_ERR_INVALID_ARGUMENT, "server lanes_per_knot must be 1 or 16");
+  if (c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup && c.lanes_per_knot != kTwoWaveLanes)
+    return fail(RBD_ERR_INVALID_ARGUMENT, "server lanes_per_knot must be 1, 16 or 32");
   if (!(c.batch_window_us >= 0)) return fail(RBD_ERR_INVALID_ARGUMENT, "negative batch window");
   if (c.zero_copy < 0 || c.zero_copy > 2) return fail(RBD_ERR_INVALID_ARGUMENT, "zero_copy must be 0, 1 or 2");
   if (c.zero_copy_max_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative zero_copy_max_knots");

@@ -76,7 +76,7 @@ typedef struct rbd_config {
                          knot (throughput); 16: one knot per 16-lane group (latency);
                          32: one knot per 16-lane group in each of two waves, Minv
                          on one and the RNEA and columns on the other (lowest
-                         latency; full path only, other calls use 16) */
+                         latency; full path and batches, other calls use 16) */
   int reserved[4];    /* must be zero */
 } rbd_config;
 
@@ -268,7 +268,8 @@ rbd_status rbd_batch_destroy(rbd_batch* b);
  * use_graph); more are copied to the device asynchronously on `stream`.  The
  * caller's trajs array may be reused on return.  The knot-to-lane mapping is
  * chosen from the total knot count against the context's batch crossover
- * (rbd_autotune_batch), unless the context fixes lanes_per_knot.
+ * (rbd_autotune_batch), unless the context fixes lanes_per_knot (32: the
+ * two-wave mapping, 4-knot chunks on workgroups of two waves).
  * use_graph != 0 replays the batch's kernels from a hipGraph captured on first
  * use (one graph per mapping and batch kind).  Calls on one batch object are
  * ordered by their streams: two streams must not run the same batch object at
@@ -277,12 +278,13 @@ rbd_status rbd_batch_destroy(rbd_batch* b);
 rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph,
                                   rbd_stream_t stream);
 
-/* Persistent grid size (waves) per mapping (default: the waves the device's
- * SIMDs keep resident, 4 per CU times 2 with FP32 columns or 1 with FP64
- * columns). */
+/* Persistent grid size (waves) per mapping, lanes_per_knot 1 or 16 (default:
+ * the waves the device's SIMDs keep resident, 4 per CU times 2 with FP32
+ * columns or 1 with FP64 columns; the two-wave mapping runs 2 workgroups of
+ * two waves per CU). */
 rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves);
 
-/* Lanes per knot used by the last rbd_fdgrad_batched_f32 call (1 or 16). */
+/* Lanes per knot used by the last rbd_fdgrad_batched_f32 call (1, 16 or 32). */
 int rbd_batch_last_lanes(const rbd_batch* b);
 
 /* ------------------------------------------------------------------------
@@ -305,7 +307,7 @@ typedef struct rbd_server rbd_server;
 
 typedef struct rbd_server_config {
   int slots;                      /* batches in flight (one stream each), 1..64 */
-  int lanes_per_knot;             /* 1 or 16, used for every batch */
+  int lanes_per_knot;             /* 1, 16 or 32 (two waves per knot group), used for every batch */
   int64_t memory_budget_bytes;    /* device memory for admitted requests (and as much pinned staging) */
   int64_t max_batch_trajectories; /* per batch */
   int64_t max_batch_knots;        /* per batch; also the longest admissible request */

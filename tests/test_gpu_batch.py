@@ -170,6 +170,51 @@ def test_last_round_counters_return_to_zero(rbd, lanes, waves):
     c.close()
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_two_wave_batches(rbd, oracle, use_graph):
+    """lanes_per_knot 32: 4-knot chunks on workgroups of two waves (Minv on one,
+    RNEA and columns on the other), pulled in a fixed order.  Back-to-back
+    launches without synchronisation: small batches (descriptors in the
+    arguments), large ones (plan kernel), zero-length trajectories, and batches
+    of more chunks than the grid's 4 workgroups per CU, so that workgroups loop
+    over several chunks.  Every trajectory is bitwise the same as when run
+    alone in a batch; against the flat two-wave kernel (the same arithmetic,
+    compiled into another kernel, so FP32 rounding may differ) within 2e-5 per
+    block, and within the 1e-4 bar of the oracle."""
+    c = rbd.Context(precision="fp32", lanes_per_knot=32)
+    b = rbd.Batch(c, 256)
+    rng = np.random.default_rng(50)
+    runs = []
+    for rep, (n_traj, hi) in enumerate([(3, 40), (64, 257), (65, 90), (200, 120), (10, 2000), (1, 5)]):
+        lengths = rng.integers(0, hi, n_traj)
+        lengths[0] = max(lengths[0], 1)
+        xs, trajs = make_batch(rbd, lengths, seed=500 + rep)
+        b.run(trajs, use_graph=use_graph)
+        assert b.last_lanes == 32
+        runs.append((xs, trajs, lengths))
+    rbd.hip().hipDeviceSynchronize()
+    for rep, (xs, trajs, lengths) in enumerate(runs):
+        got = download(trajs)
+        for t, (x, g, n) in enumerate(zip(xs, got, lengths)):
+            if not n:
+                continue
+            err = common.max_block_errors(g, c.run_aos(x))
+            assert max(err.values()) < 2e-5, (rep, t, err)
+            if t % 17 == 0:                               # alone in a batch: bitwise the same
+                one = [(rbd.DeviceBuffer.from_array(x), rbd.DeviceBuffer(int(n) * 105 * 4), int(n))]
+                b.run(one, use_graph=use_graph)
+                rbd.hip().hipDeviceSynchronize()
+                np.testing.assert_array_equal(download(one)[0], g)
+        if rep == 1:
+            err = common.max_block_errors(np.concatenate(got), oracle.fdgrad(np.concatenate(xs).astype(np.float64),
+                                                                            threads=8))
+            assert max(err.values()) < common.REL_TOL, err
+        say(f"two-wave batch: {len(lengths)} trajectories, {int(sum(lengths))} knots "
+            f"({int(sum((int(n) + 3) // 4 for n in lengths))} chunks) ok")
+    b.close()
+    c.close()
+
+
 def test_batch_autotune(rbd):
     c = rbd.Context(precision="fp32")
     thr, table = c.autotune_batch()

@@ -20,7 +20,7 @@ def ctx(rbd):
 
 
 @pytest.mark.parametrize("zero_copy,poll", [(0, 0), (1, 0), (1, 1), (2, 1)])
-@pytest.mark.parametrize("lanes", [1, 16])
+@pytest.mark.parametrize("lanes", [1, 16, 32])
 def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_copy, poll):
     # zero_copy = 2 with a 300-knot limit: the probe alone is a zero-copy batch,
     # in the mix most batches are copied
@@ -44,9 +44,15 @@ def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_c
     for x, o in zip(others[::37], outs[::37]):
         err = common.max_block_errors(o, oracle.fdgrad(x.astype(np.float64)))
         assert max(err.values()) < common.REL_TOL, err
-    # the same probe through the plain per-call path with the same mapping: identical
+    # the same probe through the plain per-call path with the same mapping:
+    # identical (the two-wave batch kernel compiles the same arithmetic into
+    # another kernel, so there FP32 rounding may differ)
     flat = rbd.Context(precision="fp32", lanes_per_knot=lanes)
-    np.testing.assert_array_equal(flat.run_aos(probe), alone)
+    if lanes == 32:
+        err = common.max_block_errors(alone, flat.run_aos(probe))
+        assert max(err.values()) < 2e-5, err
+    else:
+        np.testing.assert_array_equal(flat.run_aos(probe), alone)
     flat.close()
     srv.close()
 
