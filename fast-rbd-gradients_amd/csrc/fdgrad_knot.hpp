@@ -1331,6 +1331,19 @@ struct OutAoSMinv {
   RBD_HD void dqd(int r, int c, float x) const { p[NQ + NQ * NQ + NQ * c + r] = x; }
   RBD_HD void minv(int r, int c, float x) const { m[NQ * c + r] = x; }
 };
+// The WAFR baseline's gradient layout (time_GPU_WAFR.cu:18; finish_dqdd_v2,
+// helpers_WAFR/dynamics_WAFR.cuh:2555-2579): per knot kWafrPerKnot floats,
+// [dqdd/dq | dqdd/dqd | dqdd/du = Minv], each 7x7 column-major; qdd (7 per
+// knot) to its own buffer when q is not null.
+constexpr int kWafrPerKnot = 3 * NQ * NQ;
+struct OutWafr {
+  static constexpr bool kMinv = true;
+  float* q; float* p;
+  RBD_HD void qdd(int i, float x) const { if (q) q[i] = x; }
+  RBD_HD void dq(int r, int c, float x) const { p[NQ * c + r] = x; }
+  RBD_HD void dqd(int r, int c, float x) const { p[NQ * NQ + NQ * c + r] = x; }
+  RBD_HD void minv(int r, int c, float x) const { p[2 * NQ * NQ + NQ * c + r] = x; }
+};
 // forward dynamics only: qdd (7 per knot) and Minv (49 per knot, dense column-major)
 // an output accessor whose stores are dropped when `on` is false (lanes that
 // only take part in a workgroup's barriers)

@@ -146,8 +146,10 @@ __global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, 
   two_wave_group<P>(i, o, opt, sh[g], flags);
 }
 
-// Full path plus dqdd/du = Minv (49 per knot, dense), both mappings.
-template <class P, int LANES>
+// Full path plus dqdd/du = Minv (49 per knot, dense), both mappings.  WAFR:
+// out receives the WAFR baseline's 147-float layout (OutWafr) and minv is the
+// optional qdd buffer.
+template <class P, int LANES, bool WAFR = false>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_minv_kernel(long n, const float* __restrict__ in,
                                                              float* __restrict__ out, float* __restrict__ minv,
                                                              Options opt) {
@@ -156,7 +158,10 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_minv_kernel(l
   const long groups = long(gridDim.x) * (blockDim.x / LANES);
   for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
     InAoS i{in + kInPerKnot * k};
-    OutAoSMinv o{out + kOutPerKnot * k, minv + NQ * NQ * k};
+    using O = typename std::conditional<WAFR, OutWafr, OutAoSMinv>::type;
+    O o;
+    if constexpr (WAFR) o = OutWafr{minv ? minv + NQ * k : nullptr, out + kWafrPerKnot * k};
+    else o = OutAoSMinv{out + kOutPerKnot * k, minv + NQ * NQ * k};
     if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
     else fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
   }
@@ -492,12 +497,14 @@ const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knot
   return names[reference_mode ? 1 : 0][mapping][ctx->precision];
 }
 
-rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, float* d_out, float* d_minv,
-                                   rbd_stream_t stream) {
-  if (rbd_status s = check_ctx(ctx)) return s;
-  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
-  if (n == 0) return RBD_OK;
-  if (!d_in || !d_out || !d_minv) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+extern "C++" {
+namespace {
+
+// the Minv-output kernels of both layouts (WAFR: out is the 147-float layout,
+// minv the optional qdd buffer)
+template <bool WAFR>
+rbd_status launch_minv(rbd_context* ctx, int64_t n, const float* d_in, float* d_out, float* d_minv,
+                       rbd_stream_t stream) {
   if (rbd_status s = use_device(ctx)) return s;
   const int lanes = lanes_for(ctx, n);
   const dim3 grid(grid_for(n, lanes)), block(kBlock);
@@ -505,10 +512,11 @@ rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n, const float* d_
   auto go = [&](auto prec_tag) {
     using P = decltype(prec_tag);
     if (lanes == 1)
-      hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, 1>), grid, block, 0, st, long(n), d_in, d_out, d_minv, ctx->opt);
+      hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, 1, WAFR>), grid, block, 0, st, long(n), d_in, d_out, d_minv,
+                         ctx->opt);
     else
-      hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, rbd::kLanesPerKnotGroup>), grid, block, 0, st, long(n), d_in,
-                         d_out, d_minv, ctx->opt);
+      hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, rbd::kLanesPerKnotGroup, WAFR>), grid, block, 0, st, long(n),
+                         d_in, d_out, d_minv, ctx->opt);
   };
   switch (ctx->precision) {
     case RBD_PREC_FP32: go(rbd::PrecFP32{}); break;
@@ -519,6 +527,27 @@ rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n, const float* d_
   }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_minv_kernel launch");
+}
+
+}  // namespace
+}  // extern "C++"
+
+rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, float* d_out, float* d_minv,
+                                   rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
+  if (n == 0) return RBD_OK;
+  if (!d_in || !d_out || !d_minv) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+  return launch_minv<false>(ctx, n, d_in, d_out, d_minv, stream);
+}
+
+rbd_status rbd_fdgrad_wafr_f32(rbd_context* ctx, int64_t n, const float* d_in, float* d_qdd, float* d_dqdd,
+                               rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
+  if (n == 0) return RBD_OK;
+  if (!d_in || !d_dqdd) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
+  return launch_minv<true>(ctx, n, d_in, d_dqdd, d_qdd, stream);
 }
 
 namespace {

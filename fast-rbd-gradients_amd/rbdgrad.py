@@ -53,7 +53,7 @@ EXPORTED = [
     "rbd_server_run_poisson", "rbd_server_get_trace", "rbd_fdgrad_minv_traj_f32", "rbd_forward_dynamics_f32",
     "rbd_fdgrad_traj_fext_f32", "rbd_forward_dynamics_fext_f32",
     "rbd_dynamics_gradient_kernel_fused_f32", "rbd_dynamics_gradient_kernel_split_f32",
-    "rbd_gradient_setup_split_f32", "rbd_gradient_finish_f32", "rbd_model_name",
+    "rbd_gradient_setup_split_f32", "rbd_gradient_finish_f32", "rbd_model_name", "rbd_fdgrad_wafr_f32",
 ]
 
 
@@ -126,6 +126,7 @@ def lib(model=None):
         L.rbd_dynamics_gradient_kernel_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_fdgrad_traj_host_f32.argtypes = [vp, i64, vp, vp]
         L.rbd_fdgrad_minv_traj_f32.argtypes = [vp, i64, vp, vp, vp, vp]
+        L.rbd_fdgrad_wafr_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_forward_dynamics_f32.argtypes = [vp, i64, vp, vp, vp, vp]
         L.rbd_fdgrad_traj_fext_f32.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.rbd_dynamics_gradient_kernel_fused_f32.argtypes = [vp, i64, vp, vp, vp, vp]
@@ -448,6 +449,19 @@ class Context(_Bound):
         self._check(self._L.rbd_fdgrad_minv_traj_f32(self._h, n, d_in.handle, d_out.handle, d_m.handle, None),
                "rbd_fdgrad_minv_traj_f32")
         return d_out.download((n, OUT_PER_KNOT)), d_m.download((n, 7, 7)).transpose(0, 2, 1).copy()
+
+    def run_wafr(self, x, with_qdd=True):
+        """Full path in the WAFR baseline's gradient layout: returns (qdd (n, 7)
+        or None, dqdd (n, 147) = [dqdd/dq | dqdd/dqd | Minv] per knot, each
+        7x7 column-major)."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float32)
+        n = x.shape[0]
+        d_in = DeviceBuffer.from_array(x)
+        d_q = DeviceBuffer(n * 7 * 4) if with_qdd else None
+        d_d = DeviceBuffer(n * 147 * 4)
+        self._check(self._L.rbd_fdgrad_wafr_f32(self._h, n, d_in.handle, d_q.handle if d_q else None, d_d.handle,
+                                                None), "rbd_fdgrad_wafr_f32")
+        return (d_q.download((n, 7)) if d_q else None), d_d.download((n, 147))
 
     def forward_dynamics(self, x, fext=None):
         """qdd (n, 7) and Minv (n, 7, 7) [knot][row][col] from x (n, 21);

@@ -143,6 +143,17 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n_knots, const flo
 rbd_status rbd_fdgrad_minv_traj_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, float* d_out,
                                    float* d_minv, rbd_stream_t stream);
 
+/* Full path in the WAFR baseline's gradient layout: d_dqdd receives 147 floats
+ * per knot, [dqdd/dq | dqdd/dqd | dqdd/du = Minv], each 7x7 column-major, and
+ * d_qdd the 7 of qdd per knot unless it is null.  Input as rbd_fdgrad_traj_f32
+ * (q, qd, u; 21 floats per knot).
+ * Replaces: the WAFR baseline's gradient output d_dqdd (time_GPU_WAFR.cu:18,
+ * :134; finish_dqdd_v2, helpers_WAFR/dynamics_WAFR.cuh:2555-2579), here
+ * computed by the reference method from q, qd, u instead of from a given qdd
+ * and Minv. */
+rbd_status rbd_fdgrad_wafr_f32(rbd_context* ctx, int64_t n_knots, const float* d_in, float* d_qdd, float* d_dqdd,
+                               rbd_stream_t stream);
+
 /* Forward dynamics only: qdd (7 per knot) and the dense Minv (49 per knot,
  * column-major) from q, qd, u (21 per knot), one lane per knot.  The bias RNEA
  * runs with qdd = 0.

@@ -57,3 +57,25 @@ def test_dqdd_du_output(rbd, lanes):
     close_f32(minv, minv_fd)
     np.testing.assert_array_equal(minv, minv.transpose(0, 2, 1))
     ctx.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 16])
+def test_wafr_layout(rbd, oracle, lanes):
+    """rbd_fdgrad_wafr_f32: per knot [dqdd/dq | dqdd/dqd | dqdd/du = Minv]
+    (147 floats, each 7x7 column-major, as the WAFR harness reads d_dqdd,
+    time_GPU_WAFR.cu:18) and qdd to its own, optional buffer; the values of
+    the 105 + 49 entry point (another kernel instantiation, so to FP32
+    rounding) in the WAFR places, and within the bar of the oracle."""
+    x = common.knots(300, seed=902)
+    ctx = rbd.Context(precision="fp32", lanes_per_knot=lanes)
+    out, minv = ctx.run_with_minv(x)                      # minv [knot][row][col]
+    qdd, w = ctx.run_wafr(x)
+    err = common.max_block_errors(np.hstack([qdd, w[:, :98]]), out)
+    assert max(err.values()) < 1e-5, err
+    close_f32(w[:, 98:].reshape(-1, 7, 7).transpose(0, 2, 1), minv)   # entry (r, c) at 7c + r
+    q2, w2 = ctx.run_wafr(x, with_qdd=False)
+    assert q2 is None
+    np.testing.assert_array_equal(w2, w)
+    err = common.max_block_errors(np.hstack([qdd, w[:, :98]]), oracle.fdgrad(x.astype(np.float64)))
+    assert max(err.values()) < common.REL_TOL, err
+    ctx.close()
