@@ -417,6 +417,15 @@ def profile_one(args, stream):
         return
     if args.only == "c3":
         ctx = rbdgrad.Context(device=0, precision=args.precision, model=args.model)
+        if args.rate:
+            # one automatic-mode run at one rate and mapping (profiler runs)
+            srv = rbdgrad.Server(ctx, slots=4, lanes_per_knot=args.server_lanes, memory_budget_bytes=256 << 20,
+                                 zero_copy=2, poll_dispatch=1)
+            st = srv.run_poisson(args.rate, 1.0, [16, 32, 64, 128, 256], knots(1 << 16, seed=103), seed=args.rate)
+            st["batch_phases_us"] = batch_phases(srv.trace())
+            srv.close()
+            log(json.dumps(st))
+            return
         res = served_workload(ctx)
         log(json.dumps(res))
         return
@@ -443,6 +452,10 @@ def main():
     ap.add_argument("--quick", action="store_true", help="skip the large-batch and mixed workloads")
     ap.add_argument("--only", choices=["c1", "c1-one-lane", "c1-lane-group", "c4-aos", "c4-soa", "c2", "c3", "copy"],
                     help="run one workload only and print its timing (for profiler runs)")
+    ap.add_argument("--rate", type=int, default=0,
+                    help="with --only c3: one automatic-mode served run at this many requests/s")
+    ap.add_argument("--server-lanes", type=int, default=16, choices=[16, 32],
+                    help="with --only c3 --rate: the server's mapping (16: lane groups, 32: two waves)")
     ap.add_argument("--model", default=None,
                     help="with --only: the robot library librbdgrad_MODEL.so (default: the iiwa 14)")
     args = ap.parse_args()
