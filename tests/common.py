@@ -361,4 +361,5 @@ def hostcheck_lib(model=None):
     L.host_fdgrad_two_wave_f32.argtypes = [ctypes.c_long, fp, fp, ctypes.c_int, ctypes.c_int]
     L.host_model_name.restype = ctypes.c_char_p
     L.host_minv_columns.argtypes = [ctypes.c_long, dp, dp, dp]
+    L.host_fdgrad_wafr_f32.argtypes = [ctypes.c_long, fp, fp, fp, ctypes.c_int, ctypes.c_int]
     return L

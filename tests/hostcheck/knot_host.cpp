@@ -156,6 +156,21 @@ void host_fdgrad_minv_f32(long n, const float* in, float* out, float* minv, int 
   }
 }
 
+// full path in the WAFR harness's layout: qdd (7 per knot, when qdd is not
+// null) and 147 per knot [dqdd/dq | dqdd/dqd | Minv]; one lane (lanes = 0) or
+// lane-group emulation (lanes = 1)
+void host_fdgrad_wafr_f32(long n, const float* in, float* qdd, float* dqdd, int flags, int lanes) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    InAoS i{in + kInPerKnot * k};
+    OutWafr o{qdd ? qdd + NQ * k : nullptr, dqdd + kWafrPerKnot * k};
+    if (lanes)
+      lane_group<PrecFP32>(i, o, opt);
+    else
+      fdgrad_knot<PrecFP32>(i, o, opt);
+  }
+}
+
 // full path and forward dynamics with external link forces (knot k uses
 // fext + k * fext_stride), FP32, one-lane (lanes = 0) or lane-group emulation
 void host_fdgrad_fext_f32(long n, const float* in, const float* fext, long fext_stride, float* out, int flags,

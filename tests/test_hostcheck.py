@@ -117,6 +117,35 @@ def test_forward_dynamics_and_minv_outputs(host, oracle, flags):
         np.testing.assert_allclose(qdd[i], q_ref, rtol=0, atol=1e-5 * np.abs(q_ref).max())
 
 
+@pytest.mark.parametrize("flags", [0, 1, 2, 3])
+def test_wafr_layout(host, oracle, flags):
+    """The WAFR harness's gradient layout (time_GPU_WAFR.cu:18; finish_dqdd_v2,
+    helpers_WAFR/dynamics_WAFR.cuh:2555-2579): per knot 147 floats, dqdd/dq,
+    dqdd/dqd and dqdd/du = Minv, each 7x7 column-major (entry (r, c) at 7c + r),
+    and qdd on its own: the full path's values placed there, and Minv against
+    the oracle's forward dynamics."""
+    n = 60
+    x = np.ascontiguousarray(common.knots(n, seed=240 + flags))
+    fp = ctypes.POINTER(ctypes.c_float)
+    full = run(host, x, flags)
+    for lanes in (0, 1):
+        qdd = np.full((n, 7), np.nan, np.float32)
+        w = np.full((n, 147), np.nan, np.float32)
+        host.host_fdgrad_wafr_f32(n, x.ctypes.data_as(fp), qdd.ctypes.data_as(fp), w.ctypes.data_as(fp), flags,
+                                  lanes)
+        ref = full if lanes == 0 else run(host, x, flags, lanes=1)
+        np.testing.assert_array_equal(qdd, ref[:, :7])
+        np.testing.assert_array_equal(w[:, :98], ref[:, 7:])
+        w2 = np.full((n, 147), np.nan, np.float32)
+        host.host_fdgrad_wafr_f32(n, x.ctypes.data_as(fp), None, w2.ctypes.data_as(fp), flags, lanes)   # no qdd
+        np.testing.assert_array_equal(w2, w)
+        for i in range(0, n, 7):
+            _, m_ref = oracle.fd(x[i, :7].astype(np.float64), x[i, 7:14].astype(np.float64),
+                                 x[i, 14:].astype(np.float64), flags)
+            m = w[i, 98:].reshape(7, 7).T                                   # column-major -> [row][col]
+            np.testing.assert_allclose(m, m_ref, rtol=0, atol=1e-5 * np.abs(m_ref).max())
+
+
 @pytest.mark.parametrize("flags", [0, 1, 2])
 @pytest.mark.parametrize("per_knot", [False, True])
 def test_external_forces(host, oracle, flags, per_knot):
