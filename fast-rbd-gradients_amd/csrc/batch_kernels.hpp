@@ -188,16 +188,34 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
   const int role = lane % LANES;
   __shared__ typename P::Minv mi_sh[64 / LANES][kMinvPacked];   // Minv per lane group (LANES = 16)
 
+  // lane groups: the chunk's records (4 consecutive knots of one trajectory,
+  // 1,680 contiguous bytes) are assembled in LDS and written with consecutive
+  // lanes on consecutive floats: whole lines instead of each lane's 7-float
+  // column, which left every line partly written by each store (and, for
+  // zero-copy batches, one PCIe write per partial line)
+  __shared__ float rec_sh[LANES == 1 ? 1 : chunk * kOutPerKnot];
   auto process = [&](int c) {
     int t, first;                           // trajectory of chunk c, and its first chunk
     locate_chunk<SMALL>(descs, prefix, n_traj, chunk, c, lane, t, first);
     const TrajDesc d = descs[t];
-    const long k = long(c - first) * chunk + slot;
-    if (k < d.n) {
-      InAoS i{d.in + kInPerKnot * k};
-      OutAoS o{d.out + kOutPerKnot * k};
-      if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
-      else fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[slot]);
+    const long k0 = long(c - first) * chunk;
+    const long k = k0 + slot;
+    if constexpr (LANES == 1) {
+      if (k < d.n) {
+        InAoS i{d.in + kInPerKnot * k};
+        OutAoS o{d.out + kOutPerKnot * k};
+        fdgrad_knot<P>(i, o, opt);
+      }
+    } else {
+      if (k < d.n) {
+        InAoS i{d.in + kInPerKnot * k};
+        OutAoS o{rec_sh + kOutPerKnot * slot};
+        fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[slot]);
+      }
+      wave_lds_sync();
+      const int count = int(d.n - k0 < chunk ? d.n - k0 : chunk) * kOutPerKnot;
+      for (int e = lane; e < count; e += 64) d.out[kOutPerKnot * k0 + e] = rec_sh[e];
+      wave_lds_sync();                      // read before the next chunk overwrites it
     }
   };
 
@@ -259,21 +277,28 @@ __global__ __launch_bounds__(128, 1) void batch_two_wave_kernel(
   }
   __shared__ TwoWaveShared<P> sh[kTwoWaveKnots];
   __shared__ int flags[2];                 // [0]: bias ready (R -> M), [1]: Minv and corrections ready (M -> R)
+  // the chunk's records, assembled from both waves and written with
+  // consecutive threads on consecutive floats (as in batch_persistent_kernel)
+  __shared__ float rec_sh[kTwoWaveKnots * kOutPerKnot];
   const int lane = int(threadIdx.x % 64);
   const int g = lane / kLanesPerKnotGroup;
   for (int c = int(blockIdx.x); c < total; c += int(gridDim.x)) {
-    __syncthreads();                       // the previous chunk is done with sh and flags
+    __syncthreads();                       // the previous chunk is done with sh, flags and rec_sh
     if (threadIdx.x == 0) { flags[0] = 0; flags[1] = 0; }
     __syncthreads();
     int t, first;
     locate_chunk<SMALL>(descs, prefix, n_traj, kTwoWaveKnots, c, lane, t, first);
     const TrajDesc d = descs[t];           // d.n >= 1: chunk c lies in t
-    const long k = long(c - first) * kTwoWaveKnots + g;
+    const long k0 = long(c - first) * kTwoWaveKnots;
+    const long k = k0 + g;
     const bool valid = k < d.n;
     const long kk = valid ? k : d.n - 1;   // lanes past the trajectory's end compute a copy, store nothing
     InAoS i{d.in + kInPerKnot * kk};
-    OutMasked<OutAoS> o{OutAoS{d.out + kOutPerKnot * kk}, valid};
+    OutMasked<OutAoS> o{OutAoS{rec_sh + kOutPerKnot * g}, valid};
     two_wave_group<P>(i, o, opt, sh[g], flags);
+    __syncthreads();                       // both waves' parts of the records are in rec_sh
+    const int count = int(d.n - k0 < kTwoWaveKnots ? d.n - k0 : kTwoWaveKnots) * kOutPerKnot;
+    for (int e = int(threadIdx.x); e < count; e += int(blockDim.x)) d.out[kOutPerKnot * k0 + e] = rec_sh[e];
   }
   publish_completion(done, tail, 2u * gridDim.x, lane);
 }

@@ -131,7 +131,7 @@ struct Slot {
 constexpr size_t kTraceRecords = 65536;
 constexpr int kWarmRounds = 512;       // warm-up batches per slot at creation
 // zero_copy = 2: batches of more knots than this are copied (the kernels' PCIe
-// traffic caps zero-copy batches at about 3e7 knots/s; copied ones reach 1e8)
+// traffic caps zero-copy batches at about 4.7e7 knots/s; copied ones reach 1e8)
 constexpr int64_t kZeroCopyMaxKnots = 4096;
 constexpr size_t kDoneStride = 16;     // 64-bit words between slots' completion words (one 128-byte line each)
 // Zero-copy batches of up to this many chunks (waves) publish a completion

@@ -327,7 +327,7 @@ typedef struct rbd_server_config {
   int zero_copy;                  /* 0: requests' buffers in device memory, one upload and one download per batch;
                                      1: the kernels read inputs from and write results to the pinned staging
                                      directly (no copies: lower latency, PCIe traffic from the kernels, which caps
-                                     the server at about 3e7 knots/s);
+                                     the server at about 4.7e7 knots/s);
                                      2 (the default): per batch -- zero-copy up to zero_copy_max_knots knots,
                                      copies above */
   int poll_dispatch;              /* 1: the dispatcher thread polls for requests instead of sleeping (one busy
