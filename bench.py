@@ -257,7 +257,7 @@ def served_workload(ctx, rates=(1000, 3000, 10000, 30000, 100000, 300000), durat
     constants, admission by a fixed memory budget, continuous batching over
     4 in-flight slots, a polling dispatcher.  Latency = submit -> result in the
     caller's host buffer.  Three serving modes: requests' buffers in device
-    memory (copies), zero-copy, and zero-copy for batches up to 4096 knots with
+    memory (copies), zero-copy, and zero-copy for batches up to 16384 knots with
     copies above ("auto"); the two that copy are also run at 1M req/s, past
     what zero-copy batches sustain.  The automatic mode is run again with
     two-wave batches."""

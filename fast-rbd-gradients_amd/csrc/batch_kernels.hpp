@@ -193,7 +193,10 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
   // lanes on consecutive floats: whole lines instead of each lane's 7-float
   // column, which left every line partly written by each store (and, for
   // zero-copy batches, one PCIe write per partial line)
+  // (the chunk's inputs, 336 contiguous bytes, are read the same way into LDS
+  // first: a few whole-line reads instead of many small per-lane ones)
   __shared__ float rec_sh[LANES == 1 ? 1 : chunk * kOutPerKnot];
+  __shared__ float in_sh[LANES == 1 ? 1 : chunk * kInPerKnot];
   auto process = [&](int c) {
     int t, first;                           // trajectory of chunk c, and its first chunk
     locate_chunk<SMALL>(descs, prefix, n_traj, chunk, c, lane, t, first);
@@ -207,14 +210,16 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
         fdgrad_knot<P>(i, o, opt);
       }
     } else {
+      const int knots = int(d.n - k0 < chunk ? d.n - k0 : chunk);
+      for (int e = lane; e < knots * kInPerKnot; e += 64) in_sh[e] = d.in[kInPerKnot * k0 + e];
+      wave_lds_sync();
       if (k < d.n) {
-        InAoS i{d.in + kInPerKnot * k};
+        InAoS i{in_sh + kInPerKnot * slot};
         OutAoS o{rec_sh + kOutPerKnot * slot};
         fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[slot]);
       }
       wave_lds_sync();
-      const int count = int(d.n - k0 < chunk ? d.n - k0 : chunk) * kOutPerKnot;
-      for (int e = lane; e < count; e += 64) d.out[kOutPerKnot * k0 + e] = rec_sh[e];
+      for (int e = lane; e < knots * kOutPerKnot; e += 64) d.out[kOutPerKnot * k0 + e] = rec_sh[e];
       wave_lds_sync();                      // read before the next chunk overwrites it
     }
   };
@@ -280,6 +285,7 @@ __global__ __launch_bounds__(128, 1) void batch_two_wave_kernel(
   // the chunk's records, assembled from both waves and written with
   // consecutive threads on consecutive floats (as in batch_persistent_kernel)
   __shared__ float rec_sh[kTwoWaveKnots * kOutPerKnot];
+  __shared__ float in_sh[kTwoWaveKnots * kInPerKnot];   // the chunk's inputs, read the same way
   const int lane = int(threadIdx.x % 64);
   const int g = lane / kLanesPerKnotGroup;
   for (int c = int(blockIdx.x); c < total; c += int(gridDim.x)) {
@@ -292,13 +298,17 @@ __global__ __launch_bounds__(128, 1) void batch_two_wave_kernel(
     const long k0 = long(c - first) * kTwoWaveKnots;
     const long k = k0 + g;
     const bool valid = k < d.n;
-    const long kk = valid ? k : d.n - 1;   // lanes past the trajectory's end compute a copy, store nothing
-    InAoS i{d.in + kInPerKnot * kk};
+    const int knots = int(d.n - k0 < kTwoWaveKnots ? d.n - k0 : kTwoWaveKnots);
+    for (int e = int(threadIdx.x); e < knots * kInPerKnot; e += int(blockDim.x))
+      in_sh[e] = d.in[kInPerKnot * k0 + e];
+    __syncthreads();
+    // lanes past the trajectory's end compute a copy of its last knot and store nothing
+    InAoS i{in_sh + kInPerKnot * (valid ? g : knots - 1)};
     OutMasked<OutAoS> o{OutAoS{rec_sh + kOutPerKnot * g}, valid};
     two_wave_group<P>(i, o, opt, sh[g], flags);
     __syncthreads();                       // both waves' parts of the records are in rec_sh
-    const int count = int(d.n - k0 < kTwoWaveKnots ? d.n - k0 : kTwoWaveKnots) * kOutPerKnot;
-    for (int e = int(threadIdx.x); e < count; e += int(blockDim.x)) d.out[kOutPerKnot * k0 + e] = rec_sh[e];
+    for (int e = int(threadIdx.x); e < knots * kOutPerKnot; e += int(blockDim.x))
+      d.out[kOutPerKnot * k0 + e] = rec_sh[e];
   }
   publish_completion(done, tail, 2u * gridDim.x, lane);
 }

@@ -130,9 +130,10 @@ struct Slot {
 
 constexpr size_t kTraceRecords = 65536;
 constexpr int kWarmRounds = 512;       // warm-up batches per slot at creation
-// zero_copy = 2: batches of more knots than this are copied (the kernels' PCIe
-// traffic caps zero-copy batches at about 4.7e7 knots/s; copied ones reach 1e8)
-constexpr int64_t kZeroCopyMaxKnots = 4096;
+// zero_copy = 2: batches of more knots than this are copied.  Measured at 1M
+// req/s on MI355X: 9.3e7-9.4e7 knots/s with p50 1.2 ms at this threshold;
+// 5e7-9.6e7 (the runtime's copy stalls) at 4096, 7e7-8.6e7 with no copies.
+constexpr int64_t kZeroCopyMaxKnots = 16384;
 constexpr size_t kDoneStride = 16;     // 64-bit words between slots' completion words (one 128-byte line each)
 // Zero-copy batches of up to this many chunks (waves) publish a completion
 // word.  Every wave of such a launch writes back its L2 and waits before it is
@@ -486,7 +487,7 @@ void rbd_server_config_default(rbd_server_config* cfg) {
   cfg->max_batch_knots = 1 << 15;
   cfg->batch_window_us = 0.0;
   cfg->use_graph = 0;
-  cfg->zero_copy = 2;                         // zero-copy for batches up to 4096 knots, copies above
+  cfg->zero_copy = 2;                         // zero-copy for batches up to 16384 knots, copies above
   cfg->lanes_per_knot = rbd::kLanesPerKnotGroup;
 }
 

@@ -332,7 +332,7 @@ typedef struct rbd_server_config {
                                      copies above */
   int poll_dispatch;              /* 1: the dispatcher thread polls for requests instead of sleeping (one busy
                                      core; saves the ~10 us wake-up of a sleeping thread per batch) */
-  int zero_copy_max_knots;        /* zero_copy = 2: the largest zero-copy batch (0: 4096 knots) */
+  int zero_copy_max_knots;        /* zero_copy = 2: the largest zero-copy batch (0: 16384 knots) */
 } rbd_server_config;
 
 typedef struct rbd_server_stats {
