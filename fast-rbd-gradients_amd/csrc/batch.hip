@@ -36,6 +36,11 @@ struct rbd_batch {
   unsigned grid[3] = {0, 0, 0};       // waves: [0] one lane per knot, [1] lane groups; [2] two-wave workgroups
   hipGraphExec_t exec[6] = {};
   hipGraph_t graph[6] = {};
+  // batches of up to kSmallBatch trajectories replay a graph of one kernel
+  // node per mapping, whose arguments and grid are set before each replay
+  hipGraph_t arg_graph[3] = {};
+  hipGraphExec_t arg_exec[3] = {};
+  hipGraphNode_t arg_node[3] = {};
   int64_t last_total_knots = 0;
   int last_lanes = 0;
 };
@@ -90,6 +95,70 @@ void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
     hipLaunchKernelGGL((rbd::batch_persistent_kernel<P, LANES, false>), dim3(grid), dim3(kBlock), 0, s, q.descs,
                        q.hdr, rbd::NoArgBatch{}, b->d_tail, q.prefix, b->ctx->opt, q.done);
   }
+}
+
+// the small-batch kernel (descriptors in the arguments) of a mapping, and its workgroup size
+template <class P>
+void* small_kernel(int lanes, unsigned* block) {
+  if (lanes == 1) { *block = kBlock; return reinterpret_cast<void*>(&rbd::batch_persistent_kernel<P, 1, true>); }
+  if (lanes == rbd::kLanesPerKnotGroup) {
+    *block = kBlock;
+    return reinterpret_cast<void*>(&rbd::batch_persistent_kernel<P, rbd::kLanesPerKnotGroup, true>);
+  }
+  *block = 2 * kBlock;
+  return reinterpret_cast<void*>(&rbd::batch_two_wave_kernel<P, true>);
+}
+
+void* small_kernel_of(int precision, int lanes, unsigned* block) {
+  switch (precision) {
+    case RBD_PREC_FP32: return small_kernel<rbd::PrecFP32>(lanes, block);
+    case RBD_PREC_FP32_HW_TRIG: return small_kernel<rbd::PrecFP32HwTrig>(lanes, block);
+    case RBD_PREC_FP64_PIVOT: return small_kernel<rbd::PrecFP64Pivot>(lanes, block);
+    case RBD_PREC_FP64_COLUMNS: return small_kernel<rbd::PrecFP64Col>(lanes, block);
+    default: return small_kernel<rbd::PrecFP64>(lanes, block);
+  }
+}
+
+// A small batch replayed from its mapping's one-node graph: the batch goes in
+// the node's arguments and the grid is cut to the chunk count, set before the
+// replay (no descriptor upload, no idle workgroups); the graph is built on
+// first use.
+hipError_t replay_small(rbd_batch* b, const rbd::ArgBatch& a, int64_t chunks, int lanes, hipStream_t st) {
+  const int mi = mapping_of(lanes);
+  unsigned grid = b->grid[mi];
+  if (chunks > 0 && chunks < int64_t(grid)) grid = unsigned(chunks);
+  const rbd::TrajDesc* descs = nullptr;        // the kernel takes the batch from its arguments
+  const rbd::BatchHeader* hdr = b->d_hdr;
+  unsigned long long* tail = b->d_tail;
+  const long* prefix = b->d_prefix;
+  rbd::Options opt = b->ctx->opt;
+  rbd::Completion done{nullptr, 0};
+  rbd::ArgBatch arg = a;
+  void* args[] = {&descs, &hdr, &arg, &tail, &prefix, &opt, &done};
+  hipKernelNodeParams p{};
+  unsigned block = 0;
+  p.func = small_kernel_of(b->ctx->precision, lanes, &block);
+  p.blockDim = dim3(block);
+  p.gridDim = dim3(grid);
+  p.kernelParams = args;
+  p.extra = nullptr;
+  p.sharedMemBytes = 0;
+  hipError_t e = hipSuccess;
+  if (!b->arg_exec[mi]) {
+    e = hipGraphCreate(&b->arg_graph[mi], 0);
+    if (e == hipSuccess) e = hipGraphAddKernelNode(&b->arg_node[mi], b->arg_graph[mi], nullptr, 0, &p);
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->arg_exec[mi], b->arg_graph[mi], nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      if (b->arg_graph[mi]) (void)hipGraphDestroy(b->arg_graph[mi]);
+      b->arg_graph[mi] = nullptr;
+      b->arg_exec[mi] = nullptr;
+      return e;
+    }
+  } else {
+    e = hipGraphExecKernelNodeSetParams(b->arg_exec[mi], b->arg_node[mi], &p);
+    if (e != hipSuccess) return e;
+  }
+  return hipGraphLaunch(b->arg_exec[mi], st);
 }
 
 template <class P>
@@ -217,6 +286,10 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
 rbd_status rbd_batch_destroy(rbd_batch* b) {
   if (!b) return RBD_OK;
   (void)use_device(b->ctx);
+  for (int mi = 0; mi < 3; ++mi) {
+    if (b->arg_exec[mi]) (void)hipGraphExecDestroy(b->arg_exec[mi]);
+    if (b->arg_graph[mi]) (void)hipGraphDestroy(b->arg_graph[mi]);
+  }
   for (int gi = 0; gi < 6; ++gi) {
     if (b->exec[gi]) (void)hipGraphExecDestroy(b->exec[gi]);
     if (b->graph[gi]) (void)hipGraphDestroy(b->graph[gi]);
@@ -278,7 +351,7 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   const int lanes = lanes_req ? lanes_req : lanes_for_batch(b->ctx, total);
   b->last_lanes = lanes;
   const bool small = n_traj <= rbd::kSmallBatch;
-  if (small && !use_graph) {
+  if (small) {
     // the descriptors go in the kernel arguments: no upload
     rbd::ArgBatch a{};
     const long chunk = chunk_knots(lanes);
@@ -289,6 +362,11 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
     }
     a.n_traj = n_traj;
     a.total_chunks = int(chunks);
+    if (use_graph) {
+      hipError_t ge = replay_small(b, a, chunks, lanes, st);
+      if (ge != hipSuccess) return hip_fail(ge, "batch graph (small batch)");
+      return RBD_OK;
+    }
     Queue q{nullptr, nullptr, b->d_prefix};
     q.args = &a;
     q.chunks = chunks;
