@@ -73,8 +73,9 @@ long chunk_knots(int lanes) { return lanes == 1 ? 64 : rbd::kTwoWaveKnots; }
 template <class P, int LANES, bool SMALL>
 void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
   // a batch of fewer chunks than the persistent grid launches one workgroup
-  // per chunk (fewer to dispatch); graphs keep the full grid, whose captured
-  // launch serves every batch
+  // per chunk (fewer to dispatch); the captured graphs of large batches keep
+  // the full grid, whose launch serves every batch (small batches' graphs set
+  // theirs per replay, replay_small)
   unsigned grid = b->grid[mapping_of(LANES)];
   if (q.chunks > 0 && q.chunks < int64_t(grid)) grid = unsigned(q.chunks);
   if constexpr (LANES == kTwoWaveLanes) {

@@ -281,8 +281,9 @@ rbd_status rbd_batch_destroy(rbd_batch* b);
  * chosen from the total knot count against the context's batch crossover
  * (rbd_autotune_batch), unless the context fixes lanes_per_knot (32: the
  * two-wave mapping, 4-knot chunks on workgroups of two waves).
- * use_graph != 0 replays the batch's kernels from a hipGraph captured on first
- * use (one graph per mapping and batch kind).  Calls on one batch object are
+ * use_graph != 0 replays the batch's kernels from a hipGraph built on first
+ * use (one graph per mapping and batch kind; for up to 64 trajectories a
+ * single kernel node whose arguments and grid are set before each replay).  Calls on one batch object are
  * ordered by their streams: two streams must not run the same batch object at
  * once.  Returns RBD_ERR_CAPACITY if n_traj exceeds the batch's
  * max_trajectories. */
