@@ -33,14 +33,20 @@ struct rbd_batch {
   int ring = 0;
   // persistent grid per mapping, and the captured graphs per (mapping, small
   // batch): index 2 * mapping + small
-  unsigned grid[3] = {0, 0, 0};       // waves: [0] one lane per knot, [1] lane groups; [2] two-wave workgroups
-  hipGraphExec_t exec[6] = {};
-  hipGraph_t graph[6] = {};
-  // batches of up to kSmallBatch trajectories replay a graph of one kernel
-  // node per mapping, whose arguments and grid are set before each replay
-  hipGraph_t arg_graph[3] = {};
-  hipGraphExec_t arg_exec[3] = {};
-  hipGraphNode_t arg_node[3] = {};
+  static constexpr int kMappings = 6;
+  // waves: [0] one lane per knot, [1] 16-lane groups, [2] two-wave workgroups,
+  // [3], [4], [5] groups of 4, 7 and 8 lanes
+  unsigned grid[kMappings] = {};
+  hipGraphExec_t exec[2 * kMappings] = {};
+  hipGraph_t graph[2 * kMappings] = {};
+  // batches of up to kSmallBatch trajectories replay, per mapping, a graph
+  // captured with one trajectory list in its kernel's arguments (arg_key): the
+  // same list again replays it as it is; another list is captured anew, once
+  // the old graph's replays (arg_done) have finished
+  hipGraph_t arg_graph[kMappings] = {};
+  hipGraphExec_t arg_exec[kMappings] = {};
+  hipEvent_t arg_done[kMappings] = {};
+  std::vector<rbd_traj> arg_key[kMappings];
   int64_t last_total_knots = 0;
   int last_lanes = 0;
 };
@@ -66,9 +72,35 @@ struct Queue {
 // the batch's own header and descriptors, uploaded per launch
 Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, b->d_prefix}; }
 
-// mapping index (0 one lane, 1 lane groups, 2 two waves) and knots per chunk
-int mapping_of(int lanes) { return lanes == 1 ? 0 : (lanes == rbd::kLanesPerKnotGroup ? 1 : 2); }
-long chunk_knots(int lanes) { return lanes == 1 ? 64 : rbd::kTwoWaveKnots; }
+// mapping index (0 one lane, 1 16-lane groups, 2 two waves, 3/4/5 groups of
+// 4/7/8 lanes) and knots per chunk
+int mapping_of(int lanes) {
+  switch (lanes) {
+    case 1: return 0;
+    case rbd::kLanesPerKnotGroup: return 1;
+    case 4: return 3;
+    case 7: return 4;
+    case 8: return 5;
+    default: return 2;
+  }
+}
+long chunk_knots(int lanes) {
+  return lanes == 1 ? 64 : is_group_lanes(lanes) ? 64 / lanes : rbd::kTwoWaveKnots;
+}
+// The chunk total of a batch.  The L-lane groups take chunks of the batch's
+// flattened knots (across trajectories); the other mappings chunk each
+// trajectory on its own.
+template <class Len>
+int64_t chunk_total(int n_traj, Len len, int lanes) {
+  const int64_t chunk = chunk_knots(lanes);
+  int64_t total = 0;
+  if (is_group_lanes(lanes)) {
+    for (int t = 0; t < n_traj; ++t) total += len(t);
+    return (total + chunk - 1) / chunk;
+  }
+  for (int t = 0; t < n_traj; ++t) total += (len(t) + chunk - 1) / chunk;
+  return total;
+}
 
 template <class P, int LANES, bool SMALL>
 void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
@@ -78,7 +110,16 @@ void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
   // theirs per replay, replay_small)
   unsigned grid = b->grid[mapping_of(LANES)];
   if (q.chunks > 0 && q.chunks < int64_t(grid)) grid = unsigned(q.chunks);
-  if constexpr (LANES == kTwoWaveLanes) {
+  if constexpr (LANES == 4 || LANES == 7 || LANES == 8) {
+    static const rbd::ArgBatch kNone{};
+    const rbd::ArgBatch& a = q.args ? *q.args : kNone;
+    if constexpr (SMALL)
+      hipLaunchKernelGGL((rbd::batch_group_kernel<P, LANES, true>), dim3(grid), dim3(kBlock), 0, s,
+                         q.args ? nullptr : q.descs, q.hdr, a, b->d_tail, q.prefix, b->ctx->opt, q.done);
+    else
+      hipLaunchKernelGGL((rbd::batch_group_kernel<P, LANES, false>), dim3(grid), dim3(kBlock), 0, s, q.descs,
+                         q.hdr, rbd::NoArgBatch{}, b->d_tail, q.prefix, b->ctx->opt, q.done);
+  } else if constexpr (LANES == kTwoWaveLanes) {
     static const rbd::ArgBatch kNone{};
     const rbd::ArgBatch& a = q.args ? *q.args : kNone;
     if constexpr (SMALL)
@@ -98,68 +139,63 @@ void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
   }
 }
 
-// the small-batch kernel (descriptors in the arguments) of a mapping, and its workgroup size
-template <class P>
-void* small_kernel(int lanes, unsigned* block) {
-  if (lanes == 1) { *block = kBlock; return reinterpret_cast<void*>(&rbd::batch_persistent_kernel<P, 1, true>); }
-  if (lanes == rbd::kLanesPerKnotGroup) {
-    *block = kBlock;
-    return reinterpret_cast<void*>(&rbd::batch_persistent_kernel<P, rbd::kLanesPerKnotGroup, true>);
-  }
-  *block = 2 * kBlock;
-  return reinterpret_cast<void*>(&rbd::batch_two_wave_kernel<P, true>);
-}
+// Forward declaration: the direct launch, which the small-batch graphs capture.
+hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hipStream_t s);
 
-void* small_kernel_of(int precision, int lanes, unsigned* block) {
-  switch (precision) {
-    case RBD_PREC_FP32: return small_kernel<rbd::PrecFP32>(lanes, block);
-    case RBD_PREC_FP32_HW_TRIG: return small_kernel<rbd::PrecFP32HwTrig>(lanes, block);
-    case RBD_PREC_FP64_PIVOT: return small_kernel<rbd::PrecFP64Pivot>(lanes, block);
-    case RBD_PREC_FP64_COLUMNS: return small_kernel<rbd::PrecFP64Col>(lanes, block);
-    default: return small_kernel<rbd::PrecFP64>(lanes, block);
-  }
-}
-
-// A small batch replayed from its mapping's one-node graph: the batch goes in
-// the node's arguments and the grid is cut to the chunk count, set before the
-// replay (no descriptor upload, no idle workgroups); the graph is built on
-// first use.
-hipError_t replay_small(rbd_batch* b, const rbd::ArgBatch& a, int64_t chunks, int lanes, hipStream_t st) {
+// A small batch replayed from its mapping's graph.  The graph holds the
+// batch's one kernel with the trajectory list in its arguments and its grid
+// cut to the chunk count, as a direct launch has them, so a replay of the same
+// list is one hipGraphLaunch.  Another list is captured into a new graph: the
+// arguments of an instantiated graph could be rewritten instead
+// (hipGraphExecKernelNodeSetParams), but on ROCm 7.2 every such update keeps
+// about 34 KB of device memory until the graph is destroyed and costs about
+// 8 us of host time.
+hipError_t replay_small(rbd_batch* b, const rbd_traj* trajs, int n_traj, const rbd::ArgBatch& a, int64_t chunks,
+                        int lanes, hipStream_t st) {
   const int mi = mapping_of(lanes);
-  unsigned grid = b->grid[mi];
-  if (chunks > 0 && chunks < int64_t(grid)) grid = unsigned(chunks);
-  const rbd::TrajDesc* descs = nullptr;        // the kernel takes the batch from its arguments
-  const rbd::BatchHeader* hdr = b->d_hdr;
-  unsigned long long* tail = b->d_tail;
-  const long* prefix = b->d_prefix;
-  rbd::Options opt = b->ctx->opt;
-  rbd::Completion done{nullptr, 0};
-  rbd::ArgBatch arg = a;
-  void* args[] = {&descs, &hdr, &arg, &tail, &prefix, &opt, &done};
-  hipKernelNodeParams p{};
-  unsigned block = 0;
-  p.func = small_kernel_of(b->ctx->precision, lanes, &block);
-  p.blockDim = dim3(block);
-  p.gridDim = dim3(grid);
-  p.kernelParams = args;
-  p.extra = nullptr;
-  p.sharedMemBytes = 0;
+  std::vector<rbd_traj>& key = b->arg_key[mi];
+  const bool same = b->arg_exec[mi] && int(key.size()) == n_traj &&
+                    std::memcmp(key.data(), trajs, sizeof(rbd_traj) * size_t(n_traj)) == 0;
   hipError_t e = hipSuccess;
-  if (!b->arg_exec[mi]) {
-    e = hipGraphCreate(&b->arg_graph[mi], 0);
-    if (e == hipSuccess) e = hipGraphAddKernelNode(&b->arg_node[mi], b->arg_graph[mi], nullptr, 0, &p);
+  if (!same) {
+    if (b->arg_exec[mi]) {
+      e = hipEventSynchronize(b->arg_done[mi]);        // the old graph's last replay has finished
+      (void)hipGraphExecDestroy(b->arg_exec[mi]);
+      (void)hipGraphDestroy(b->arg_graph[mi]);
+      b->arg_exec[mi] = nullptr;
+      b->arg_graph[mi] = nullptr;
+      key.clear();
+      if (e != hipSuccess) return e;
+    }
+    if (!b->arg_done[mi]) {
+      e = hipEventCreateWithFlags(&b->arg_done[mi], hipEventDisableTiming);
+      if (e != hipSuccess) return e;
+    }
+    hipStream_t cs = nullptr;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      Queue q{nullptr, nullptr, b->d_prefix};
+      q.args = &a;
+      q.chunks = chunks;
+      const hipError_t le = launch_batch(b, q, lanes, true, cs);
+      e = hipStreamEndCapture(cs, &b->arg_graph[mi]);
+      if (le != hipSuccess) e = le;
+    }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->arg_exec[mi], b->arg_graph[mi], nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
     if (e != hipSuccess) {
       if (b->arg_graph[mi]) (void)hipGraphDestroy(b->arg_graph[mi]);
       b->arg_graph[mi] = nullptr;
       b->arg_exec[mi] = nullptr;
       return e;
     }
-  } else {
-    e = hipGraphExecKernelNodeSetParams(b->arg_exec[mi], b->arg_node[mi], &p);
-    if (e != hipSuccess) return e;
+    key.assign(trajs, trajs + n_traj);
   }
-  return hipGraphLaunch(b->arg_exec[mi], st);
+  e = hipGraphLaunch(b->arg_exec[mi], st);
+  if (e == hipSuccess) e = hipEventRecord(b->arg_done[mi], st);
+  return e;
 }
 
 template <class P>
@@ -171,6 +207,12 @@ hipError_t launch_batch_p(rbd_batch* b, const Queue& q, int lanes, bool small, h
   } else if (lanes == kTwoWaveLanes) {
     if (small) launch_persistent<P, kTwoWaveLanes, true>(b, q, s);
     else launch_persistent<P, kTwoWaveLanes, false>(b, q, s);
+  } else if (lanes == 4) {
+    if (small) launch_persistent<P, 4, true>(b, q, s); else launch_persistent<P, 4, false>(b, q, s);
+  } else if (lanes == 7) {
+    if (small) launch_persistent<P, 7, true>(b, q, s); else launch_persistent<P, 7, false>(b, q, s);
+  } else if (lanes == 8) {
+    if (small) launch_persistent<P, 8, true>(b, q, s); else launch_persistent<P, 8, false>(b, q, s);
   } else {
     if (small) launch_persistent<P, rbd::kLanesPerKnotGroup, true>(b, q, s);
     else launch_persistent<P, rbd::kLanesPerKnotGroup, false>(b, q, s);
@@ -193,15 +235,13 @@ hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hip
 long stage(char* dst, const rbd_traj* trajs, int n_traj, int lanes) {
   rbd::BatchHeader* h = reinterpret_cast<rbd::BatchHeader*>(dst);
   std::memset(h, 0, sizeof(*h));
-  const long chunk = chunk_knots(lanes);
-  long total = 0;
   rbd::TrajDesc* d = reinterpret_cast<rbd::TrajDesc*>(dst + kHeaderBytes);
-  for (int t = 0; t < n_traj; ++t) {
-    d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
-    total += (long(trajs[t].n_knots) + chunk - 1) / chunk;
-  }
+  for (int t = 0; t < n_traj; ++t) d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
+  const long total = long(chunk_total(n_traj, [trajs](int t) { return int64_t(trajs[t].n_knots); }, lanes));
   h->n_traj = n_traj;
-  h->chunk_knots = int(chunk);
+  // the plan kernel's prefix sum counts chunks of this many knots per
+  // trajectory: single knots for the L-lane groups, whose chunks cross trajectories
+  h->chunk_knots = is_group_lanes(lanes) ? 1 : int(chunk_knots(lanes));
   h->total_chunks = total;
   return total;
 }
@@ -280,6 +320,7 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
                      ctx->precision == RBD_PREC_FP32_HW_TRIG) ? 2 : 1;
   b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4 * waves);
   b->grid[2] = unsigned(ctx->compute_units * 2);   // two-wave workgroups (256 registers: one wave per SIMD)
+  b->grid[3] = b->grid[4] = b->grid[5] = unsigned(ctx->compute_units * 4);   // L-lane groups: one wave per SIMD
   *out = b;
   return RBD_OK;
 }
@@ -287,11 +328,12 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
 rbd_status rbd_batch_destroy(rbd_batch* b) {
   if (!b) return RBD_OK;
   (void)use_device(b->ctx);
-  for (int mi = 0; mi < 3; ++mi) {
+  for (int mi = 0; mi < rbd_batch::kMappings; ++mi) {
+    if (b->arg_done[mi]) { (void)hipEventSynchronize(b->arg_done[mi]); (void)hipEventDestroy(b->arg_done[mi]); }
     if (b->arg_exec[mi]) (void)hipGraphExecDestroy(b->arg_exec[mi]);
     if (b->arg_graph[mi]) (void)hipGraphDestroy(b->arg_graph[mi]);
   }
-  for (int gi = 0; gi < 6; ++gi) {
+  for (int gi = 0; gi < 2 * rbd_batch::kMappings; ++gi) {
     if (b->exec[gi]) (void)hipGraphExecDestroy(b->exec[gi]);
     if (b->graph[gi]) (void)hipGraphDestroy(b->graph[gi]);
   }
@@ -307,10 +349,10 @@ rbd_status rbd_batch_destroy(rbd_batch* b) {
 
 rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves) {
   if (!b) return fail(RBD_ERR_INVALID_ARGUMENT, "null batch");
-  if (lanes_per_knot != 1 && lanes_per_knot != rbd::kLanesPerKnotGroup)
-    return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 1 or 16");
+  if (lanes_per_knot != 1 && lanes_per_knot != rbd::kLanesPerKnotGroup && !is_group_lanes(lanes_per_knot))
+    return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 1, 4, 7, 8 or 16");
   if (waves == 0 || waves > (1u << 20)) return fail(RBD_ERR_INVALID_ARGUMENT, "waves must be in [1, 2^20]");
-  const int mi = lanes_per_knot == 1 ? 0 : 1;
+  const int mi = mapping_of(lanes_per_knot);
   if (b->grid[mi] != waves) {
     b->grid[mi] = waves;
     for (int small = 0; small < 2; ++small) {
@@ -347,7 +389,7 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   if (max_chunks > kMaxBatchChunks) return fail(RBD_ERR_CAPACITY, "batch too large (more than 2^31 chunks)");
   b->last_total_knots = total;
   if (total == 0) return RBD_OK;
-  if (rbd_status s = use_device(b->ctx)) return s;
+  if (rbd_status s = ready_to_launch(b->ctx)) return s;
   const hipStream_t st = (hipStream_t)stream;
   const int lanes = lanes_req ? lanes_req : lanes_for_batch(b->ctx, total);
   b->last_lanes = lanes;
@@ -355,16 +397,12 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   if (small) {
     // the descriptors go in the kernel arguments: no upload
     rbd::ArgBatch a{};
-    const long chunk = chunk_knots(lanes);
-    long chunks = 0;
-    for (int t = 0; t < n_traj; ++t) {
-      a.d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
-      chunks += (long(trajs[t].n_knots) + chunk - 1) / chunk;
-    }
+    for (int t = 0; t < n_traj; ++t) a.d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
+    const long chunks = long(chunk_total(n_traj, [trajs](int t) { return int64_t(trajs[t].n_knots); }, lanes));
     a.n_traj = n_traj;
     a.total_chunks = int(chunks);
     if (use_graph) {
-      hipError_t ge = replay_small(b, a, chunks, lanes, st);
+      hipError_t ge = replay_small(b, trajs, n_traj, a, chunks, lanes, st);
       if (ge != hipSuccess) return hip_fail(ge, "batch graph (small batch)");
       return RBD_OK;
     }
@@ -434,17 +472,19 @@ extern "C" {
 
 int rbd_batch_last_lanes(const rbd_batch* b) { return b ? b->last_lanes : 0; }
 
-// Times both mappings of the batched path on batches of 132-knot trajectories
-// (the mean length of lengths uniform over 8..256) at 8 ... 512 trajectories
-// and sets the total at which the lane-group mapping stops being faster,
-// interpolated between the two sizes that bracket it.
+// Times every mapping of the batched path (kAutotuneLanes) on batches of
+// 132-knot trajectories (the mean length of lengths uniform over 8..256), 8 to
+// 512 of them, and fills the context's batch mapping table with the fastest
+// at each total; the count 69 (9,108 knots) sits just under one wave of
+// 7-lane groups per SIMD on 256 CUs.
 rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                               int table_len) {
   if (rbd_status s = check_ctx(ctx)) return s;
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   constexpr int64_t kLen = 132;
-  static const int counts[] = {8, 16, 32, 48, 64, 96, 128, 256, 512};
+  static const int counts[] = {8, 16, 32, 48, 64, 69, 80, 96, 128, 256, 512};
   constexpr int kSizes = sizeof(counts) / sizeof(counts[0]);
+  static_assert(kSizes <= kMaxTuned, "mapping table size");
   const int max_traj = counts[kSizes - 1];
   const size_t knots = size_t(max_traj) * kLen;
   const hipStream_t st = (hipStream_t)stream;
@@ -462,10 +502,10 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (e == hipSuccess && rs == RBD_OK) e = hipEventCreate(&t0);
   if (e == hipSuccess && rs == RBD_OK) e = hipEventCreate(&t1);
-  float best_us[2][kSizes] = {};
+  float best_us[kAutotuneMappings][kSizes] = {};
   for (int si = 0; si < kSizes && e == hipSuccess && rs == RBD_OK; ++si) {
-    for (int li = 0; li < 2 && e == hipSuccess && rs == RBD_OK; ++li) {
-      const int lanes = li ? rbd::kLanesPerKnotGroup : 1;
+    for (int li = 0; li < kAutotuneMappings && e == hipSuccess && rs == RBD_OK; ++li) {
+      const int lanes = kAutotuneLanes[li];
       float best = 1e30f;
       for (int rep = 0; rep < 3 && e == hipSuccess && rs == RBD_OK; ++rep) {
         rs = rbdi::batch_run(b, trajs.data(), counts[si], 0, stream, lanes);   // warm
@@ -481,18 +521,6 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
       best_us[li][si] = best;
     }
   }
-  // crossover: interpolated where the lane-group mapping stops winning
-  int64_t threshold = 0;
-  for (int si = 0; si < kSizes; ++si) {
-    const double d = double(best_us[0][si]) - double(best_us[1][si]);   // > 0: lane groups faster
-    if (d > 0) { threshold = counts[si] * kLen; continue; }
-    if (si > 0 && threshold == counts[si - 1] * kLen) {
-      const double d0 = double(best_us[0][si - 1]) - double(best_us[1][si - 1]);
-      const double n0 = double(counts[si - 1] * kLen), n1 = double(counts[si] * kLen);
-      threshold = int64_t(n0 + (n1 - n0) * d0 / (d0 - d));
-    }
-    break;
-  }
   (void)hipStreamSynchronize(st);
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);
@@ -500,11 +528,26 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
   (void)hipFree(buf);
   if (rs != RBD_OK) return rs;
   if (e != hipSuccess) return hip_fail(e, "batch autotune");
+  // the table: the fastest mapping at each total; the crossover between one
+  // lane and 16-lane groups keeps its meaning (the largest total at which the
+  // 16-lane groups still win)
+  int64_t threshold = 0;
+  ctx->batch_tuned.count.store(0, std::memory_order_release);
+  for (int si = 0; si < kSizes; ++si) {
+    int bi = 0;
+    for (int li = 1; li < kAutotuneMappings; ++li)
+      if (best_us[li][si] < best_us[bi][si]) bi = li;
+    ctx->batch_tuned.n[si] = counts[si] * kLen;
+    ctx->batch_tuned.lanes[si] = kAutotuneLanes[bi];
+    if (best_us[1][si] < best_us[0][si]) threshold = counts[si] * kLen;
+  }
+  ctx->batch_tuned.count.store(kSizes, std::memory_order_release);
   ctx->batch_group_threshold.store(threshold);
   if (threshold_out) *threshold_out = threshold;
+  constexpr int kRow = 1 + kAutotuneMappings;
   if (table_us)
-    for (int i = 0; i < table_len && i < 3 * kSizes; ++i)
-      table_us[i] = (i % 3 == 0) ? float(counts[i / 3] * kLen) : best_us[(i % 3) - 1][i / 3];
+    for (int i = 0; i < table_len && i < kRow * kSizes; ++i)
+      table_us[i] = (i % kRow == 0) ? float(counts[i / kRow] * kLen) : best_us[(i % kRow) - 1][i / kRow];
   return RBD_OK;
 }
 
@@ -513,6 +556,7 @@ int64_t rbd_batch_group_threshold(rbd_context* ctx) { return ctx ? ctx->batch_gr
 rbd_status rbd_set_batch_group_threshold(rbd_context* ctx, int64_t n_knots) {
   if (rbd_status s = check_ctx(ctx)) return s;
   if (n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative threshold");
+  ctx->batch_tuned.count.store(0, std::memory_order_release);   // set by hand: the threshold decides
   ctx->batch_group_threshold.store(n_knots);
   return RBD_OK;
 }

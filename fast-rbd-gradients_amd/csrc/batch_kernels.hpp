@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include "fdgrad_knot.hpp"
+#include "group_mapping.hpp"
 #include "two_wave.hpp"
 
 namespace rbd {
@@ -139,19 +140,59 @@ __device__ inline void locate_chunk(const TrajDesc* descs, const long* prefix, i
 // wave's stores (zero-copy outputs in host memory included) are written back
 // and acknowledged at system scope before the wave is counted (a release
 // fence: no cache invalidation, which would evict the data of other launches
-// running beside this one); the last of the launch's `waves` waves to be
-// counted has seen every other count, so every output is visible when it
-// resets the count for the next launch and publishes seq with a release store.
+// running beside this one); the count is an acquire-release add, so the last
+// of the launch's `waves` waves to be counted acquires every other wave's
+// release, and every output is visible when it resets the count for the next
+// launch and publishes seq with a release store.
 __device__ inline void publish_completion(const Completion& done, unsigned long long* tail, unsigned waves,
                                           int lane) {
   if (done.flag == nullptr) return;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
   if (lane == 0) {
     unsigned long long* fin = tail + kDoneCounter;
-    if (atomicAdd(fin, 1ull) == static_cast<unsigned long long>(waves - 1)) {
+    if (__hip_atomic_fetch_add(fin, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+        static_cast<unsigned long long>(waves - 1)) {
       atomicExch(fin, 0ull);
       __hip_atomic_store(done.flag, done.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+  }
+}
+
+// The calling wave's share of a launch's `total` chunks, handed to process(c)
+// one at a time: the full rounds (all chunks when they fit one round) in a
+// fixed order with no atomics, then the last partial round through the tail
+// counters (tail: the kTailCounters counters kTailStride words apart, zero on
+// entry and left at zero).  One wave per workgroup.
+template <class F>
+__device__ inline void drain_chunks(int total, unsigned long long* tail, int lane, F&& process) {
+  const int G = int(gridDim.x);
+  const int w = int(blockIdx.x);
+  const int fixed = total <= G ? total : total - total % G;
+  const int rest = total - fixed;
+  const int S = G < kTailCounters ? G : kTailCounters;
+  const int s = w % S;
+  const int mine = s < rest ? (rest - 1 - s) / S + 1 : 0;     // tail chunks dealt to counter s
+  const int waves = (G - 1 - s) / S + 1;                      // waves that claim from counter s
+  unsigned long long* ctr = tail + s * kTailStride;
+  int next = w;
+  for (;;) {
+    int c;
+    if (next < fixed) {
+      c = next;
+      next += G;
+    } else {
+      if (rest == 0) break;
+      int j = 0;
+      if (lane == 0) {
+        j = int(atomicAdd(ctr, 1ull));
+        // the launch's last claim on this counter: back to zero for the next launch
+        if (j == mine + waves - 1) atomicExch(ctr, 0ull);
+      }
+      j = __builtin_amdgcn_readfirstlane(j);
+      if (j >= mine) break;
+      c = fixed + s + j * S;
+    }
+    process(c);
   }
 }
 
@@ -181,8 +222,6 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
     n_traj = hdr->n_traj;
     total = int(hdr->total_chunks);
   }
-  const int G = int(gridDim.x);
-  const int w = int(blockIdx.x);
   const int lane = int(threadIdx.x);
   const int slot = lane / LANES;            // knot slot in the chunk
   const int role = lane % LANES;
@@ -224,36 +263,89 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
     }
   };
 
-  // full rounds (all chunks when they fit one round): fixed order, no atomics;
-  // the last partial round: through the counters
-  const int fixed = total <= G ? total : total - total % G;
-  const int rest = total - fixed;
-  const int S = G < kTailCounters ? G : kTailCounters;
-  const int s = w % S;
-  const int mine = s < rest ? (rest - 1 - s) / S + 1 : 0;     // tail chunks dealt to counter s
-  const int waves = (G - 1 - s) / S + 1;                      // waves that claim from counter s
-  unsigned long long* ctr = tail + s * kTailStride;
-  int next = w;
-  for (;;) {
-    int c;
-    if (next < fixed) {
-      c = next;
-      next += G;
+  drain_chunks(total, tail, lane, process);
+  publish_completion(done, tail, unsigned(gridDim.x), lane);
+}
+
+// Batches on the L-lane groups (L = 4, 7, 8; group_mapping.hpp).  A chunk is
+// kSlots = floor(64 / L) consecutive knots of the batch's flattened knot
+// sequence (the trajectories one after another), so a chunk may span several
+// trajectories and no lane idles at a trajectory's end: the 9,014 knots of 64
+// trajectories of 8..256 knots are 1,002 chunks of 9 with L = 7.  Each lane
+// finds its own knot's trajectory: SMALL batches from a shuffle scan of the
+// (up to 64) lengths, searched per lane; larger ones by a binary search of the
+// plan kernel's knot prefix sum (header chunk_knots = 1) for the chunk's first
+// knot, from which each lane walks on.  Queue, descriptors in the arguments
+// and completion word as in batch_persistent_kernel.
+template <class P, int L, bool SMALL>
+__global__ __launch_bounds__(64, 1) void batch_group_kernel(
+    const TrajDesc* __restrict__ descs, const BatchHeader* __restrict__ hdr,
+    const typename std::conditional<SMALL, ArgBatch, NoArgBatch>::type arg, unsigned long long* __restrict__ tail,
+    const long* __restrict__ prefix, Options opt, Completion done) {
+  using Gr = Group<L>;
+  int n_traj, total;
+  if constexpr (SMALL) {
+    if (descs == nullptr) {
+      descs = arg.d;
+      n_traj = arg.n_traj;
+      total = arg.total_chunks;
     } else {
-      if (rest == 0) break;
-      int j = 0;
-      if (lane == 0) {
-        j = int(atomicAdd(ctr, 1ull));
-        // the launch's last claim on this counter: back to zero for the next launch
-        if (j == mine + waves - 1) atomicExch(ctr, 0ull);
-      }
-      j = __builtin_amdgcn_readfirstlane(j);
-      if (j >= mine) break;
-      c = fixed + s + j * S;
+      n_traj = hdr->n_traj;
+      total = int(hdr->total_chunks);
     }
-    process(c);
+  } else {
+    n_traj = hdr->n_traj;
+    total = int(hdr->total_chunks);
   }
-  publish_completion(done, tail, unsigned(G), lane);
+  const int lane = int(threadIdx.x);
+  const int slot = group_slot<L>(lane);
+  __shared__ typename P::Minv mi_sh[Gr::kMinvColumns ? Gr::kSlots : 1][kMinvPacked];   // Minv per knot (L >= 7)
+  // SMALL: lane l holds the knots of trajectories 0..l (inclusive prefix sum)
+  long incl = 0;
+  if constexpr (SMALL) {
+    incl = lane < n_traj ? descs[lane].n : 0;
+    for (int off = 1; off < 64; off <<= 1) {
+      const long o = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += o;
+    }
+  }
+  const long knots = SMALL ? __shfl(incl, 63, 64) : prefix[n_traj];
+  auto process = [&](int c) {
+    long kk = long(c) * Gr::kSlots + slot;      // the lane's knot in the batch
+    const bool valid = kk < knots && lane < Gr::kLanesUsed;
+    if (kk >= knots) kk = knots - 1;            // past the last knot: a copy of it, nothing stored
+    int t;
+    long first;
+    if constexpr (SMALL) {
+      // t = the number of trajectories that end at or before knot kk
+      int lo = 0;
+      for (int step = 32; step >= 1; step >>= 1) {
+        const long v = __shfl(incl, lo + step - 1, 64);
+        if (v <= kk) lo += step;
+      }
+      t = lo;
+      first = __shfl(incl, t > 0 ? t - 1 : 0, 64);
+      if (t == 0) first = 0;
+    } else {
+      const long kk0 = long(c) * Gr::kSlots;    // the chunk's first knot: wave-uniform search
+      int lo = 0, hi = n_traj;                  // prefix[lo] <= kk0 < prefix[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= kk0) lo = mid; else hi = mid;
+      }
+      t = lo;
+      while (t + 1 < n_traj && prefix[t + 1] <= kk) ++t;
+      first = prefix[t];
+    }
+    const TrajDesc d = descs[t];
+    const long k = kk - first;
+    InAoS i{d.in + kInPerKnot * k};
+    OutMasked<OutAoS> o{OutAoS{d.out + kOutPerKnot * k}, valid};
+    fdgrad_knot_group<P, L>(i, o, opt, lane, mi_sh[Gr::kMinvColumns ? slot : 0]);
+    wave_lds_sync();                            // every lane is done with mi_sh before the next chunk
+  };
+  drain_chunks(total, tail, lane, process);
+  publish_completion(done, tail, unsigned(gridDim.x), lane);
 }
 
 // Batches on the two-wave mapping: each workgroup of two waves takes

@@ -26,6 +26,23 @@ constexpr int64_t kDefaultGroupThreshold = 8192;
 // measured on MI355X: 8.4 us against 9.0 us for one-wave lane groups at 4,096
 // knots, 18.4 against 12.6 us at 8,192); rbd_autotune measures it.
 constexpr int kTwoWaveLanes = 32;   // lanes_per_knot value of the two-wave mapping (16 lanes in each of 2 waves)
+// lanes_per_knot values of the L-lane groups of group_mapping.hpp (full path only)
+inline bool is_group_lanes(int lanes) { return lanes == 4 || lanes == 7 || lanes == 8; }
+// Mapping tables the autotuners write: a call (or batch) of n knots uses the
+// mapping measured best at the smallest tabulated size >= n, and the largest
+// size's mapping above it.
+constexpr int kMaxTuned = 16;
+struct MappingTable {
+  std::atomic<int> count{0};           // 0: no table, the thresholds decide
+  int64_t n[kMaxTuned] = {};
+  int lanes[kMaxTuned] = {};
+  int lookup(int64_t knots) const {
+    const int c = count.load(std::memory_order_acquire);
+    for (int i = 0; i < c; ++i)
+      if (knots <= n[i]) return lanes[i];
+    return c ? lanes[c - 1] : 0;
+  }
+};
 // The same crossover for batches (rbd_batch_*, total knots of the batch),
 // measured by rbd_autotune_batch on batches of 132-knot trajectories
 // (7,957 knots on MI355X: one-lane 17.0 us vs lane-group 18.4 us at 8,448).
@@ -66,27 +83,69 @@ struct rbd_context {
   std::atomic<int64_t> two_wave_threshold{0};    // set by rbd_create / rbd_autotune
   std::atomic<int64_t> batch_group_threshold{rbdi::kDefaultBatchGroupThreshold};
   int compute_units = 0;                         // of the device
+  rbdi::MappingTable tuned;                      // per call, set by rbd_autotune
+  rbdi::MappingTable batch_tuned;                // per batch (total knots), set by rbd_autotune_batch
   rbd::Options opt;
   // device buffer of the host-pointer convenience call
   std::mutex host_mu;
   float* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  // the word a two-wave kernel's handoff wait sets when it gives up: pinned,
+  // coherent host memory (opt.fault is its device address)
+  volatile unsigned* fault_host = nullptr;
 };
 
 namespace rbdi {
 
-// Mapping of a call of n knots: 1, 16 or kTwoWaveLanes.  Only the full path
-// (AoS, SoA) has a two-wave kernel; other entry points pass two_wave = false
-// and get the one-wave lane group instead.
-inline int lanes_for(const rbd_context* ctx, int64_t n, bool two_wave = false) {
-  if (ctx->lanes_per_knot == kTwoWaveLanes) return two_wave ? kTwoWaveLanes : rbd::kLanesPerKnotGroup;
-  if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
-  if (two_wave && n <= ctx->two_wave_threshold.load(std::memory_order_relaxed)) return kTwoWaveLanes;
-  return n <= ctx->group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
+// The mappings the autotuners compare (lanes per knot) and the flat sizes.
+constexpr int kAutotuneMappings = 6;
+constexpr int kAutotuneLanes[kAutotuneMappings] = {1, rbd::kLanesPerKnotGroup, kTwoWaveLanes, 4, 7, 8};
+constexpr int kAutotuneSizes = 12;
+
+// The mapping tables a context starts with, from MI355X measurements of the
+// autotuners (scaled by the device's SIMDs, 4 per CU): per call, the two-wave
+// mapping up to 4 knots per SIMD (4,096 knots on 1,024 SIMDs), groups of 7
+// lanes up to one wave of 9 knots per SIMD (9,216), groups of 4 lanes up to
+// one wave of 16 (16,384), then one lane per knot; per batch the same without
+// the two-wave mapping, whose looping batch kernel trails the groups of 7
+// lanes there.
+// (Measured: a call of 9,014 knots 8.5 us with 7 lanes per knot, 12.9 with two
+// waves, 15.0 with one lane; 16,384 knots 11.5 us with 4, 15.3 with one lane;
+// the 64-trajectory mixed batch of 9,014 knots 9.6 us with 7 against 16.4
+// with one lane or 16.)
+inline void set_default_tables(rbd_context* ctx) {
+  const int64_t simds = 4 * int64_t(ctx->compute_units);
+  const int64_t n[3] = {4 * simds, 9 * simds, 16 * simds};
+  const int flat[4] = {kTwoWaveLanes, 7, 4, 1};
+  for (int i = 0; i < 3; ++i) { ctx->tuned.n[i] = n[i]; ctx->tuned.lanes[i] = flat[i]; }
+  ctx->tuned.n[3] = INT64_MAX;
+  ctx->tuned.lanes[3] = flat[3];
+  ctx->tuned.count.store(4, std::memory_order_release);
+  ctx->batch_tuned.n[0] = 9 * simds; ctx->batch_tuned.lanes[0] = 7;
+  ctx->batch_tuned.n[1] = 16 * simds; ctx->batch_tuned.lanes[1] = 4;
+  ctx->batch_tuned.n[2] = INT64_MAX; ctx->batch_tuned.lanes[2] = 1;
+  ctx->batch_tuned.count.store(3, std::memory_order_release);
+}
+
+// Mapping of a call of n knots: 1, 4, 7, 8, 16 or kTwoWaveLanes -- the
+// context's fixed lanes_per_knot, else the autotuned table, else the
+// thresholds.  Only the full path (AoS, SoA) has two-wave and L-lane group
+// kernels; other entry points pass full_path = false and get the 16-lane
+// group instead.
+inline int lanes_for(const rbd_context* ctx, int64_t n, bool full_path = false) {
+  int l = ctx->lanes_per_knot;
+  if (!l) l = ctx->tuned.lookup(n);
+  if (!l) {
+    if (full_path && n <= ctx->two_wave_threshold.load(std::memory_order_relaxed)) return kTwoWaveLanes;
+    return n <= ctx->group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
+  }
+  if (!full_path && l != 1) return rbd::kLanesPerKnotGroup;
+  return l;
 }
 
 inline int lanes_for_batch(const rbd_context* ctx, int64_t total_knots) {
   if (ctx->lanes_per_knot) return ctx->lanes_per_knot;
+  if (int l = ctx->batch_tuned.lookup(total_knots)) return l;
   return total_knots <= ctx->batch_group_threshold.load(std::memory_order_relaxed) ? rbd::kLanesPerKnotGroup : 1;
 }
 
@@ -130,6 +189,21 @@ inline rbd_status use_device(const rbd_context* ctx) {
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   }
   return RBD_OK;
+}
+
+// RBD_ERR_DEVICE_TIMEOUT once a kernel of the context gave up a handoff wait
+inline rbd_status device_status(const rbd_context* ctx) {
+  if (ctx->fault_host && *ctx->fault_host != 0)
+    return fail(RBD_ERR_DEVICE_TIMEOUT,
+                "a two-wave kernel of this context gave up its LDS handoff wait: its results are invalid "
+                "(rbd_context_clear_status to go on)");
+  return RBD_OK;
+}
+
+// Before a launch: the context's device current, and no device-side failure pending.
+inline rbd_status ready_to_launch(const rbd_context* ctx) {
+  if (rbd_status s = device_status(ctx)) return s;
+  return use_device(ctx);
 }
 
 }  // namespace rbdi

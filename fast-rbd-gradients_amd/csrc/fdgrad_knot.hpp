@@ -14,10 +14,11 @@
 // transform, inertia and offset are folded into straight-line code: structural
 // zeros of E, r and I cost nothing when the file is compiled with
 // -fno-signed-zeros -ffinite-math-only (these let x*0 and x+0 fold away).
-// The column loop of step 6 runs at run time so that its body stays small; a
-// column's d/dq and d/dqd sweeps share one pass over the links, and each output
-// column -Minv dtau[:, c] is stored as soon as it is known, so dtau never has
-// to be held whole.
+// In one lane the thirteen columns of step 6 are unrolled at compile time, so
+// each column's sweep starts at its own link with its seed folded into the
+// code (a lane group instead gives each lane a run-time column of its own,
+// dtau_column_uniform); each output column -Minv dtau[:, c] is stored as soon
+// as it is known, so dtau never has to be held whole.
 //
 // The file is plain C++17 when RBD_HD is empty, so the same arithmetic is also
 // compiled for the host by the CPU test harness.
@@ -714,11 +715,31 @@ using Wider = typename std::conditional<(sizeof(A) >= sizeof(B)), A, B>::type;
 // compatible mode].  Out: qdd(i, x), dq(r, c, x), dqd(r, c, x).  All FP32.
 // ---------------------------------------------------------------------------
 
-// Run-time model options (the reference's compile-time VEL_DAMPING and MPC_MODE).
+// Run-time model options (the reference's compile-time VEL_DAMPING and MPC_MODE),
+// and the bound on the two-wave mapping's LDS handoff waits.
 struct Options {
   float damping = 0.0f;               // joint viscous damping: 0.5 with VEL_DAMPING (dynamicsGradient.h:680,840)
   float gravity = float(kGravity);    // 0 in MPC_MODE (dynamicsGradient.h:635-637)
+  // A handoff wait polls its flag at most handoff_spins + 1 times, with a
+  // short sleep between polls (2^24 sleeps of 64 cycles: about half a second);
+  // a negative value makes every wait give up at once.  A wait that gives up
+  // writes 1 to *fault, a word the host can read (null: nowhere), and the
+  // kernel carries on and exits normally; its results are then invalid.
+  unsigned* fault = nullptr;
+  int handoff_spins = 1 << 24;
 };
+
+// The bounded wait of a handoff, for any poll and pause: true when poll()
+// returned true within spins + 1 polls; false at once for spins < 0.
+template <class Poll, class Pause>
+RBD_HD RBD_INLINE bool bounded_wait(Poll&& poll, Pause&& pause, int spins) {
+  if (spins < 0) return false;
+  for (int spin = 0;; ++spin) {
+    if (poll()) return true;
+    if (spin >= spins) return false;
+    pause();
+  }
+}
 
 // sin/cos type of the full path: computed once in the widest arithmetic type
 template <class P>

@@ -1,25 +1,31 @@
 // librbdgrad.so: HIP kernels for gfx950 and the C ABI declared in include/rbdgrad.h.
 //
-// Two mappings of knot points onto lanes, both running the per-knot solver
-// (fdgrad_knot.hpp) out of registers -- no LDS, no barriers, no cross-lane
-// traffic; the only memory traffic is the knot's 84 input bytes and its 420
-// output bytes:
+// Mappings of knot points onto lanes, all running the per-knot solver
+// (fdgrad_knot.hpp) out of registers; the memory traffic is the knot's 84
+// input bytes and its 420 output bytes, plus LDS exchanges inside a knot's
+// lane group:
 //   * one lane per knot (throughput): a wave computes 64 knots in lock step
-//     with uniform control flow (the derivative-column loop runs over the same
-//     column index in every lane);
-//   * one knot per 16-lane group (latency): every lane of the group runs the
-//     shared steps and then one derivative column, so a knot's critical path is
-//     the shared steps plus one column instead of all thirteen.
-// The context picks the mapping per call from the batch size (crossover set by
-// rbd_autotune or the built-in default).
+//     with uniform control flow (the derivative columns are unrolled at
+//     compile time, the same column in every lane);
+//   * one knot per group of 4, 7 or 8 lanes (group_mapping.hpp): every lane
+//     runs the shared steps and a share of the derivative columns;
+//   * one knot per 16-lane group: the shared steps, then one derivative column
+//     per lane, so a knot's critical path is the shared steps plus one column
+//     instead of all thirteen;
+//   * two waves per knot group (lowest latency, two_wave.hpp): Minv on one
+//     wave, the RNEA and the columns on the other.
+// The context picks the mapping per call from the size (its mapping table,
+// measured by rbd_autotune, or the built-in defaults).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
 
 #include "context.hpp"
+#include "group_mapping.hpp"
 #include "two_wave.hpp"
 
 namespace rbdi {
@@ -146,6 +152,31 @@ __global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, 
   two_wave_group<P>(i, o, opt, sh[g], flags);
 }
 
+// L lanes per knot (L = 4, 7, 8; group_mapping.hpp): floor(64 / L) knots per
+// wave, one wave per workgroup, the waves striding over the call's knots.
+// Lanes of knots past the end compute a copy of the last knot and store nothing.
+template <class P, int L, bool SOA>
+__global__ __launch_bounds__(kBlock, 1) void fdgrad_group_kernel(long n, const float* __restrict__ in, long ld_in,
+                                                                float* __restrict__ out, long ld_out, Options opt) {
+  using G = Group<L>;
+  __shared__ typename P::Minv mi_sh[G::kMinvColumns ? G::kSlots : 1][kMinvPacked];   // Minv per knot (L >= 7)
+  const int lane = int(threadIdx.x);
+  const int slot = group_slot<L>(lane);
+  using In = typename std::conditional<SOA, InSoA, InAoS>::type;
+  using O = typename std::conditional<SOA, OutSoA, OutAoS>::type;
+  for (long w = blockIdx.x; w * G::kSlots < n; w += gridDim.x) {
+    const long k = w * G::kSlots + slot;
+    const long kk = k < n ? k : n - 1;
+    In i;
+    O o0;
+    if constexpr (SOA) { i = InSoA{in + kk, ld_in}; o0 = OutSoA{out + kk, ld_out}; }
+    else { i = InAoS{in + kInPerKnot * kk}; o0 = OutAoS{out + kOutPerKnot * kk}; }
+    OutMasked<O> o{o0, k < n && lane < G::kLanesUsed};
+    fdgrad_knot_group<P, L>(i, o, opt, lane, mi_sh[G::kMinvColumns ? slot : 0]);
+    wave_lds_sync();                        // every lane is done with mi_sh before the next knots
+  }
+}
+
 // Full path plus dqdd/du = Minv (49 per knot, dense), both mappings.  WAFR:
 // out receives the WAFR baseline's 147-float layout (OutWafr) and minv is the
 // optional qdd buffer.
@@ -250,7 +281,7 @@ namespace rbdi {
 
 // grid for n knots at `lanes` lanes per knot, capped so huge batches grid-stride
 unsigned grid_for(int64_t n, int lanes) {
-  const int64_t knots_per_block = kBlock / lanes;
+  const int64_t knots_per_block = lanes == 1 ? kBlock : kBlock / lanes;   // 9 for 7 lanes per knot
   const int64_t blocks = (n + knots_per_block - 1) / knots_per_block;
   const int64_t cap = int64_t(1) << 20;
   return unsigned(blocks < cap ? blocks : cap);
@@ -290,6 +321,20 @@ hipError_t launch_flat_p(const FlatCall& c) {
         hipLaunchKernelGGL(rbd::fdgrad_refmode_kernel<P>, grid, block, 0, c.stream, n, c.in, c.out, c.opt);
         break;
     }
+  } else if (is_group_lanes(c.lanes) && c.layout != Layout::Ref) {
+    // L lanes per knot: one wave per workgroup, floor(64 / L) knots per wave
+    auto go = [&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+      if (c.layout == Layout::AoS)
+        hipLaunchKernelGGL((rbd::fdgrad_group_kernel<P, L, false>), grid, block, 0, c.stream, n, c.in, 0L, c.out, 0L,
+                           c.opt);
+      else
+        hipLaunchKernelGGL((rbd::fdgrad_group_kernel<P, L, true>), grid, block, 0, c.stream, n, c.in, long(c.ld_in),
+                           c.out, long(c.ld_out), c.opt);
+    };
+    if (c.lanes == 4) go(rbd::IC<4>{});
+    else if (c.lanes == 7) go(rbd::IC<7>{});
+    else go(rbd::IC<8>{});
   } else if (c.lanes == kTwoWaveLanes && c.layout != Layout::Ref) {
     {
       // latency: Minv and the RNEA + columns on two waves per knot
@@ -330,7 +375,7 @@ using namespace rbdi;
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int rbd_abi_version(void) { return 1; }
+int rbd_abi_version(void) { return 2; }
 
 const char* rbd_model_name(void) { return rbd::kModelName; }
 
@@ -351,6 +396,7 @@ const char* rbd_status_string(rbd_status s) {
     case RBD_ERR_OUT_OF_MEMORY: return "out of device memory";
     case RBD_ERR_UNSUPPORTED: return "unsupported";
     case RBD_ERR_CAPACITY: return "capacity exceeded";
+    case RBD_ERR_DEVICE_TIMEOUT: return "device wait timed out";
   }
   return "unknown status";
 }
@@ -369,8 +415,8 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   for (int r : c.reserved)
     if (r != 0) return fail(RBD_ERR_INVALID_ARGUMENT, "reserved config fields must be zero");
   if (c.lanes_per_knot != 0 && c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup &&
-      c.lanes_per_knot != kTwoWaveLanes)
-    return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 0, 1, 16 or 32");
+      c.lanes_per_knot != kTwoWaveLanes && !is_group_lanes(c.lanes_per_knot))
+    return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 0, 1, 4, 7, 8, 16 or 32");
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev <= 0) return fail(RBD_ERR_NO_DEVICE, "no HIP device visible");
@@ -385,8 +431,28 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   ctx->precision = c.precision;
   ctx->lanes_per_knot = c.lanes_per_knot;
   ctx->two_wave_threshold.store(int64_t(16) * ctx->compute_units);
+  set_default_tables(ctx);
   ctx->opt.damping = (c.flags & RBD_FLAG_VEL_DAMPING) ? 0.5f : 0.0f;
   ctx->opt.gravity = (c.flags & RBD_FLAG_NO_GRAVITY) ? 0.0f : float(rbd::kGravity);
+  // the handoff-failure word: pinned, coherent host memory that the kernels
+  // write through its device address
+  int cur = -1;
+  (void)hipGetDevice(&cur);
+  e = hipSetDevice(c.device);
+  void* word = nullptr;
+  if (e == hipSuccess)
+    e = hipHostMalloc(&word, 64, hipHostMallocMapped | hipHostMallocCoherent | hipHostMallocPortable);
+  void* dword = nullptr;
+  if (e == hipSuccess) e = hipHostGetDevicePointer(&dword, word, 0);
+  if (cur >= 0 && cur != c.device) (void)hipSetDevice(cur);
+  if (e != hipSuccess) {
+    if (word) (void)hipHostFree(word);
+    delete ctx;
+    return hip_fail(e, "rbd_create: handoff-failure word");
+  }
+  *static_cast<volatile unsigned*>(word) = 0u;
+  ctx->fault_host = static_cast<volatile unsigned*>(word);
+  ctx->opt.fault = static_cast<unsigned*>(dword);
   *out = ctx;
   return RBD_OK;
 }
@@ -400,7 +466,25 @@ rbd_status rbd_destroy(rbd_context* ctx) {
     (void)hipFree(ctx->d_scratch);
     if (cur >= 0) (void)hipSetDevice(cur);
   }
+  if (ctx->fault_host) (void)hipHostFree(const_cast<unsigned*>(ctx->fault_host));
   delete ctx;
+  return RBD_OK;
+}
+
+rbd_status rbd_context_status(rbd_context* ctx) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  return device_status(ctx);
+}
+
+rbd_status rbd_context_clear_status(rbd_context* ctx) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (ctx->fault_host) *ctx->fault_host = 0u;
+  return RBD_OK;
+}
+
+rbd_status rbd_set_handoff_spins(rbd_context* ctx, int spins) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  ctx->opt.handoff_spins = spins;
   return RBD_OK;
 }
 
@@ -409,7 +493,7 @@ rbd_status rbd_fdgrad_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, f
   if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
   if (n == 0) return RBD_OK;
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const FlatCall c{Layout::AoS, lanes_for(ctx, n, true), n, d_in, 0, d_out, 0, ctx->opt,
                    (hipStream_t)stream, 4 * ctx->compute_units};
   hipError_t e = launch_flat(ctx->precision, c);
@@ -423,7 +507,7 @@ rbd_status rbd_fdgrad_traj_soa_f32(rbd_context* ctx, int64_t n, const float* d_i
   if (n == 0) return RBD_OK;
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (ld_in < n || ld_out < n) return fail(RBD_ERR_INVALID_ARGUMENT, "leading dimension smaller than knot count");
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const FlatCall c{Layout::SoA, lanes_for(ctx, n, true), n, d_in, ld_in, d_out, ld_out, ctx->opt,
                    (hipStream_t)stream, 4 * ctx->compute_units};
   hipError_t e = launch_flat(ctx->precision, c);
@@ -437,7 +521,7 @@ rbd_status rbd_dynamics_gradient_kernel_f32(rbd_context* ctx, int64_t n, float* 
   if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
   if (n == 0) return RBD_OK;
   if (!d_dqdd || !d_mem_vol) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const FlatCall c{Layout::Ref, lanes_for(ctx, n), n, d_mem_vol, 0, d_dqdd, 0, ctx->opt,
                    (hipStream_t)stream, 4 * ctx->compute_units};
   hipError_t e = launch_flat(ctx->precision, c);
@@ -449,7 +533,7 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
   if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
   if (n == 0) return RBD_OK;
   if (!h_in || !h_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   std::lock_guard<std::mutex> lock(ctx->host_mu);
   const size_t in_bytes = size_t(n) * rbd::kInPerKnot * sizeof(float);
   const size_t out_bytes = size_t(n) * rbd::kOutPerKnot * sizeof(float);
@@ -476,25 +560,16 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
 
 const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knots) {
   if (!ctx) return "";
-  static const char* names[2][3][5] = {
-      {{"fdgrad_aos_kernel<PrecFP32>", "fdgrad_aos_kernel<PrecFP64Pivot>", "fdgrad_aos_kernel<PrecFP64Col>",
-        "fdgrad_aos_kernel<PrecFP64>", "fdgrad_aos_kernel<PrecFP32HwTrig>"},
-       {"fdgrad_aos_lanes_kernel<PrecFP32>", "fdgrad_aos_lanes_kernel<PrecFP64Pivot>",
-        "fdgrad_aos_lanes_kernel<PrecFP64Col>", "fdgrad_aos_lanes_kernel<PrecFP64>",
-        "fdgrad_aos_lanes_kernel<PrecFP32HwTrig>"},
-       {"fdgrad_two_wave_kernel<PrecFP32>", "fdgrad_two_wave_kernel<PrecFP64Pivot>",
-        "fdgrad_two_wave_kernel<PrecFP64Col>", "fdgrad_two_wave_kernel<PrecFP64>",
-        "fdgrad_two_wave_kernel<PrecFP32HwTrig>"}},
-      {{"fdgrad_refmode_kernel<PrecFP32>", "fdgrad_refmode_kernel<PrecFP64Pivot>",
-        "fdgrad_refmode_kernel<PrecFP64Col>", "fdgrad_refmode_kernel<PrecFP64>",
-        "fdgrad_refmode_kernel<PrecFP32HwTrig>"},
-       {"fdgrad_refmode_lanes_kernel<PrecFP32>", "fdgrad_refmode_lanes_kernel<PrecFP64Pivot>",
-        "fdgrad_refmode_lanes_kernel<PrecFP64Col>", "fdgrad_refmode_lanes_kernel<PrecFP64>",
-        "fdgrad_refmode_lanes_kernel<PrecFP32HwTrig>"},
-       {"", "", "", "", ""}}};
+  static const char* prec[5] = {"PrecFP32", "PrecFP64Pivot", "PrecFP64Col", "PrecFP64", "PrecFP32HwTrig"};
+  thread_local std::string name;
   const int lanes = lanes_for(ctx, n_knots, !reference_mode);
-  const int mapping = lanes == 1 ? 0 : (lanes == kTwoWaveLanes ? 2 : 1);
-  return names[reference_mode ? 1 : 0][mapping][ctx->precision];
+  const char* base = reference_mode ? (lanes == 1 ? "fdgrad_refmode_kernel" : "fdgrad_refmode_lanes_kernel")
+                     : lanes == 1 ? "fdgrad_aos_kernel"
+                     : lanes == kTwoWaveLanes ? "fdgrad_two_wave_kernel"
+                     : is_group_lanes(lanes) ? "fdgrad_group_kernel" : "fdgrad_aos_lanes_kernel";
+  name = std::string(base) + "<" + prec[ctx->precision] +
+         (is_group_lanes(lanes) && !reference_mode ? ", " + std::to_string(lanes) : std::string()) + ">";
+  return name.c_str();
 }
 
 extern "C++" {
@@ -505,7 +580,7 @@ namespace {
 template <bool WAFR>
 rbd_status launch_minv(rbd_context* ctx, int64_t n, const float* d_in, float* d_out, float* d_minv,
                        rbd_stream_t stream) {
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const int lanes = lanes_for(ctx, n);
   const dim3 grid(grid_for(n, lanes)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
@@ -565,7 +640,7 @@ rbd_status forward_dynamics(rbd_context* ctx, int64_t n, const float* d_in, cons
   if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
   if (n == 0) return RBD_OK;
   if (!d_in || !d_qdd || !d_minv) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const dim3 grid(grid_for(n, 1)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
   auto go = [&](auto prec_tag) {
@@ -609,7 +684,7 @@ rbd_status rbd_fdgrad_traj_fext_f32(rbd_context* ctx, int64_t n, const float* d_
   if (n == 0) return RBD_OK;
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (rbd_status s = check_fext(d_fext, fext_stride)) return s;
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const int lanes = lanes_for(ctx, n);
   const dim3 grid(grid_for(n, lanes)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
@@ -642,7 +717,7 @@ rbd_status id_gradient(rbd_context* ctx, int64_t n, float* d_cdu, const float* d
   if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
   if (n == 0) return RBD_OK;
   if (!d_cdu || !d_vol) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const int lanes = lanes_for(ctx, n);
   const dim3 grid(grid_for(n, lanes)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
@@ -687,7 +762,7 @@ rbd_status rbd_gradient_setup_split_f32(rbd_context* ctx, int64_t n, float* d_me
   if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
   if (n == 0) return RBD_OK;
   if (!d_mem_vol_split) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const dim3 grid(grid_for(n, 1)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
   if (ctx->precision == RBD_PREC_FP64)
@@ -704,7 +779,7 @@ rbd_status rbd_gradient_finish_f32(rbd_context* ctx, int64_t n, float* d_dqdd, c
   if (n < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative knot count");
   if (n == 0) return RBD_OK;
   if (!d_dqdd || !d_cdu || !d_minv) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
-  if (rbd_status s = use_device(ctx)) return s;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
   const dim3 grid(grid_for(n, 1)), block(kBlock);
   hipLaunchKernelGGL(rbd::finish_kernel, grid, block, 0, (hipStream_t)stream, long(n), d_minv, d_cdu, d_dqdd);
   hipError_t e = hipGetLastError();
@@ -715,9 +790,11 @@ int64_t rbd_group_threshold(rbd_context* ctx) { return ctx ? ctx->group_threshol
 
 int64_t rbd_two_wave_threshold(rbd_context* ctx) { return ctx ? ctx->two_wave_threshold.load() : -1; }
 
+// Setting a threshold by hand drops the mapping table: the thresholds decide.
 rbd_status rbd_set_two_wave_threshold(rbd_context* ctx, int64_t n_knots) {
   if (rbd_status s = check_ctx(ctx)) return s;
   if (n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative threshold");
+  ctx->tuned.count.store(0, std::memory_order_release);
   ctx->two_wave_threshold.store(n_knots);
   return RBD_OK;
 }
@@ -725,17 +802,34 @@ rbd_status rbd_set_two_wave_threshold(rbd_context* ctx, int64_t n_knots) {
 rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots) {
   if (rbd_status s = check_ctx(ctx)) return s;
   if (n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative threshold");
+  ctx->tuned.count.store(0, std::memory_order_release);
   ctx->group_threshold.store(n_knots);
   return RBD_OK;
+}
+
+int rbd_mapping_table(rbd_context* ctx, int batched, int64_t* n_knots, int* lanes, int capacity) {
+  if (!ctx) return -1;
+  const MappingTable& t = batched ? ctx->batch_tuned : ctx->tuned;
+  const int c = t.count.load(std::memory_order_acquire);
+  for (int i = 0; i < c && i < capacity; ++i) {
+    if (n_knots) n_knots[i] = t.n[i];
+    if (lanes) lanes[i] = t.lanes[i];
+  }
+  return c;
 }
 
 rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                         int table_len) {
   if (rbd_status s = check_ctx(ctx)) return s;
-  if (rbd_status s = use_device(ctx)) return s;
-  static const int64_t sizes[] = {64, 256, 1024, 2048, 4096, 8192, 16384, 32768, 65536};
-  constexpr int kSizes = sizeof(sizes) / sizeof(sizes[0]);
-  const int64_t nmax = sizes[kSizes - 1];
+  if (rbd_status s = ready_to_launch(ctx)) return s;
+  // sizes, including the knot counts that fill the device's SIMDs with one
+  // wave of 7-lane groups (9 knots each) or of 4-lane groups (16)
+  const int64_t simds = 4 * int64_t(ctx->compute_units);
+  int64_t sizes[kAutotuneSizes] = {64,        256,       1024,       2048,  4096,  8192,
+                                   9 * simds, 12 * simds, 16 * simds, 24 * simds, 32 * simds, 65536};
+  std::sort(sizes, sizes + kAutotuneSizes);
+  int64_t nmax = 0;
+  for (int64_t n : sizes) nmax = n > nmax ? n : nmax;
   float* buf = nullptr;
   hipError_t e = hipMalloc(&buf, size_t(nmax) * (rbd::kInPerKnot + rbd::kOutPerKnot) * sizeof(float));
   if (e != hipSuccess) return fail(RBD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -746,12 +840,10 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&t0);
   if (e == hipSuccess) e = hipEventCreate(&t1);
-  int64_t threshold = 0, two_wave = 0;
-  static const int kLanes[3] = {1, rbd::kLanesPerKnotGroup, kTwoWaveLanes};
-  float best_us[3][kSizes] = {};
-  for (int si = 0; si < kSizes && e == hipSuccess; ++si) {
-    for (int li = 0; li < 3 && e == hipSuccess; ++li) {
-      const FlatCall c{Layout::AoS, kLanes[li], sizes[si], in, 0, out, 0, ctx->opt,
+  float best_us[kAutotuneMappings][kAutotuneSizes] = {};
+  for (int si = 0; si < kAutotuneSizes && e == hipSuccess; ++si) {
+    for (int li = 0; li < kAutotuneMappings && e == hipSuccess; ++li) {
+      const FlatCall c{Layout::AoS, kAutotuneLanes[li], sizes[si], in, 0, out, 0, ctx->opt,
                        (hipStream_t)stream, 4 * ctx->compute_units};
       float best = 1e30f;
       for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
@@ -766,20 +858,32 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
       }
       best_us[li][si] = best;
     }
-    if (e == hipSuccess && best_us[1][si] < best_us[0][si]) threshold = sizes[si];
-    if (e == hipSuccess && best_us[2][si] < best_us[1][si] && best_us[2][si] < best_us[0][si])
-      two_wave = sizes[si];
   }
   if (t0) (void)hipEventDestroy(t0);
   if (t1) (void)hipEventDestroy(t1);
   (void)hipFree(buf);
   if (e != hipSuccess) return hip_fail(e, "autotune");
+  // the table: at each size the fastest mapping; the thresholds keep their
+  // meaning for the one-lane, 16-lane and two-wave mappings
+  int64_t threshold = 0, two_wave = 0;
+  ctx->tuned.count.store(0, std::memory_order_release);
+  for (int si = 0; si < kAutotuneSizes; ++si) {
+    int bi = 0;
+    for (int li = 1; li < kAutotuneMappings; ++li)
+      if (best_us[li][si] < best_us[bi][si]) bi = li;
+    ctx->tuned.n[si] = sizes[si];
+    ctx->tuned.lanes[si] = kAutotuneLanes[bi];
+    if (best_us[1][si] < best_us[0][si]) threshold = sizes[si];
+    if (best_us[2][si] < best_us[1][si] && best_us[2][si] < best_us[0][si]) two_wave = sizes[si];
+  }
+  ctx->tuned.count.store(kAutotuneSizes, std::memory_order_release);
   ctx->group_threshold.store(threshold);
   ctx->two_wave_threshold.store(two_wave);
   if (threshold_out) *threshold_out = threshold;
+  constexpr int kRow = 1 + kAutotuneMappings;
   if (table_us)
-    for (int i = 0; i < table_len && i < 4 * kSizes; ++i)
-      table_us[i] = (i % 4 == 0) ? float(sizes[i / 4]) : best_us[(i % 4) - 1][i / 4];
+    for (int i = 0; i < table_len && i < kRow * kAutotuneSizes; ++i)
+      table_us[i] = (i % kRow == 0) ? float(sizes[i / kRow]) : best_us[(i % kRow) - 1][i / kRow];
   return RBD_OK;
 }
 

@@ -222,6 +222,13 @@ void dispatch_loop(rbd_server* s) {
       s->cv_dispatch.wait(lk, ready);
     }
     if (s->stop) return;
+    if (s->async_error != RBD_OK) {
+      // a batch failed: admit nothing more, fail what is queued (submit refuses new requests)
+      for (auto& r : s->queue) r->status = -1;
+      s->queue.clear();
+      s->cv_done.notify_all();
+      continue;
+    }
     // optional batching window: let requests right behind the head join
     if (s->cfg.batch_window_us > 0) {
       const auto deadline = s->queue.front()->t_submit +
@@ -379,7 +386,13 @@ void complete_loop(rbd_server* s, int slot) {
     if (!sl.launched) return;              // stop with nothing in flight
     const bool failed_launch = s->async_error != RBD_OK;
     lk.unlock();
-    const hipError_t e = failed_launch ? hipErrorUnknown : wait_batch(s, slot);
+    hipError_t e = failed_launch ? hipErrorUnknown : wait_batch(s, slot);
+    // a failed batch's copies and kernel may still be queued or running on the
+    // slot's stream: wait for them before its range of the pools is reused
+    if (e != hipSuccess) (void)hipStreamSynchronize(sl.stream);
+    // a two-wave kernel that gave up a handoff wait leaves invalid results
+    const rbd_status dev = e == hipSuccess ? device_status(s->run_ctx) : RBD_OK;
+    if (dev != RBD_OK) e = hipErrorUnknown;
     sl.rec.done_us = since(s, Clock::now());
     if (e == hipSuccess)
       for (auto& r : sl.reqs) std::memcpy(r->h_out, s->h_pool + r->out_off, r->out_bytes);
@@ -400,7 +413,7 @@ void complete_loop(rbd_server* s, int slot) {
         s->t_last = now;
       }
     }
-    if (e != hipSuccess && s->async_error == RBD_OK) s->async_error = RBD_ERR_HIP;
+    if (e != hipSuccess && s->async_error == RBD_OK) s->async_error = dev != RBD_OK ? dev : RBD_ERR_HIP;
     sl.reqs.clear();
     sl.busy = false;
     s->mem_gen++;
@@ -501,8 +514,9 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
   if (c.memory_budget_bytes < int64_t(kAlign)) return fail(RBD_ERR_INVALID_ARGUMENT, "memory budget too small");
   if (c.max_batch_trajectories < 1 || c.max_batch_knots < 1)
     return fail(RBD_ERR_INVALID_ARGUMENT, "batch limits must be positive");
-  if (c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup && c.lanes_per_knot != kTwoWaveLanes)
-    return fail(RBD_ERR_INVALID_ARGUMENT, "server lanes_per_knot must be 1, 16 or 32");
+  if (c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup && c.lanes_per_knot != kTwoWaveLanes &&
+      !is_group_lanes(c.lanes_per_knot))
+    return fail(RBD_ERR_INVALID_ARGUMENT, "server lanes_per_knot must be 1, 4, 7, 8, 16 or 32");
   if (!(c.batch_window_us >= 0)) return fail(RBD_ERR_INVALID_ARGUMENT, "negative batch window");
   if (c.zero_copy < 0 || c.zero_copy > 2) return fail(RBD_ERR_INVALID_ARGUMENT, "zero_copy must be 0, 1 or 2");
   if (c.zero_copy_max_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative zero_copy_max_knots");
@@ -519,6 +533,7 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
   rc.precision = ctx->precision;
   rc.lanes_per_knot = c.lanes_per_knot;
   rbd_status st = rbd_create(&rc, &s->run_ctx);
+  if (st == RBD_OK) s->run_ctx->opt.handoff_spins = ctx->opt.handoff_spins;   // the caller's wait bound
   const size_t budget = round_up(size_t(c.memory_budget_bytes));
   hipError_t e = hipSuccess;
   s->zero_copy_max_knots = c.zero_copy_max_knots > 0 ? c.zero_copy_max_knots : kZeroCopyMaxKnots;

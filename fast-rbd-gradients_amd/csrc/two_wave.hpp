@@ -3,9 +3,12 @@
 // wave 1 the RNEA and the columns (two_wave_* in fdgrad_knot.hpp).  The waves
 // hand data over in LDS and signal with one LDS flag per direction: each wave
 // sets its flag unconditionally before it waits for the other's, so every
-// wait ends (and gives up after ~2^24 sleeps in any case).  Used by the flat
-// kernel (one knot group per workgroup, rbdgrad.hip) and the batch kernel
-// (knot groups pulled in a loop, batch_kernels.hpp).
+// wait ends.  A wait is also bounded (Options::handoff_spins polls): one that
+// gives up writes 1 to the host-visible word Options::fault, and the host
+// turns that into an error status (RBD_ERR_DEVICE_TIMEOUT) instead of
+// returning the invalid results as good ones.  Used by the flat kernel (one
+// knot group per workgroup, rbdgrad.hip) and the batch kernel (knot groups
+// pulled in a loop, batch_kernels.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,12 +27,14 @@ __device__ inline void lds_signal(int* flag) {
   if (threadIdx.x % 64 == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   __builtin_amdgcn_sched_barrier(0);
 }
-__device__ inline void lds_wait(int* flag) {
+__device__ inline void lds_wait(int* flag, const Options& opt) {
   __builtin_amdgcn_sched_barrier(0);
-  for (int spin = 0; spin < (1 << 24); ++spin) {
-    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0) break;
-    __builtin_amdgcn_s_sleep(1);
-  }
+  const bool seen = bounded_wait(
+      [flag] { return __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0; },
+      [] { __builtin_amdgcn_s_sleep(1); }, opt.handoff_spins);
+  // the flag is read from LDS by every lane, so `seen` is the same in all of them
+  if (!seen && opt.fault != nullptr && threadIdx.x % 64 == 0)
+    __hip_atomic_store(opt.fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 // Materialise a value before a wait: the compiler would otherwise sink work
@@ -57,7 +62,7 @@ __device__ RBD_INLINE void two_wave_group(const In& i, O& o, const Options& opt,
       pin(tw[r].s);
       for (int c = r; c < NQ; ++c) pin(Mi[r][c]);
     }
-    lds_wait(&flags[0]);
+    lds_wait(&flags[0], opt);
     two_wave_m2<P>(role, tw, Mi, sh, o);
     lds_signal(&flags[1]);
   } else {
@@ -67,7 +72,7 @@ __device__ RBD_INLINE void two_wave_group(const In& i, O& o, const Options& opt,
     lds_signal(&flags[0]);
     two_wave_r1b<P>(opt, role, st, tau_col);
     for (int r = 0; r < NQ; ++r) pin(tau_col[r]);
-    lds_wait(&flags[1]);
+    lds_wait(&flags[1], opt);
     two_wave_r2<P>(role, sh, tau_col, o);
   }
 }

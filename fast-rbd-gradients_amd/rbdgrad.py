@@ -54,7 +54,13 @@ EXPORTED = [
     "rbd_fdgrad_traj_fext_f32", "rbd_forward_dynamics_fext_f32",
     "rbd_dynamics_gradient_kernel_fused_f32", "rbd_dynamics_gradient_kernel_split_f32",
     "rbd_gradient_setup_split_f32", "rbd_gradient_finish_f32", "rbd_model_name", "rbd_fdgrad_wafr_f32",
+    "rbd_context_status", "rbd_context_clear_status", "rbd_set_handoff_spins", "rbd_mapping_table",
 ]
+
+# the mappings the autotuners compare, in the order of their table rows (lanes per knot)
+AUTOTUNE_LANES = (1, 16, 32, 4, 7, 8)
+
+ERR_DEVICE_TIMEOUT = 7   # RBD_ERR_DEVICE_TIMEOUT
 
 
 class RbdError(RuntimeError):
@@ -164,6 +170,11 @@ def lib(model=None):
         L.rbd_server_run_poisson.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(i64), ctypes.c_int,
                                              vp, i64, ctypes.c_uint64, ctypes.POINTER(ServerStats)]
         L.rbd_model_name.restype = ctypes.c_char_p
+        L.rbd_context_status.argtypes = [vp]
+        L.rbd_context_clear_status.argtypes = [vp]
+        L.rbd_set_handoff_spins.argtypes = [vp, ctypes.c_int]
+        L.rbd_mapping_table.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.c_int]
         _libs[key] = L
     return _libs[key]
 
@@ -194,6 +205,7 @@ def hip():
         H.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
         H.hipSetDevice.argtypes = [ctypes.c_int]
         H.hipHostMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t, ctypes.c_uint]
+        H.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
         H.hipHostFree.argtypes = [vp]
         _hip = H
     return _hip
@@ -214,6 +226,13 @@ def device_count():
     n = ctypes.c_int(0)
     err = hip().hipGetDeviceCount(ctypes.byref(n))
     return n.value if err == 0 else 0
+
+
+def mem_info():
+    """(free, total) bytes of the current device (hipMemGetInfo)."""
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    _hipcheck(hip().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)), "hipMemGetInfo")
+    return free.value, total.value
 
 
 H2D, D2H, D2D = 1, 2, 3
@@ -363,18 +382,40 @@ class Context(_Bound):
                                                       d_const.handle if d_const is not None else None,
                                                       _sh(stream)), "rbd_dynamics_gradient_kernel_f32")
 
+    def status(self):
+        """rbd_context_status: 0, or ERR_DEVICE_TIMEOUT once a two-wave kernel
+        of this context gave up a handoff wait (synchronise first)."""
+        return self._L.rbd_context_status(self._h)
+
+    def clear_status(self):
+        self._check(self._L.rbd_context_clear_status(self._h), "rbd_context_clear_status")
+
+    def set_handoff_spins(self, spins):
+        """Bound of the two-wave handoff waits (< 0: every wait gives up at once, a test hook)."""
+        self._check(self._L.rbd_set_handoff_spins(self._h, int(spins)), "rbd_set_handoff_spins")
+
     def kernel_name(self, n_knots, reference_mode=False):
         return self._L.rbd_kernel_name(self._h, int(bool(reference_mode)), int(n_knots)).decode()
 
     def autotune(self, stream=None):
-        """Time the three knot-to-lane mappings on the device; sets and returns
-        the crossover plus the table {n: (us one-lane, us lane-group, us two-wave)}
-        (the two-wave limit is then self.two_wave_threshold)."""
+        """Time the six knot-to-lane mappings on the device and install the
+        context's mapping table; returns the 16-lane crossover and the timings
+        {n: {lanes per knot: us}} (the two-wave limit is then
+        self.two_wave_threshold, the installed table self.mapping_table())."""
         thr = ctypes.c_int64()
-        tab = (ctypes.c_float * 36)()
-        self._check(self._L.rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 36), "rbd_autotune")
-        table = {int(tab[i]): (tab[i + 1], tab[i + 2], tab[i + 3]) for i in range(0, 36, 4)}
+        row = 1 + len(AUTOTUNE_LANES)
+        tab = (ctypes.c_float * (16 * row))()
+        self._check(self._L.rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 16 * row), "rbd_autotune")
+        table = {int(tab[i]): {l: tab[i + 1 + j] for j, l in enumerate(AUTOTUNE_LANES)}
+                 for i in range(0, 16 * row, row) if tab[i] > 0}
         return thr.value, table
+
+    def mapping_table(self, batched=False):
+        """The installed mapping table: [(largest knots, lanes per knot), ...] ascending."""
+        n = (ctypes.c_int64 * 16)()
+        lanes = (ctypes.c_int * 16)()
+        c = self._L.rbd_mapping_table(self._h, int(bool(batched)), n, lanes, 16)
+        return [(int(n[i]), int(lanes[i])) for i in range(max(c, 0))]
 
     @property
     def two_wave_threshold(self):
@@ -385,12 +426,16 @@ class Context(_Bound):
         self._check(self._L.rbd_set_two_wave_threshold(self._h, int(n)), "rbd_set_two_wave_threshold")
 
     def autotune_batch(self, stream=None):
-        """Time both mappings of the batched path; sets and returns its crossover
-        (total knots) plus the table {total knots: (us one-lane, us lane-group)}."""
+        """Time the six mappings of the batched path and install the batch
+        mapping table; returns the 16-lane crossover (total knots) and the
+        timings {total knots: {lanes per knot: us}}."""
         thr = ctypes.c_int64()
-        tab = (ctypes.c_float * 27)()
-        self._check(self._L.rbd_autotune_batch(self._h, _sh(stream), ctypes.byref(thr), tab, 27), "rbd_autotune_batch")
-        table = {int(tab[i]): (tab[i + 1], tab[i + 2]) for i in range(0, 27, 3)}
+        row = 1 + len(AUTOTUNE_LANES)
+        tab = (ctypes.c_float * (16 * row))()
+        self._check(self._L.rbd_autotune_batch(self._h, _sh(stream), ctypes.byref(thr), tab, 16 * row),
+                    "rbd_autotune_batch")
+        table = {int(tab[i]): {l: tab[i + 1 + j] for j, l in enumerate(AUTOTUNE_LANES)}
+                 for i in range(0, 16 * row, row) if tab[i] > 0}
         return thr.value, table
 
     @property

@@ -50,7 +50,8 @@ typedef enum rbd_status {
   RBD_ERR_NO_DEVICE = 3,
   RBD_ERR_OUT_OF_MEMORY = 4,
   RBD_ERR_UNSUPPORTED = 5,
-  RBD_ERR_CAPACITY = 6
+  RBD_ERR_CAPACITY = 6,
+  RBD_ERR_DEVICE_TIMEOUT = 7 /* a kernel of this context gave up a bounded wait (see rbd_context_status) */
 } rbd_status;
 
 /* model options: the reference's compile-time VEL_DAMPING and MPC_MODE
@@ -72,8 +73,11 @@ typedef struct rbd_config {
   int device;         /* HIP device ordinal */
   unsigned flags;     /* RBD_FLAG_* */
   int precision;      /* rbd_precision */
-  int lanes_per_knot; /* 0: chosen per call from the batch size; 1: one lane per
-                         knot (throughput); 16: one knot per 16-lane group (latency);
+  int lanes_per_knot; /* 0: chosen per call from the size (the context's mapping
+                         table, see rbd_autotune); 1: one lane per knot
+                         (throughput); 4, 7, 8: one knot per group of that many
+                         lanes, 16, 9 and 8 knots per wave (full path and batches;
+                         other calls use 16); 16: one knot per 16-lane group;
                          32: one knot per 16-lane group in each of two waves, Minv
                          on one and the RNEA and columns on the other (lowest
                          latency; full path and batches, other calls use 16) */
@@ -92,6 +96,25 @@ rbd_status rbd_destroy(rbd_context* ctx);
 
 const char* rbd_status_string(rbd_status s);
 const char* rbd_last_error(void);
+
+/* Device-side failures.  The two-wave mapping (lanes_per_knot 32, flat calls,
+ * batches and servers) hands data between the two waves of a workgroup
+ * through LDS; each handoff wait is bounded.  A wait that gives up marks a
+ * host-visible word of the context, and the kernel still exits normally, but
+ * its results are invalid.  Once the word is set, every launching call on the
+ * context returns RBD_ERR_DEVICE_TIMEOUT, a server fails the batches that
+ * complete after it, and rbd_context_status reports it (call it after the
+ * stream has been synchronised to learn about the launches on it).
+ * rbd_context_clear_status clears the word.  No reference counterpart: the
+ * reference's kernels synchronise with __syncthreads and report nothing
+ * (gpuErrchk aborts on API errors only, helpers_GPU/dynamicsGradient.cuh:28-35). */
+rbd_status rbd_context_status(rbd_context* ctx);
+rbd_status rbd_context_clear_status(rbd_context* ctx);
+/* The bound of a handoff wait, in polls with a 64-cycle sleep between them
+ * (default 2^24, about half a second); spins < 0 makes every wait give up at
+ * once (a test hook: it forces the failure path from the host, with every
+ * kernel still running to its end).  Applies to launches made after the call. */
+rbd_status rbd_set_handoff_spins(rbd_context* ctx, int spins);
 /* Library ABI version (incremented on incompatible changes). */
 int rbd_abi_version(void);
 /* Name of the robot this library's kernels are compiled for: "iiwa14" for
@@ -222,35 +245,50 @@ rbd_status rbd_gradient_finish_f32(rbd_context* ctx, int64_t n_knots, float* d_d
  * it (template arguments abbreviated).  No reference counterpart. */
 const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knots);
 
-/* Autotuner for the knot-to-lane mapping (lanes_per_knot = 0): times the
- * three mappings (1, 16, 32 lanes per knot) on the device at batch sizes
- * 64 ... 65536; sets the largest size at which the 16-lane-group mapping beats
- * one lane per knot as the context's crossover (returned in threshold_out) and
- * the largest at which the two-wave mapping beats both as the two-wave limit.
+/* Autotuner for the knot-to-lane mapping (lanes_per_knot = 0): times the six
+ * mappings -- 1, 16, 32 (two waves), 4, 7 and 8 lanes per knot, which also
+ * fixes the knots per wave (64, 4, 4 per two waves, 16, 9, 8), the LDS each
+ * wave holds (none, one Minv per knot group for 7, 8 and 16 lanes, the
+ * two-wave handoff blocks) and the waves per SIMD each kernel is built for --
+ * on the device at 12 call sizes from 64 to 65,536 knots (with the sizes that
+ * fill the SIMDs with one wave of 9 or 16 knots), and installs the context's
+ * mapping table: a call of n knots uses the mapping fastest at the smallest
+ * measured size >= n (the largest size's above it).  Also sets the largest
+ * size at which the 16-lane mapping beats one lane per knot as the crossover
+ * (threshold_out) and the largest at which two waves beat both as the two-wave
+ * limit.  A context starts with a table of measured MI355X defaults.
  * Allocates and frees a temporary device buffer; synchronises `stream`.  Not
  * to be called while other threads launch on the same context.  table_us, if
- * not NULL, receives up to table_len floats as (n, us one-lane, us lane-group,
- * us two-wave) quadruples.  No reference counterpart (the reference fixes
+ * not NULL, receives up to table_len floats as rows of 7: (n, us with 1, 16,
+ * 32, 4, 7, 8 lanes per knot).  No reference counterpart (the reference fixes
  * dim3(32,7) per knot, time_GPU.cu:10-19). */
 rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                         int table_len);
 /* Current crossover (knots) and its manual override; the same for the
- * two-wave limit (calls of up to that many knots use 32 lanes per knot). */
+ * two-wave limit (calls of up to that many knots use 32 lanes per knot).
+ * Setting either by hand drops the mapping table, so the two thresholds then
+ * decide between the 1, 16 and 32-lane mappings. */
 int64_t rbd_group_threshold(rbd_context* ctx);
 rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots);
 int64_t rbd_two_wave_threshold(rbd_context* ctx);
 rbd_status rbd_set_two_wave_threshold(rbd_context* ctx, int64_t n_knots);
 
-/* The batched path (rbd_fdgrad_batched_f32) has its own crossover on a
- * batch's total knots.  rbd_autotune_batch times both mappings on batches of
- * 8, 16, 32, 48, 64, 96, 128, 256 and 512 trajectories of 132 knots and sets
- * it where the lane-group mapping stops being faster (interpolated); the table
- * holds up to nine (total knots, us one-lane, us lane-group) triples.  Same
- * conditions as rbd_autotune.  No reference counterpart. */
+/* The batched path (rbd_fdgrad_batched_f32) has its own mapping table on a
+ * batch's total knots.  rbd_autotune_batch times the six mappings on batches
+ * of 8, 16, 32, 48, 64, 69, 80, 96, 128, 256 and 512 trajectories of 132 knots
+ * and installs the fastest at each total; it also sets the crossover (the
+ * largest total at which 16-lane groups beat one lane per knot), which decides
+ * alone once set by hand.  table_us receives rows of 7 as rbd_autotune's, on
+ * total knots.  Same conditions as rbd_autotune.  No reference counterpart. */
 rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                               int table_len);
 int64_t rbd_batch_group_threshold(rbd_context* ctx);
 rbd_status rbd_set_batch_group_threshold(rbd_context* ctx, int64_t n_knots);
+/* The context's mapping table, per call (batched = 0) or per batch: up to
+ * `capacity` entries (largest size in knots, lanes per knot), ascending;
+ * returns the table's length (0: the thresholds decide), -1 for a null
+ * context.  No reference counterpart. */
+int rbd_mapping_table(rbd_context* ctx, int batched, int64_t* n_knots, int* lanes, int capacity);
 
 /* ------------------------------------------------------------------------
  * Batched trajectories of uneven length.
@@ -275,28 +313,35 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
 rbd_status rbd_batch_destroy(rbd_batch* b);
 
 /* Run the forward-dynamics gradient over n_traj trajectories on `stream`.
- * Up to 64 trajectories travel in the kernel's arguments (no upload, unless
- * use_graph); more are copied to the device asynchronously on `stream`.  The
- * caller's trajs array may be reused on return.  The knot-to-lane mapping is
- * chosen from the total knot count against the context's batch crossover
- * (rbd_autotune_batch), unless the context fixes lanes_per_knot (32: the
- * two-wave mapping, 4-knot chunks on workgroups of two waves).
- * use_graph != 0 replays the batch's kernels from a hipGraph built on first
- * use (one graph per mapping and batch kind; for up to 64 trajectories a
- * single kernel node whose arguments and grid are set before each replay).  Calls on one batch object are
- * ordered by their streams: two streams must not run the same batch object at
- * once.  Returns RBD_ERR_CAPACITY if n_traj exceeds the batch's
+ * Up to 64 trajectories travel in the kernel's arguments (no upload); more
+ * are copied to the device asynchronously on `stream`.  The caller's trajs
+ * array may be reused on return.  The knot-to-lane mapping is chosen from the
+ * total knot count by the context's batch mapping table (rbd_autotune_batch),
+ * unless the context fixes lanes_per_knot (32: the two-wave mapping, 4-knot
+ * chunks on workgroups of two waves; 4, 7, 8: chunks of 16, 9, 8 knots of the
+ * batch's knots taken in order across trajectories).
+ * use_graph != 0 replays the batch's kernels from a hipGraph.  Up to 64
+ * trajectories: a graph of the one kernel with this trajectory list in its
+ * arguments, captured when the list (buffers and lengths) differs from the
+ * previous use_graph call's of the same mapping and replayed unchanged when it
+ * is the same (new inputs written into the same buffers need no new capture).
+ * More: a graph captured once per mapping, reading the descriptors the call
+ * copies to the device first.  On ROCm 7.2 a graph launch costs about 15 us
+ * of host time against 2.6 us for a direct launch, so direct launches are the
+ * faster form where the host issues every batch.  Calls on one batch object
+ * are ordered by their streams: two streams must not run the same batch
+ * object at once.  Returns RBD_ERR_CAPACITY if n_traj exceeds the batch's
  * max_trajectories. */
 rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph,
                                   rbd_stream_t stream);
 
-/* Persistent grid size (waves) per mapping, lanes_per_knot 1 or 16 (default:
- * the waves the device's SIMDs keep resident, 4 per CU times 2 with FP32
- * columns or 1 with FP64 columns; the two-wave mapping runs 2 workgroups of
- * two waves per CU). */
+/* Persistent grid size (waves) per mapping, lanes_per_knot 1, 4, 7, 8 or 16
+ * (default: the waves the device's SIMDs keep resident, 4 per CU times 2 with
+ * FP32 columns or 1 with FP64 columns for 1 and 16 lanes, 4 per CU for 4, 7
+ * and 8; the two-wave mapping runs 2 workgroups of two waves per CU). */
 rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves);
 
-/* Lanes per knot used by the last rbd_fdgrad_batched_f32 call (1, 16 or 32). */
+/* Lanes per knot used by the last rbd_fdgrad_batched_f32 call (1, 4, 7, 8, 16 or 32). */
 int rbd_batch_last_lanes(const rbd_batch* b);
 
 /* ------------------------------------------------------------------------
@@ -319,7 +364,7 @@ typedef struct rbd_server rbd_server;
 
 typedef struct rbd_server_config {
   int slots;                      /* batches in flight (one stream each), 1..64 */
-  int lanes_per_knot;             /* 1, 16 or 32 (two waves per knot group), used for every batch */
+  int lanes_per_knot;             /* 1, 4, 7, 8, 16 or 32 (two waves per knot group), used for every batch */
   int64_t memory_budget_bytes;    /* device memory for admitted requests (and as much pinned staging) */
   int64_t max_batch_trajectories; /* per batch */
   int64_t max_batch_knots;        /* per batch; also the longest admissible request */
@@ -328,7 +373,7 @@ typedef struct rbd_server_config {
   int zero_copy;                  /* 0: requests' buffers in device memory, one upload and one download per batch;
                                      1: the kernels read inputs from and write results to the pinned staging
                                      directly (no copies: lower latency, PCIe traffic from the kernels, which caps
-                                     the server at about 4.7e7 knots/s);
+                                     the server at about 9e7 knots/s);
                                      2 (the default): per batch -- zero-copy up to zero_copy_max_knots knots,
                                      copies above */
   int poll_dispatch;              /* 1: the dispatcher thread polls for requests instead of sleeping (one busy

@@ -4,6 +4,7 @@
 // kernel's arithmetic, in every precision mode, against the oracle before any
 // GPU run; it is not a product path and nothing outside tests/ loads it.
 #include "../../fast-rbd-gradients_amd/csrc/fdgrad_knot.hpp"
+#include "../../fast-rbd-gradients_amd/csrc/group_mapping.hpp"
 
 #include <cmath>
 
@@ -82,6 +83,33 @@ static void two_wave(long n, const float* in, float* out, int flags) {
     for (int role = 0; role < kLanesPerKnotGroup; ++role) two_wave_r2<P>(role, sh, tau[role], o);
   }
 }
+// The L-lane group mapping (group_mapping.hpp), emulated: stage 1 of every
+// lane role of the knot (its share of Minv into the group's block), then
+// stage 2 of every role (its tasks' stores).
+template <class P, int L>
+static void group(long n, const float* in, float* out, int flags) {
+  const Options opt = options(flags);
+  for (long k = 0; k < n; ++k) {
+    InAoS i{in + kInPerKnot * k};
+    OutAoS o{out + kOutPerKnot * k};
+    typename P::Minv mi[kMinvPacked];
+    CS<KnotTrig<P>> tw[L][NQ];
+    typename P::Minv Mi[L][NQ][NQ];
+    for (int role = 0; role < L; ++role) group_stage1<P, L>(i, 0, role, true, tw[role], mi, Mi[role]);
+    for (int role = 0; role < L; ++role) group_stage2<P, L>(i, o, opt, role, tw[role], mi, Mi[role]);
+  }
+}
+template <int L>
+static void group_prec(long n, const float* in, float* out, int flags, int prec) {
+  switch (prec) {
+    case 0: group<PrecFP32, L>(n, in, out, flags); break;
+    case 1: group<PrecFP64Pivot, L>(n, in, out, flags); break;
+    case 2: group<PrecFP64Col, L>(n, in, out, flags); break;
+    case 3: group<PrecFP64, L>(n, in, out, flags); break;
+    default: break;
+  }
+}
+
 extern "C" {
 // The device model tables: spatial inertia of link k (6x6 column-major) and the
 // joint transform T_k(q) (6x6 column-major, built by applying T to unit vectors).
@@ -248,5 +276,28 @@ void host_fdgrad_f32(long n, const float* in, float* out, int flags, int prec, i
 }
 void host_fdgrad_refmode_f32(long n, const float* vol, float* out, int flags, int prec, int lanes) {
   by_precision(prec, true, flags, n, vol, out, lanes);
+}
+
+void host_fdgrad_group_f32(long n, const float* in, float* out, int flags, int prec, int lanes) {
+  if (lanes == 4) group_prec<4>(n, in, out, flags, prec);
+  else if (lanes == 7) group_prec<7>(n, in, out, flags, prec);
+  else if (lanes == 8) group_prec<8>(n, in, out, flags, prec);
+}
+
+// The two-wave handoff wait's bound (bounded_wait, the loop of lds_wait) with
+// a flag that is set once `set_after` polls have seen it clear (set_after < 0:
+// never): returns 1 if the wait saw it, and the polls and pauses it made.
+int host_bounded_wait(int spins, int set_after, int* polls, int* pauses) {
+  int p = 0, z = 0;
+  const bool seen = bounded_wait(
+      [&] {
+        const bool set = set_after >= 0 && p >= set_after;
+        ++p;
+        return set;
+      },
+      [&] { ++z; }, spins);
+  *polls = p;
+  *pauses = z;
+  return seen ? 1 : 0;
 }
 }

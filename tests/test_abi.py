@@ -57,8 +57,8 @@ def test_library_is_built_for_gfx950(model):
 def test_status_strings_and_config_default():
     import rbdgrad
     L = rbdgrad.lib()
-    assert L.rbd_abi_version() == 1
-    for s, text in [(0, "ok"), (1, "invalid argument"), (3, "no HIP device")]:
+    assert L.rbd_abi_version() == 2
+    for s, text in [(0, "ok"), (1, "invalid argument"), (3, "no HIP device"), (7, "device wait timed out")]:
         assert L.rbd_status_string(s).decode() == text
     cfg = rbdgrad.Config()
     L.rbd_config_default(ctypes.byref(cfg))
@@ -84,4 +84,12 @@ def test_invalid_arguments_are_reported_without_a_device():
     assert L.rbd_fdgrad_traj_soa_f32(None, 1, None, 1, None, 1, None) == 1
     assert L.rbd_dynamics_gradient_kernel_f32(None, 1, None, None, None, None) == 1
     assert L.rbd_fdgrad_traj_host_f32(None, 1, None, None) == 1
+    assert L.rbd_context_status(None) == 1
+    assert L.rbd_context_clear_status(None) == 1
+    assert L.rbd_set_handoff_spins(None, 0) == 1
+    assert L.rbd_mapping_table(None, 0, None, None, 0) == -1
+    cfg.flags = 0
+    cfg.lanes_per_knot = 5
+    assert L.rbd_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
+    assert "lanes_per_knot" in L.rbd_last_error().decode()
     assert L.rbd_destroy(None) == 0

@@ -27,6 +27,18 @@ def download(trajs):
     return [d_out.download((n, 105)) for _, d_out, n in trajs]
 
 
+def same_as_flat(batched, flat, lanes):
+    """A batch's trajectory against the per-call kernel of the same mapping:
+    bitwise for one lane and 16-lane groups (the same kernel code); within
+    1e-5 per block for the 4-, 7- and 8-lane groups, whose batch kernel is
+    another compilation of the same arithmetic (FMA contraction may differ)."""
+    if lanes in (1, 16):
+        np.testing.assert_array_equal(batched, flat)
+    else:
+        err = common.max_block_errors(batched, flat.astype(np.float64))
+        assert max(err.values()) < 1e-5, err
+
+
 @pytest.fixture(scope="module")
 def ctx(rbd):
     c = rbd.Context(precision="fp32")
@@ -42,11 +54,12 @@ def test_uneven_batch_matches_per_trajectory_calls(rbd, ctx, oracle, use_graph):
     b = rbd.Batch(ctx, 256)
     b.run(trajs, use_graph=use_graph)
     rbd.hip().hipDeviceSynchronize()
-    assert b.last_lanes in (1, 16)
+    total = int(lengths.sum())
+    assert b.last_lanes == next(l for n, l in ctx.mapping_table(batched=True) if total <= n)
     got = download(trajs)
     flat_ctx = rbd.Context(precision="fp32", lanes_per_knot=b.last_lanes)
     for x, g in zip(xs, got):
-        np.testing.assert_array_equal(g, flat_ctx.run_aos(x))
+        same_as_flat(g, flat_ctx.run_aos(x), b.last_lanes)
     allx = np.concatenate(xs)
     err = common.max_block_errors(np.concatenate(got), oracle.fdgrad(allx.astype(np.float64), threads=8))
     assert max(err.values()) < common.REL_TOL, err
@@ -58,7 +71,7 @@ def test_graph_replays_with_changing_batches(rbd, ctx):
     """Replay the captured graph several times with different trajectory sets,
     with eager work and synchronisation between replays; check every replay."""
     b = rbd.Batch(ctx, 128)
-    flat = {lanes: rbd.Context(precision="fp32", lanes_per_knot=lanes) for lanes in (1, 16)}
+    flat = {lanes: rbd.Context(precision="fp32", lanes_per_knot=lanes) for lanes in rbd.AUTOTUNE_LANES}
     rng = np.random.default_rng(12)
     for rep in range(6):
         lengths = rng.integers(1, 200, rng.integers(1, 100))
@@ -68,7 +81,7 @@ def test_graph_replays_with_changing_batches(rbd, ctx):
         lanes = b.last_lanes                                          # the mapping this batch ran
         flat[1].run_aos(common.knots(33, seed=rep))                   # eager work in between
         for x, g in zip(xs, download(trajs)):
-            np.testing.assert_array_equal(g, flat[lanes].run_aos(x))
+            same_as_flat(g, flat[lanes].run_aos(x), lanes)
         say(f"replay {rep}: {len(lengths)} trajectories, {sum(lengths)} knots, lanes {lanes} ok")
     for c in flat.values():
         c.close()
@@ -96,16 +109,19 @@ def test_small_grid_still_drains_the_queue(rbd, ctx):
     until the queue is empty, no knot is skipped."""
     lengths = [300, 5, 129, 64, 63]
     xs, trajs = make_batch(rbd, lengths, seed=4)
-    b = rbd.Batch(ctx, 8)
-    b.set_grid(16, 3)
-    b.set_grid(1, 2)
-    b.run(trajs)
-    rbd.hip().hipDeviceSynchronize()
-    flat = rbd.Context(precision="fp32", lanes_per_knot=b.last_lanes)
-    for x, g in zip(xs, download(trajs)):
-        np.testing.assert_array_equal(g, flat.run_aos(x))
-    flat.close()
-    b.close()
+    for lanes in (16, 1, 7, 4):
+        c = rbd.Context(precision="fp32", lanes_per_knot=lanes)
+        b = rbd.Batch(c, 8)
+        b.set_grid(lanes, 3 if lanes != 1 else 2)
+        b.run(trajs)
+        rbd.hip().hipDeviceSynchronize()
+        assert b.last_lanes == lanes
+        flat = rbd.Context(precision="fp32", lanes_per_knot=lanes)
+        for x, g in zip(xs, download(trajs)):
+            same_as_flat(g, flat.run_aos(x), lanes)
+        flat.close()
+        b.close()
+        c.close()
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
@@ -125,11 +141,11 @@ def test_alternating_small_and_large_trajectory_counts(rbd, ctx, use_graph):
         b.run(trajs, use_graph=use_graph)
         runs.append((xs, trajs, b.last_lanes, lengths))
     rbd.hip().hipDeviceSynchronize()
-    flat = {lanes: rbd.Context(precision="fp32", lanes_per_knot=lanes) for lanes in (1, 16)}
+    flat = {lanes: rbd.Context(precision="fp32", lanes_per_knot=lanes) for lanes in rbd.AUTOTUNE_LANES}
     for xs, trajs, lanes, lengths in runs:
         for x, g, n in zip(xs, download(trajs), lengths):
             if n:
-                np.testing.assert_array_equal(g, flat[lanes].run_aos(x))
+                same_as_flat(g, flat[lanes].run_aos(x), lanes)
         say(f"{len(lengths)} trajectories, {int(sum(lengths))} knots, lanes {lanes} ok")
     for c in flat.values():
         c.close()
@@ -217,16 +233,17 @@ def test_two_wave_batches(rbd, oracle, use_graph):
 
 def test_batch_autotune(rbd):
     c = rbd.Context(precision="fp32")
+    assert [l for _, l in c.mapping_table(batched=True)] == [7, 4, 1]   # the built-in table
     thr, table = c.autotune_batch()
     sizes = sorted(table)
-    assert len(sizes) == 9 and sizes[0] == 8 * 132 and sizes[-1] == 512 * 132
-    assert all(a > 0 and g > 0 for a, g in table.values())
+    assert len(sizes) == 11 and sizes[0] == 8 * 132 and sizes[-1] == 512 * 132
+    assert all(min(t.values()) > 0 and set(t) == set(rbd.AUTOTUNE_LANES) for t in table.values())
     assert c.batch_group_threshold == thr
-    faster = [n for n in sizes if table[n][1] < table[n][0]]
-    if faster and faster[0] == sizes[0]:
-        assert thr >= sizes[0]
-    else:
-        assert thr == 0
+    assert thr == max([0] + [n for n in sizes if table[n][16] < table[n][1]])
+    installed = c.mapping_table(batched=True)
+    assert [n for n, _ in installed] == sizes
+    for n, lanes in installed:
+        assert table[n][lanes] == min(table[n].values())
     say(f"batch crossover {thr} knots; table {table}")
     c.close()
 
@@ -238,5 +255,106 @@ def test_batch_errors(rbd, ctx):
         b.run(trajs)
     b.run([])                                     # empty batch is a no-op
     with pytest.raises(rbd.RbdError):
-        b.set_grid(4, 10)
+        b.set_grid(5, 10)
     b.close()
+
+
+def run_batches(rbd, b, batches, use_graph, stream=None):
+    """Launch every batch back to back (buffers allocated and filled first),
+    synchronise once, return each batch's outputs and the mappings used."""
+    lanes = []
+    for xs, trajs in batches:
+        b.run(trajs, use_graph=use_graph, stream=stream)
+        lanes.append(b.last_lanes)
+    rbd.hip().hipDeviceSynchronize()
+    return [download(trajs) for _, trajs in batches], lanes
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+@pytest.mark.parametrize("lanes", [4, 7, 8])
+def test_group_mapping_batches(rbd, oracle, lanes, use_graph):
+    """lanes_per_knot 4, 7, 8: chunks of floor(64 / L) knots of the batch's
+    flattened knots, so a chunk spans trajectories.  Batches of up to 64
+    trajectories (each lane finds its trajectory by a shuffle search) and of
+    more (plan kernel's knot prefix sum), zero- and one-knot trajectories,
+    back to back without synchronisation, more chunks than the grid.  Every
+    trajectory is bitwise the same in another batch composition (reversed
+    order), within 1e-5 per block of the flat kernel of the same mapping and
+    within the 1e-4 bar of the oracle."""
+    c = rbd.Context(precision="fp32", lanes_per_knot=lanes)
+    b = rbd.Batch(c, 256)
+    rng = np.random.default_rng(60 + lanes)
+    specs = [(64, 257), (3, 40), (200, 120), (65, 90), (10, 2000), (1, 5), (64, 30)]
+    batches = []
+    for rep, (n_traj, hi) in enumerate(specs):
+        lengths = rng.integers(0, hi, n_traj)
+        lengths[0] = max(lengths[0], 1)
+        if n_traj > 3:
+            lengths[1:3] = [0, 1]
+        batches.append(make_batch(rbd, lengths, seed=600 + 10 * lanes + rep))
+    got, used = run_batches(rbd, b, batches, use_graph)
+    assert used == [lanes] * len(specs)
+    # the same trajectories in reverse order, another composition of every chunk
+    rev = [(xs[::-1], [(d_in, rbd.DeviceBuffer(max(1, n) * 105 * 4), n) for d_in, _, n in trajs[::-1]])
+           for xs, trajs in batches]
+    got_rev, _ = run_batches(rbd, b, rev, use_graph)
+    for (xs, trajs), g, gr in zip(batches, got, got_rev):
+        for x, a, r in zip(xs, g, gr[::-1]):
+            if len(x):
+                np.testing.assert_array_equal(a, r)
+                err = common.max_block_errors(a, c.run_aos(x))
+                assert max(err.values()) < 1e-5, err
+        allx = np.concatenate([x for x in xs if len(x)])
+        err = common.max_block_errors(np.concatenate([a for x, a in zip(xs, g) if len(x)]),
+                                      oracle.fdgrad(allx.astype(np.float64), threads=8))
+        assert max(err.values()) < common.REL_TOL, err
+        say(f"lanes {lanes}: {len(xs)} trajectories, {len(allx)} knots ok")
+    b.close()
+    c.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 7, 16])
+def test_small_graph_replays_back_to_back(rbd, lanes):
+    """Graph replays of small batches (up to 64 trajectories: the batch goes
+    into the one kernel node's arguments before each replay) enqueued back to
+    back on one non-default stream with no host synchronisation between them:
+    each replay runs with its own arguments and grid.  Buffers are allocated
+    and filled first, so nothing else synchronises.  Then 100,000 replays of one
+    list, and 20,000 calls alternating between two lists (a new capture each),
+    leave the device's free memory within one 8 MiB pool step of where it was:
+    nothing is allocated per replay (rewriting a graph's kernel arguments
+    instead kept about 34 KB per replay, 3.9 GB over 100,000)."""
+    c = rbd.Context(precision="fp32", lanes_per_knot=lanes)
+    b = rbd.Batch(c, 64)
+    s = rbd.Stream()
+    rng = np.random.default_rng(70 + lanes)
+    batches = [make_batch(rbd, rng.integers(1, 150, int(rng.integers(1, 65))), seed=700 + 10 * lanes + i)
+               for i in range(10)]
+    got, used = run_batches(rbd, b, batches, True, stream=s)
+    assert used == [lanes] * 10
+    for (xs, trajs), g in zip(batches, got):
+        # the same batch launched directly (no graph): bitwise the same
+        b.run(trajs, use_graph=False, stream=s)
+        s.synchronize()
+        for a, d in zip(g, download(trajs)):
+            np.testing.assert_array_equal(a, d)
+    tl = [rbd.TrajList(batches[0][1]), rbd.TrajList(batches[1][1])]
+    for i in range(100):
+        b.run(tl[i % 2], use_graph=True, stream=s)
+    s.synchronize()
+    free0 = rbd.mem_info()[0]
+    for i in range(100000):
+        b.run(tl[0], use_graph=True, stream=s)
+        if i % 5000 == 0:
+            s.synchronize()
+    s.synchronize()
+    free1 = rbd.mem_info()[0]
+    for i in range(20000):
+        b.run(tl[i % 2], use_graph=True, stream=s)
+    s.synchronize()
+    free2 = rbd.mem_info()[0]
+    say(f"lanes {lanes}: free device memory {free0} -> {free1} (100,000 replays) -> {free2} (20,000 captures)")
+    assert free1 >= free0 - (8 << 20) and free2 >= free1 - (8 << 20)
+    s.destroy()
+    b.close()
+    c.close()

@@ -41,10 +41,14 @@ def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_c
     st = srv.stats()
     say("stats", st)
     assert st["requests_completed"] == 302
-    for x, o in zip(others[::37], outs[::37]):
-        err = common.max_block_errors(o, oracle.fdgrad(x.astype(np.float64)))
-        assert max(err.values()) < common.REL_TOL, err
-    # the same probe through the plain per-call path with the same mapping:
+    # every one of the 300 against the oracle, and bitwise against the same
+    # request served again on its own
+    ref = oracle.fdgrad(np.concatenate(others).astype(np.float64), threads=8)
+    err = common.max_block_errors(np.concatenate(outs), ref)
+    assert max(err.values()) < common.REL_TOL, err
+    for x, o in zip(others, outs):
+        np.testing.assert_array_equal(srv.wait(srv.submit(x)), o)
+    # the same requests through the plain per-call path with the same mapping:
     # identical (the two-wave batch kernel compiles the same arithmetic into
     # another kernel, so there FP32 rounding may differ)
     flat = rbd.Context(precision="fp32", lanes_per_knot=lanes)
@@ -53,6 +57,8 @@ def test_result_is_the_same_alone_and_in_the_mix(rbd, ctx, oracle, lanes, zero_c
         assert max(err.values()) < 2e-5, err
     else:
         np.testing.assert_array_equal(flat.run_aos(probe), alone)
+        for x, o in zip(others, outs):
+            np.testing.assert_array_equal(flat.run_aos(x), o)
     flat.close()
     srv.close()
 
@@ -125,3 +131,21 @@ def test_server_config_errors(rbd, ctx):
     for bad in ({"zero_copy": 3}, {"zero_copy": -1}, {"zero_copy_max_knots": -5}):
         with pytest.raises(rbd.RbdError):
             rbd.Server(ctx, **bad)
+
+
+def test_handoff_timeout_fails_the_requests(rbd):
+    """Two-wave batches of a server whose context forces every handoff wait to
+    give up (a host-set bound; every kernel still runs to its end): each
+    request fails with RBD_ERR_DEVICE_TIMEOUT instead of returning the invalid
+    results, and the server refuses new requests."""
+    ctx = rbd.Context(precision="fp32")
+    ctx.set_handoff_spins(-1)
+    srv = rbd.Server(ctx, lanes_per_knot=32, slots=2, zero_copy=2, poll_dispatch=1)
+    rid = srv.submit(common.knots(64, seed=700))
+    with pytest.raises(rbd.RbdError, match="device wait timed out"):
+        srv.wait(rid)
+    with pytest.raises(rbd.RbdError, match="device wait timed out"):
+        srv.submit(common.knots(64, seed=710))
+    assert srv.stats()["requests_completed"] == 0
+    srv.close()
+    ctx.close()

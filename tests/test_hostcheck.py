@@ -244,3 +244,40 @@ def test_minv_by_columns_equals_the_whole_recursion(host):
     full, cols = full.reshape(200, 49)[:, upper], cols.reshape(200, 49)[:, upper]
     assert np.all(np.isfinite(full))
     np.testing.assert_array_equal(cols, full)
+
+
+@pytest.mark.parametrize("spins,set_after,seen,polls,pauses", [
+    (10, 0, 1, 1, 0),         # flag already set: one poll, no sleep
+    (10, 3, 1, 4, 3),         # set after three clear polls: seen on the fourth
+    (10, 10, 1, 11, 10),      # seen on the last poll the bound allows
+    (10, 11, 0, 11, 10),      # one poll too late: the wait gives up
+    (10, -1, 0, 11, 10),      # never set: gives up after spins + 1 polls
+    (0, -1, 0, 1, 0),         # spins 0: one poll
+    (-1, 0, 0, 0, 0),         # spins < 0: gives up at once without polling
+])
+def test_handoff_wait_bound(host, spins, set_after, seen, polls, pauses):
+    """The two-wave mapping's LDS handoff wait (bounded_wait, the loop of
+    lds_wait): it returns as soon as the flag is seen, and gives up -- which
+    makes the kernel write its context's failure word -- after spins + 1
+    polls, or at once for spins < 0."""
+    p, z = ctypes.c_int(-1), ctypes.c_int(-1)
+    assert host.host_bounded_wait(spins, set_after, ctypes.byref(p), ctypes.byref(z)) == seen
+    assert (p.value, z.value) == (polls, pauses)
+
+
+@pytest.mark.parametrize("lanes", [4, 7, 8])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+@pytest.mark.parametrize("flags", [0, 3])
+def test_group_mapping(host, oracle, flags, prec, lanes):
+    """The L-lane group mappings (4, 7, 8 lanes per knot: each lane a share of
+    the joints' sin/cos, of Minv and of the 14 column tasks) give every output
+    exactly as the 16-lane groups do -- the same arithmetic, split another way
+    -- and every output is written; against the oracle at random and
+    stretched-arm poses."""
+    n = 200
+    x = np.ascontiguousarray(common.knots(n, seed=295 + flags, near_singular_frac=0.2))
+    out = np.full((n, 105), np.nan, np.float32)
+    host.host_fdgrad_group_f32(n, _p(x), _p(out), flags, prec, lanes)
+    np.testing.assert_array_equal(out, run(host, x, flags, prec, lanes=1))
+    err = common.max_block_errors(out, oracle.fdgrad(x.astype(np.float64), flags))
+    assert max(err.values()) < (common.REL_TOL if prec in (0, 1) else 1e-5), err
