@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -139,6 +140,18 @@ void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
   }
 }
 
+// The persistent grids: as many waves as the SIMDs (4 per CU) keep resident --
+// two per SIMD with FP32 columns, one with FP64 columns; two-wave workgroups
+// at 256 registers, one wave per SIMD; the L-lane groups one wave per SIMD.
+void set_default_grid(rbd_batch* b) {
+  const rbd_context* ctx = b->ctx;
+  const int waves = (ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT ||
+                     ctx->precision == RBD_PREC_FP32_HW_TRIG) ? 2 : 1;
+  b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4 * waves);
+  b->grid[2] = unsigned(ctx->compute_units * 2);
+  b->grid[3] = b->grid[4] = b->grid[5] = unsigned(ctx->compute_units * 4);
+}
+
 // Forward declaration: the direct launch, which the small-batch graphs capture.
 hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hipStream_t s);
 
@@ -272,7 +285,49 @@ rbd_status ensure_graph(rbd_batch* b, int lanes, bool small) {
   return RBD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The allocation-free batched call (rbd_fdgrad_batched_noalloc_f32).  Its one
+// piece of device memory beyond the caller's inputs and outputs is the queue's
+// last-round counters (kTailWords words, 8,320 bytes), kept in this file-scope
+// array: each device the library runs on holds its own copy until the process
+// ends.  In host code `scratch` names a host-side stand-in, not the device
+// copy, so its device address is looked up once per device
+// (hipGetSymbolAddress, at the first rbd_create on that device) and only that
+// address is used.  Every call on a device shares its counters; the kernels
+// leave them at zero after each launch, so calls in stream order need no
+// clearing, but calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned long long scratch[rbd::kTailWords];
+
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+__global__ void clear_scratch_kernel(unsigned long long* p, int n) {
+  const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) p[i] = 0ull;
+}
+
 }  // namespace
+
+// Once per device, with that device current: the device address of its copy
+// of `scratch`, cleared by a kernel (the counters start at zero).
+rbd_status rbdi::noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(RBD_ERR_UNSUPPORTED, "device ordinal beyond the no-allocation table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return RBD_OK;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress (no-allocation scratch)");
+  unsigned long long* p = static_cast<unsigned long long*>(ptr);
+  hipLaunchKernelGGL(clear_scratch_kernel, dim3((rbd::kTailWords + 255) / 256), dim3(256), 0, nullptr, p,
+                     rbd::kTailWords);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "clearing the no-allocation scratch");
+  g_scratch_ptr[device] = p;
+  return RBD_OK;
+}
 
 extern "C" {
 
@@ -314,13 +369,7 @@ rbd_status rbd_batch_create(rbd_context* ctx, int max_trajectories, rbd_batch** 
     rbd_batch_destroy(b);
     return hip_fail(e, "batch staging");
   }
-  // persistent grid: as many waves as the SIMDs (4 per CU) keep resident --
-  // two per SIMD with FP32 columns, one with FP64 columns
-  const int waves = (ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT ||
-                     ctx->precision == RBD_PREC_FP32_HW_TRIG) ? 2 : 1;
-  b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4 * waves);
-  b->grid[2] = unsigned(ctx->compute_units * 2);   // two-wave workgroups (256 registers: one wave per SIMD)
-  b->grid[3] = b->grid[4] = b->grid[5] = unsigned(ctx->compute_units * 4);   // L-lane groups: one wave per SIMD
+  set_default_grid(b);
   *out = b;
   return RBD_OK;
 }
@@ -471,6 +520,46 @@ rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, const r
 extern "C" {
 
 int rbd_batch_last_lanes(const rbd_batch* b) { return b ? b->last_lanes : 0; }
+
+rbd_status rbd_fdgrad_batched_noalloc_f32(rbd_context* ctx, const rbd_traj* trajs, int n_traj, rbd_stream_t stream) {
+  if (rbd_status s = check_ctx(ctx)) return s;
+  if (n_traj < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative trajectory count");
+  if (n_traj > rbd::kSmallBatch)
+    return fail(RBD_ERR_CAPACITY, "rbd_fdgrad_batched_noalloc_f32 takes at most 64 trajectories");
+  if (n_traj > 0 && !trajs) return fail(RBD_ERR_INVALID_ARGUMENT, "null trajectory list");
+  int64_t total = 0;
+  for (int t = 0; t < n_traj; ++t) {
+    if (trajs[t].n_knots < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative trajectory length");
+    if (trajs[t].n_knots > 0 && (!trajs[t].d_in || !trajs[t].d_out))
+      return fail(RBD_ERR_INVALID_ARGUMENT, "null trajectory buffer");
+    total += trajs[t].n_knots;
+  }
+  if ((total + 3) / 4 + n_traj > kMaxBatchChunks) return fail(RBD_ERR_CAPACITY, "batch too large (more than 2^31 chunks)");
+  if (total == 0) return RBD_OK;
+  if (rbd_status s = ready_to_launch(ctx)) return s;
+  unsigned long long* tail = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < kMaxDevices) tail = g_scratch_ptr[ctx->device];
+  }
+  if (!tail) return fail(RBD_ERR_UNSUPPORTED, "no-allocation scratch not set up on this device");
+  const int lanes = lanes_for_batch(ctx, total);
+  // the batch in the kernel arguments, the counters in the device's scratch,
+  // the grid as a batch object's: nothing is allocated
+  rbd_batch b;
+  b.ctx = ctx;
+  b.d_tail = tail;
+  set_default_grid(&b);
+  rbd::ArgBatch a{};
+  for (int t = 0; t < n_traj; ++t) a.d[t] = rbd::TrajDesc{trajs[t].d_in, trajs[t].d_out, long(trajs[t].n_knots), 0};
+  a.n_traj = n_traj;
+  a.total_chunks = int(chunk_total(n_traj, [trajs](int t) { return int64_t(trajs[t].n_knots); }, lanes));
+  Queue q{nullptr, nullptr, nullptr};
+  q.args = &a;
+  q.chunks = a.total_chunks;
+  const hipError_t e = launch_batch(&b, q, lanes, true, (hipStream_t)stream);
+  return e == hipSuccess ? RBD_OK : hip_fail(e, "batch kernel launch (no allocation)");
+}
 
 // Times every mapping of the batched path (kAutotuneLanes) on batches of
 // 132-knot trajectories (the mean length of lengths uniform over 8..256), 8 to

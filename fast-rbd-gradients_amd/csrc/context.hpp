@@ -168,6 +168,10 @@ rbd_status batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_gr
 // last wave writes done_seq there once all outputs are visible to the host,
 // so the caller can poll memory instead of an event.  A batch of zero knots
 // launches nothing and writes nothing.
+// Once per device, with that device current (rbd_create does it): the device
+// address of the no-allocation batched call's scratch (batch.hip).
+rbd_status noalloc_setup(int device);
+
 size_t batch_stage_bytes(int n_traj);
 int64_t batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int lanes);
 rbd_status batch_launch_staged(rbd_batch* b, const void* d_staged, const rbd_traj* trajs, int n_traj,

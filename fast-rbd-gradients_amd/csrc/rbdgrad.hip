@@ -444,11 +444,13 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
     e = hipHostMalloc(&word, 64, hipHostMallocMapped | hipHostMallocCoherent | hipHostMallocPortable);
   void* dword = nullptr;
   if (e == hipSuccess) e = hipHostGetDevicePointer(&dword, word, 0);
+  // the no-allocation batched call's scratch on this device (once per device)
+  const rbd_status ns = e == hipSuccess ? noalloc_setup(c.device) : RBD_OK;
   if (cur >= 0 && cur != c.device) (void)hipSetDevice(cur);
-  if (e != hipSuccess) {
+  if (e != hipSuccess || ns != RBD_OK) {
     if (word) (void)hipHostFree(word);
     delete ctx;
-    return hip_fail(e, "rbd_create: handoff-failure word");
+    return ns != RBD_OK ? ns : hip_fail(e, "rbd_create: handoff-failure word");
   }
   *static_cast<volatile unsigned*>(word) = 0u;
   ctx->fault_host = static_cast<volatile unsigned*>(word);

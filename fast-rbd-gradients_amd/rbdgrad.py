@@ -55,6 +55,7 @@ EXPORTED = [
     "rbd_dynamics_gradient_kernel_fused_f32", "rbd_dynamics_gradient_kernel_split_f32",
     "rbd_gradient_setup_split_f32", "rbd_gradient_finish_f32", "rbd_model_name", "rbd_fdgrad_wafr_f32",
     "rbd_context_status", "rbd_context_clear_status", "rbd_set_handoff_spins", "rbd_mapping_table",
+    "rbd_fdgrad_batched_noalloc_f32",
 ]
 
 # the mappings the autotuners compare, in the order of their table rows (lanes per knot)
@@ -157,6 +158,7 @@ def lib(model=None):
         L.rbd_batch_destroy.argtypes = [vp]
         L.rbd_fdgrad_batched_f32.argtypes = [vp, ctypes.POINTER(Traj), ctypes.c_int, ctypes.c_int, vp]
         L.rbd_batch_set_grid.argtypes = [vp, ctypes.c_int, ctypes.c_uint]
+        L.rbd_fdgrad_batched_noalloc_f32.argtypes = [vp, ctypes.POINTER(Traj), ctypes.c_int, vp]
         L.rbd_batch_last_lanes.argtypes = [vp]
         L.rbd_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
         L.rbd_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
@@ -393,6 +395,14 @@ class Context(_Bound):
     def set_handoff_spins(self, spins):
         """Bound of the two-wave handoff waits (< 0: every wait gives up at once, a test hook)."""
         self._check(self._L.rbd_set_handoff_spins(self._h, int(spins)), "rbd_set_handoff_spins")
+
+    def batched_noalloc(self, trajs, stream=None):
+        """rbd_fdgrad_batched_noalloc_f32: up to 64 trajectories (a TrajList or
+        [(d_in, d_out, n_knots), ...]) with nothing allocated by the library."""
+        if not isinstance(trajs, TrajList):
+            trajs = TrajList(trajs)
+        self._check(self._L.rbd_fdgrad_batched_noalloc_f32(self._h, trajs.arr, trajs.n, _sh(stream)),
+                    "rbd_fdgrad_batched_noalloc_f32")
 
     def kernel_name(self, n_knots, reference_mode=False):
         return self._L.rbd_kernel_name(self._h, int(bool(reference_mode)), int(n_knots)).decode()

@@ -335,6 +335,19 @@ rbd_status rbd_batch_destroy(rbd_batch* b);
 rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph,
                                   rbd_stream_t stream);
 
+/* The batched call without a batch object, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * device allocation): up to 64 trajectories (RBD_ERR_CAPACITY beyond), their
+ * descriptors in the kernel's arguments, the mapping from the context's batch
+ * table as for rbd_fdgrad_batched_f32, the results bitwise the same.  It calls
+ * no allocating routine: the queue's last-round counters (8,320 bytes) live in
+ * a file-scope device array of the library, one per device, set up by the
+ * first rbd_create on that device and held until the process ends.  Every
+ * call on a device shares that array: calls in one stream's order are safe,
+ * calls that overlap on one device (several streams or threads at once) race.
+ * No counterpart in the reference; for callers that cannot allocate. */
+rbd_status rbd_fdgrad_batched_noalloc_f32(rbd_context* ctx, const rbd_traj* trajs, int n_traj, rbd_stream_t stream);
+
 /* Persistent grid size (waves) per mapping, lanes_per_knot 1, 4, 7, 8 or 16
  * (default: the waves the device's SIMDs keep resident, 4 per CU times 2 with
  * FP32 columns or 1 with FP64 columns for 1 and 16 lanes, 4 per CU for 4, 7
