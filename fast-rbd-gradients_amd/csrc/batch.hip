@@ -140,14 +140,17 @@ void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
   }
 }
 
-// The persistent grids: as many waves as the SIMDs (4 per CU) keep resident --
-// two per SIMD with FP32 columns, one with FP64 columns; two-wave workgroups
-// at 256 registers, one wave per SIMD; the L-lane groups one wave per SIMD.
+// The persistent grids: as many waves as a CU keeps resident -- with one lane
+// per knot 6 (the 26,880-byte record blocks fill the LDS) with FP32 columns,
+// 4 (one per SIMD) with FP64 columns; with 16-lane groups two per SIMD with
+// FP32 columns, one with FP64 columns; two-wave workgroups at 256 registers,
+// one wave per SIMD; the L-lane groups one wave per SIMD.
 void set_default_grid(rbd_batch* b) {
   const rbd_context* ctx = b->ctx;
-  const int waves = (ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT ||
-                     ctx->precision == RBD_PREC_FP32_HW_TRIG) ? 2 : 1;
-  b->grid[0] = b->grid[1] = unsigned(ctx->compute_units * 4 * waves);
+  const bool fp32_columns = ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT ||
+                            ctx->precision == RBD_PREC_FP32_HW_TRIG;
+  b->grid[0] = unsigned(ctx->compute_units * (fp32_columns ? 6 : 4));
+  b->grid[1] = unsigned(ctx->compute_units * 4 * (fp32_columns ? 2 : 1));
   b->grid[2] = unsigned(ctx->compute_units * 2);
   b->grid[3] = b->grid[4] = b->grid[5] = unsigned(ctx->compute_units * 4);
 }
@@ -564,14 +567,16 @@ rbd_status rbd_fdgrad_batched_noalloc_f32(rbd_context* ctx, const rbd_traj* traj
 // Times every mapping of the batched path (kAutotuneLanes) on batches of
 // 132-knot trajectories (the mean length of lengths uniform over 8..256), 8 to
 // 512 of them, and fills the context's batch mapping table with the fastest
-// at each total; the count 69 (9,108 knots) sits just under one wave of
-// 7-lane groups per SIMD on 256 CUs.
+// at each total.  One more point is 64 trajectories of 142 knots (9,088
+// knots): just under one wave of 7-lane groups per SIMD on 256 CUs, in the
+// kernel-argument form of batches of up to 64 trajectories.
 rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                               int table_len) {
   if (rbd_status s = check_ctx(ctx)) return s;
   if (rbd_status s = ready_to_launch(ctx)) return s;
   constexpr int64_t kLen = 132;
-  static const int counts[] = {8, 16, 32, 48, 64, 69, 80, 96, 128, 256, 512};
+  static const int counts[] = {8, 16, 32, 48, 64, 64, 80, 96, 128, 256, 512};
+  static const int64_t lens[] = {kLen, kLen, kLen, kLen, kLen, 142, kLen, kLen, kLen, kLen, kLen};
   constexpr int kSizes = sizeof(counts) / sizeof(counts[0]);
   static_assert(kSizes <= kMaxTuned, "mapping table size");
   const int max_traj = counts[kSizes - 1];
@@ -584,15 +589,23 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
   e = hipMemsetAsync(buf, 0, knots * rbd::kInPerKnot * sizeof(float), st);
   rbd_batch* b = nullptr;
   rbd_status rs = e == hipSuccess ? rbd_batch_create(ctx, max_traj, &b) : RBD_OK;
+  // trajectory t of a batch of length len starts at knot t * len (len <= 142
+  // for t < 64, 132 above: every batch stays inside the buffer)
+  static_assert(64 * 142 <= 512 * kLen, "the longer trajectories fit the buffer");
   std::vector<rbd_traj> trajs(static_cast<size_t>(max_traj));
-  for (int t = 0; t < max_traj; ++t)
-    trajs[size_t(t)] = rbd_traj{buf + size_t(t) * kLen * rbd::kInPerKnot,
-                                buf + knots * rbd::kInPerKnot + size_t(t) * kLen * rbd::kOutPerKnot, kLen};
+  auto place = [&](int64_t len) {
+    for (int t = 0; t < max_traj; ++t) {
+      const int64_t n = t < 64 ? len : kLen;
+      const size_t k0 = t < 64 ? size_t(t) * size_t(len) : size_t(t) * size_t(kLen);
+      trajs[size_t(t)] = rbd_traj{buf + k0 * rbd::kInPerKnot, buf + knots * rbd::kInPerKnot + k0 * rbd::kOutPerKnot, n};
+    }
+  };
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (e == hipSuccess && rs == RBD_OK) e = hipEventCreate(&t0);
   if (e == hipSuccess && rs == RBD_OK) e = hipEventCreate(&t1);
   float best_us[kAutotuneMappings][kSizes] = {};
   for (int si = 0; si < kSizes && e == hipSuccess && rs == RBD_OK; ++si) {
+    place(lens[si]);
     for (int li = 0; li < kAutotuneMappings && e == hipSuccess && rs == RBD_OK; ++li) {
       const int lanes = kAutotuneLanes[li];
       float best = 1e30f;
@@ -626,9 +639,9 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
     int bi = 0;
     for (int li = 1; li < kAutotuneMappings; ++li)
       if (best_us[li][si] < best_us[bi][si]) bi = li;
-    ctx->batch_tuned.n[si] = counts[si] * kLen;
+    ctx->batch_tuned.n[si] = counts[si] * lens[si];
     ctx->batch_tuned.lanes[si] = kAutotuneLanes[bi];
-    if (best_us[1][si] < best_us[0][si]) threshold = counts[si] * kLen;
+    if (best_us[1][si] < best_us[0][si]) threshold = counts[si] * lens[si];
   }
   ctx->batch_tuned.count.store(kSizes, std::memory_order_release);
   ctx->batch_group_threshold.store(threshold);
@@ -636,7 +649,7 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
   constexpr int kRow = 1 + kAutotuneMappings;
   if (table_us)
     for (int i = 0; i < table_len && i < kRow * kSizes; ++i)
-      table_us[i] = (i % kRow == 0) ? float(counts[i / kRow] * kLen) : best_us[(i % kRow) - 1][i / kRow];
+      table_us[i] = (i % kRow == 0) ? float(counts[i / kRow] * lens[i / kRow]) : best_us[(i % kRow) - 1][i / kRow];
   return RBD_OK;
 }
 

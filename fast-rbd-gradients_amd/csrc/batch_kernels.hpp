@@ -34,6 +34,7 @@
 
 #include "fdgrad_knot.hpp"
 #include "group_mapping.hpp"
+#include "records.hpp"
 #include "two_wave.hpp"
 
 namespace rbd {
@@ -234,7 +235,10 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
   // zero-copy batches, one PCIe write per partial line)
   // (the chunk's inputs, 336 contiguous bytes, are read the same way into LDS
   // first: a few whole-line reads instead of many small per-lane ones)
-  __shared__ float rec_sh[LANES == 1 ? 1 : chunk * kOutPerKnot];
+  // one lane per knot: the chunk's records (64 consecutive knots of one
+  // trajectory, one contiguous block) are assembled in LDS and copied out
+  // whole (copy_record_block), as the flat one-lane kernel does
+  __shared__ __align__(16) float rec_sh[chunk * kOutPerKnot];
   __shared__ float in_sh[LANES == 1 ? 1 : chunk * kInPerKnot];
   auto process = [&](int c) {
     int t, first;                           // trajectory of chunk c, and its first chunk
@@ -243,11 +247,14 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
     const long k0 = long(c - first) * chunk;
     const long k = k0 + slot;
     if constexpr (LANES == 1) {
-      if (k < d.n) {
-        InAoS i{d.in + kInPerKnot * k};
-        OutAoS o{d.out + kOutPerKnot * k};
-        fdgrad_knot<P>(i, o, opt);
-      }
+      const int knots = int(d.n - k0 < chunk ? d.n - k0 : chunk);
+      // lanes past the trajectory's end compute a copy of its last knot and store nothing
+      InAoS i{d.in + kInPerKnot * (k < d.n ? k : d.n - 1)};
+      OutAoS o{rec_sh + kOutPerKnot * slot};
+      fdgrad_knot<P>(i, o, opt);
+      wave_lds_sync();
+      copy_record_block(rec_sh, d.out + kOutPerKnot * k0, knots * kOutPerKnot, lane);
+      wave_lds_sync();                      // read out before the next chunk overwrites it
     } else {
       const int knots = int(d.n - k0 < chunk ? d.n - k0 : chunk);
       for (int e = lane; e < knots * kInPerKnot; e += 64) in_sh[e] = d.in[kInPerKnot * k0 + e];

@@ -653,16 +653,13 @@ RBD_HD RBD_INLINE void make_phase_c(PhaseC<SC>& P, const CS<SC>* t, const SR* qd
   });
 }
 
-// All derivative columns in one lane (one lane per knot) and the stores of
-// dqdd/dq, dqdd/dqd.  MINV = false stores the inverse-dynamics gradient
-// dtau/dq, dtau/dqd itself (the reference's cdu) instead of -Minv times it.
+// The d/dq columns in one lane (one lane per knot) and their stores.  MINV =
+// false stores the inverse-dynamics gradient dtau/dq itself (the reference's
+// cdu) instead of -Minv times it.
 template <bool MINV = true, typename SC, class Out>
-RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out& out) {
+RBD_HD RBD_INLINE void gradient_dq_columns(const PhaseC<SC>& P, Out& out) {
   // d/dq column 0 is identically zero (rotation about the gravity axis)
   for (int r = 0; r < NQ; ++r) out.dq(r, 0, 0.0f);
-  // the d/dq columns, then the d/dqd columns: the order of the outputs in
-  // memory, so that each knot's AoS record fills front to back and few of its
-  // cache lines are partly written at a time
   static_for<1, NQ>([&](auto Cc) {
     constexpr int c = decltype(Cc)::value;
     SC tau[NQ], col[NQ];
@@ -671,6 +668,11 @@ RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out&
     else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
     for (int r = 0; r < NQ; ++r) out.dq(r, c, float(col[r]));
   });
+}
+
+// The d/dqd columns in one lane and their stores (damping on the diagonal).
+template <bool MINV = true, typename SC, class Out>
+RBD_HD RBD_INLINE void gradient_dqd_columns(const PhaseC<SC>& P, float damping, Out& out) {
   static_for<0, NQ>([&](auto Cc) {
     constexpr int c = decltype(Cc)::value;
     SC tau[NQ], col[NQ];
@@ -680,6 +682,16 @@ RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out&
     else for (int r = 0; r < NQ; ++r) col[r] = tau[r];
     for (int r = 0; r < NQ; ++r) out.dqd(r, c, float(col[r]));
   });
+}
+
+// All derivative columns in one lane and the stores of dqdd/dq, dqdd/dqd: the
+// d/dq columns, then the d/dqd columns -- the order of the outputs in memory,
+// so that each knot's AoS record fills front to back and few of its cache
+// lines are partly written at a time.
+template <bool MINV = true, typename SC, class Out>
+RBD_HD RBD_INLINE void gradient_columns(const PhaseC<SC>& P, float damping, Out& out) {
+  gradient_dq_columns<MINV>(P, out);
+  gradient_dqd_columns<MINV>(P, damping, out);
 }
 
 // Lane roles when one knot is spread over a group of kLanesPerKnotGroup lanes:

@@ -275,8 +275,8 @@ rbd_status rbd_set_two_wave_threshold(rbd_context* ctx, int64_t n_knots);
 
 /* The batched path (rbd_fdgrad_batched_f32) has its own mapping table on a
  * batch's total knots.  rbd_autotune_batch times the six mappings on batches
- * of 8, 16, 32, 48, 64, 69, 80, 96, 128, 256 and 512 trajectories of 132 knots
- * and installs the fastest at each total; it also sets the crossover (the
+ * of 8, 16, 32, 48, 64, 80, 96, 128, 256 and 512 trajectories of 132 knots and
+ * 64 of 142 (9,088 knots) and installs the fastest at each total; it also sets the crossover (the
  * largest total at which 16-lane groups beat one lane per knot), which decides
  * alone once set by hand.  table_us receives rows of 7 as rbd_autotune's, on
  * total knots.  Same conditions as rbd_autotune.  No reference counterpart. */
