@@ -83,13 +83,26 @@ def knots(n, seed, near_singular_frac=0.0):
     return np.concatenate([q, qd, u], axis=1).astype(np.float32)
 
 
+# Untimed work before every timed region: from idle, the GPU takes tens of
+# milliseconds to reach its steady clocks (a first 11 ms of 2^20-knot launches
+# measured 227 us per launch against 190-195 us after), and each workload here
+# starts after host work (uploads, downloads, oracle checks) that idles it.
+SETTLE_MS = 100.0
+
+
 def time_launches(launch, steps, warmup, stream):
-    """Warm up, then time exactly `steps` launches on `stream` with HIP events
+    """Warm up (`warmup` launches, then more until SETTLE_MS of wall time has
+    passed), then time exactly `steps` launches on `stream` with HIP events
     recorded on that stream, synchronising the device on both sides.  Returns
     total milliseconds."""
+    t_warm = time.perf_counter()
     for _ in range(warmup):
         launch()
     rbdgrad.hip().hipDeviceSynchronize()
+    while (time.perf_counter() - t_warm) * 1e3 < SETTLE_MS:
+        for _ in range(8):
+            launch()
+        rbdgrad.hip().hipDeviceSynchronize()
     t0, t1 = rbdgrad.Event(), rbdgrad.Event()
     t0.record(stream)
     for _ in range(steps):
@@ -324,6 +337,8 @@ def mixed_batch(ctx, stream, steps=500):
         out = np.concatenate([d_out.download((n, rbdgrad.OUT_PER_KNOT)) for _, d_out, n in bufs])
         r = res["graph" if use_graph else "direct"] = {
             "us_per_batch": ms * 1e3, "knots_per_s": n2 / (ms * 1e-3), "lanes_per_knot": batch.last_lanes,
+            "waves_per_group": next((w for n, l, w in ctx.mapping_table(batched=True)
+                                     if n2 <= n and l == batch.last_lanes), 1),
             "hbm_frac": gbs / HBM_PEAK_GBS, "fp32_frac": tfl / FP32_PEAK_TFLOPS}
         CHECKS.add(r, "accuracy", out, x2, sample=4096)
     batch.close()
@@ -331,7 +346,8 @@ def mixed_batch(ctx, stream, steps=500):
         d_in.free()
         d_out.free()
     log(f"[mixed] {n2} knots: per-call kernel {res['per_call_kernel']['us_per_batch']:.1f} us, batched "
-        f"{res['direct']['us_per_batch']:.1f} us (lanes {res['direct']['lanes_per_knot']}), graph "
+        f"{res['direct']['us_per_batch']:.1f} us (lanes {res['direct']['lanes_per_knot']}, "
+        f"{res['direct']['waves_per_group']} waves per workgroup), graph "
         f"{res['graph']['us_per_batch']:.1f} us")
     return res
 
@@ -579,12 +595,10 @@ def main():
     thr, tune = ctx.autotune(stream)
     bthr, btune = ctx.autotune_batch(stream)
     detail["autotune"] = {"group_threshold_knots": thr, "two_wave_threshold_knots": ctx.two_wave_threshold,
-                          "us_per_launch_by_lanes_per_knot": {str(k): {str(l): t for l, t in v.items()}
-                                                              for k, v in tune.items()},
+                          "us_per_launch_by_shape": {str(k): v for k, v in tune.items()},
                           "mapping_table": ctx.mapping_table(),
                           "batch_group_threshold_knots": bthr,
-                          "batch_us_by_lanes_per_knot": {str(k): {str(l): t for l, t in v.items()}
-                                                         for k, v in btune.items()},
+                          "batch_us_by_shape": {str(k): v for k, v in btune.items()},
                           "batch_mapping_table": ctx.mapping_table(batched=True)}
     log(f"[autotune] per call {ctx.mapping_table()}; per batch {ctx.mapping_table(batched=True)}")
     detail["per_call_n128"], vol1 = per_call(ctx, stream, args.call_steps, 100)
@@ -635,6 +649,7 @@ def main():
                                "one launch per step",
                    "knots_per_step": LARGE, "layout": h["layout"], "precision": h["precision"], "kernel": h["kernel"],
                    "mapping": "one lane per knot",
+                   "settle_ms": SETTLE_MS,
                    "headline_choice": "FP32 holds the bar on every knot" if key == "fp32_soa" else
                                       f"FP32 breaks the bar somewhere: fastest passing mode {key}"},
         "roofline": {"bound": "fp32-valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -655,6 +670,7 @@ def main():
                            "us_graph": round(mixed["graph"]["us_per_batch"], 2),
                            "us_per_call_kernel": round(mixed["per_call_kernel"]["us_per_batch"], 2),
                            "lanes_per_knot": mixed["direct"]["lanes_per_knot"],
+                           "waves_per_group": mixed["direct"]["waves_per_group"],
                            "knots_per_s": mixed["direct"]["knots_per_s"]},
         "c3_served": {"mode": "automatic (zero-copy batches up to 16,384 knots, copies above), two-wave batches, "
                               "4 slots, 256 MiB budget; p50/p99 us, knots/s",

@@ -32,11 +32,12 @@ struct rbd_batch {
   char* h_stage[kRing] = {};
   hipEvent_t staged[kRing] = {};
   int ring = 0;
-  // persistent grid per mapping, and the captured graphs per (mapping, small
-  // batch): index 2 * mapping + small
-  static constexpr int kMappings = 6;
+  // persistent grid per shape, and the captured graphs per (shape, small
+  // batch): index 2 * shape + small
+  static constexpr int kMappings = 12;
   // waves: [0] one lane per knot, [1] 16-lane groups, [2] two-wave workgroups,
-  // [3], [4], [5] groups of 4, 7 and 8 lanes
+  // [3 + 3 w + g] groups of 4, 7, 8 lanes (g = 0, 1, 2) in workgroups of 1, 2,
+  // 4 waves (w = 0, 1, 2)
   unsigned grid[kMappings] = {};
   hipGraphExec_t exec[2 * kMappings] = {};
   hipGraph_t graph[2 * kMappings] = {};
@@ -73,15 +74,17 @@ struct Queue {
 // the batch's own header and descriptors, uploaded per launch
 Queue own_queue(rbd_batch* b) { return Queue{b->d_hdr, b->d_desc, b->d_prefix}; }
 
-// mapping index (0 one lane, 1 16-lane groups, 2 two waves, 3/4/5 groups of
-// 4/7/8 lanes) and knots per chunk
-int mapping_of(int lanes) {
-  switch (lanes) {
+// shape index (0 one lane, 1 16-lane groups, 2 two waves, 3 + 3 w + g groups
+// of 4/7/8 lanes in workgroups of 1/2/4 waves) and knots per chunk
+int mapping_of(Shape sh) {
+  switch (sh.lanes) {
     case 1: return 0;
     case rbd::kLanesPerKnotGroup: return 1;
-    case 4: return 3;
-    case 7: return 4;
-    case 8: return 5;
+    case 4: case 7: case 8: {
+      const int g = sh.lanes == 4 ? 0 : sh.lanes == 7 ? 1 : 2;
+      const int w = sh.wpg == 4 ? 2 : sh.wpg == 2 ? 1 : 0;
+      return 3 + 3 * w + g;
+    }
     default: return 2;
   }
 }
@@ -104,21 +107,23 @@ int64_t chunk_total(int n_traj, Len len, int lanes) {
 }
 
 template <class P, int LANES, bool SMALL>
-void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
-  // a batch of fewer chunks than the persistent grid launches one workgroup
-  // per chunk (fewer to dispatch); the captured graphs of large batches keep
-  // the full grid, whose launch serves every batch (small batches' graphs set
-  // theirs per replay, replay_small)
-  unsigned grid = b->grid[mapping_of(LANES)];
+void launch_persistent(rbd_batch* b, const Queue& q, int wpg, hipStream_t s) {
+  // a batch of fewer chunks than the persistent grid launches one wave per
+  // chunk (fewer to dispatch); the captured graphs of large batches keep the
+  // full grid, whose launch serves every batch (small batches' graphs are
+  // captured with theirs, replay_small)
+  unsigned grid = b->grid[mapping_of(Shape{LANES, wpg})];
   if (q.chunks > 0 && q.chunks < int64_t(grid)) grid = unsigned(q.chunks);
   if constexpr (LANES == 4 || LANES == 7 || LANES == 8) {
+    // `grid` waves in workgroups of wpg
+    const dim3 groups((grid + unsigned(wpg) - 1) / unsigned(wpg)), block(kBlock * wpg);
     static const rbd::ArgBatch kNone{};
     const rbd::ArgBatch& a = q.args ? *q.args : kNone;
     if constexpr (SMALL)
-      hipLaunchKernelGGL((rbd::batch_group_kernel<P, LANES, true>), dim3(grid), dim3(kBlock), 0, s,
+      hipLaunchKernelGGL((rbd::batch_group_kernel<P, LANES, true>), groups, block, 0, s,
                          q.args ? nullptr : q.descs, q.hdr, a, b->d_tail, q.prefix, b->ctx->opt, q.done);
     else
-      hipLaunchKernelGGL((rbd::batch_group_kernel<P, LANES, false>), dim3(grid), dim3(kBlock), 0, s, q.descs,
+      hipLaunchKernelGGL((rbd::batch_group_kernel<P, LANES, false>), groups, block, 0, s, q.descs,
                          q.hdr, rbd::NoArgBatch{}, b->d_tail, q.prefix, b->ctx->opt, q.done);
   } else if constexpr (LANES == kTwoWaveLanes) {
     static const rbd::ArgBatch kNone{};
@@ -144,7 +149,8 @@ void launch_persistent(rbd_batch* b, const Queue& q, hipStream_t s) {
 // per knot 6 (the 26,880-byte record blocks fill the LDS) with FP32 columns,
 // 4 (one per SIMD) with FP64 columns; with 16-lane groups two per SIMD with
 // FP32 columns, one with FP64 columns; two-wave workgroups at 256 registers,
-// one wave per SIMD; the L-lane groups one wave per SIMD.
+// one wave per SIMD; the L-lane groups one wave per SIMD, whatever their
+// workgroup size.
 void set_default_grid(rbd_batch* b) {
   const rbd_context* ctx = b->ctx;
   const bool fp32_columns = ctx->precision == RBD_PREC_FP32 || ctx->precision == RBD_PREC_FP64_PIVOT ||
@@ -152,11 +158,11 @@ void set_default_grid(rbd_batch* b) {
   b->grid[0] = unsigned(ctx->compute_units * (fp32_columns ? 6 : 4));
   b->grid[1] = unsigned(ctx->compute_units * 4 * (fp32_columns ? 2 : 1));
   b->grid[2] = unsigned(ctx->compute_units * 2);
-  b->grid[3] = b->grid[4] = b->grid[5] = unsigned(ctx->compute_units * 4);
+  for (int m = 3; m < rbd_batch::kMappings; ++m) b->grid[m] = unsigned(ctx->compute_units * 4);
 }
 
 // Forward declaration: the direct launch, which the small-batch graphs capture.
-hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hipStream_t s);
+hipError_t launch_batch(rbd_batch* b, const Queue& q, Shape sh, bool small, hipStream_t s);
 
 // A small batch replayed from its mapping's graph.  The graph holds the
 // batch's one kernel with the trajectory list in its arguments and its grid
@@ -167,8 +173,8 @@ hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hip
 // about 34 KB of device memory until the graph is destroyed and costs about
 // 8 us of host time.
 hipError_t replay_small(rbd_batch* b, const rbd_traj* trajs, int n_traj, const rbd::ArgBatch& a, int64_t chunks,
-                        int lanes, hipStream_t st) {
-  const int mi = mapping_of(lanes);
+                        Shape sh, hipStream_t st) {
+  const int mi = mapping_of(sh);
   std::vector<rbd_traj>& key = b->arg_key[mi];
   const bool same = b->arg_exec[mi] && int(key.size()) == n_traj &&
                     std::memcmp(key.data(), trajs, sizeof(rbd_traj) * size_t(n_traj)) == 0;
@@ -195,7 +201,7 @@ hipError_t replay_small(rbd_batch* b, const rbd_traj* trajs, int n_traj, const r
       Queue q{nullptr, nullptr, b->d_prefix};
       q.args = &a;
       q.chunks = chunks;
-      const hipError_t le = launch_batch(b, q, lanes, true, cs);
+      const hipError_t le = launch_batch(b, q, sh, true, cs);
       e = hipStreamEndCapture(cs, &b->arg_graph[mi]);
       if (le != hipSuccess) e = le;
     }
@@ -215,34 +221,36 @@ hipError_t replay_small(rbd_batch* b, const rbd_traj* trajs, int n_traj, const r
 }
 
 template <class P>
-hipError_t launch_batch_p(rbd_batch* b, const Queue& q, int lanes, bool small, hipStream_t s) {
+hipError_t launch_batch_p(rbd_batch* b, const Queue& q, Shape sh, bool small, hipStream_t s) {
   if (!small)
     hipLaunchKernelGGL(rbd::batch_plan_kernel, dim3(1), dim3(rbd::kPlanThreads), 0, s, q.descs, q.hdr, q.prefix);
+  const int lanes = sh.lanes;
+  const int wpg = is_group_lanes(lanes) && valid_wpg(sh.wpg) ? sh.wpg : 1;
   if (lanes == 1) {
-    if (small) launch_persistent<P, 1, true>(b, q, s); else launch_persistent<P, 1, false>(b, q, s);
+    if (small) launch_persistent<P, 1, true>(b, q, 1, s); else launch_persistent<P, 1, false>(b, q, 1, s);
   } else if (lanes == kTwoWaveLanes) {
-    if (small) launch_persistent<P, kTwoWaveLanes, true>(b, q, s);
-    else launch_persistent<P, kTwoWaveLanes, false>(b, q, s);
+    if (small) launch_persistent<P, kTwoWaveLanes, true>(b, q, 1, s);
+    else launch_persistent<P, kTwoWaveLanes, false>(b, q, 1, s);
   } else if (lanes == 4) {
-    if (small) launch_persistent<P, 4, true>(b, q, s); else launch_persistent<P, 4, false>(b, q, s);
+    if (small) launch_persistent<P, 4, true>(b, q, wpg, s); else launch_persistent<P, 4, false>(b, q, wpg, s);
   } else if (lanes == 7) {
-    if (small) launch_persistent<P, 7, true>(b, q, s); else launch_persistent<P, 7, false>(b, q, s);
+    if (small) launch_persistent<P, 7, true>(b, q, wpg, s); else launch_persistent<P, 7, false>(b, q, wpg, s);
   } else if (lanes == 8) {
-    if (small) launch_persistent<P, 8, true>(b, q, s); else launch_persistent<P, 8, false>(b, q, s);
+    if (small) launch_persistent<P, 8, true>(b, q, wpg, s); else launch_persistent<P, 8, false>(b, q, wpg, s);
   } else {
-    if (small) launch_persistent<P, rbd::kLanesPerKnotGroup, true>(b, q, s);
-    else launch_persistent<P, rbd::kLanesPerKnotGroup, false>(b, q, s);
+    if (small) launch_persistent<P, rbd::kLanesPerKnotGroup, true>(b, q, 1, s);
+    else launch_persistent<P, rbd::kLanesPerKnotGroup, false>(b, q, 1, s);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_batch(rbd_batch* b, const Queue& q, int lanes, bool small, hipStream_t s) {
+hipError_t launch_batch(rbd_batch* b, const Queue& q, Shape sh, bool small, hipStream_t s) {
   switch (b->ctx->precision) {
-    case RBD_PREC_FP32: return launch_batch_p<rbd::PrecFP32>(b, q, lanes, small, s);
-    case RBD_PREC_FP32_HW_TRIG: return launch_batch_p<rbd::PrecFP32HwTrig>(b, q, lanes, small, s);
-    case RBD_PREC_FP64_PIVOT: return launch_batch_p<rbd::PrecFP64Pivot>(b, q, lanes, small, s);
-    case RBD_PREC_FP64_COLUMNS: return launch_batch_p<rbd::PrecFP64Col>(b, q, lanes, small, s);
-    default: return launch_batch_p<rbd::PrecFP64>(b, q, lanes, small, s);
+    case RBD_PREC_FP32: return launch_batch_p<rbd::PrecFP32>(b, q, sh, small, s);
+    case RBD_PREC_FP32_HW_TRIG: return launch_batch_p<rbd::PrecFP32HwTrig>(b, q, sh, small, s);
+    case RBD_PREC_FP64_PIVOT: return launch_batch_p<rbd::PrecFP64Pivot>(b, q, sh, small, s);
+    case RBD_PREC_FP64_COLUMNS: return launch_batch_p<rbd::PrecFP64Col>(b, q, sh, small, s);
+    default: return launch_batch_p<rbd::PrecFP64>(b, q, sh, small, s);
   }
 }
 
@@ -262,18 +270,18 @@ long stage(char* dst, const rbd_traj* trajs, int n_traj, int lanes) {
   return total;
 }
 
-int graph_index(int lanes, bool small) { return 2 * mapping_of(lanes) + (small ? 1 : 0); }
+int graph_index(Shape sh, bool small) { return 2 * mapping_of(sh) + (small ? 1 : 0); }
 
-// capture the batch's kernels for one (mapping, small) pair into a graph (once)
-rbd_status ensure_graph(rbd_batch* b, int lanes, bool small) {
-  const int gi = graph_index(lanes, small);
+// capture the batch's kernels for one (shape, small) pair into a graph (once)
+rbd_status ensure_graph(rbd_batch* b, Shape sh, bool small) {
+  const int gi = graph_index(sh, small);
   if (b->exec[gi]) return RBD_OK;
   hipStream_t cs = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "hipStreamCreate (capture)");
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t le = launch_batch(b, own_queue(b), lanes, small, cs);
+    hipError_t le = launch_batch(b, own_queue(b), sh, small, cs);
     e = hipStreamEndCapture(cs, &b->graph[gi]);
     if (le != hipSuccess) e = le;
   }
@@ -404,11 +412,14 @@ rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves) 
   if (lanes_per_knot != 1 && lanes_per_knot != rbd::kLanesPerKnotGroup && !is_group_lanes(lanes_per_knot))
     return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 1, 4, 7, 8 or 16");
   if (waves == 0 || waves > (1u << 20)) return fail(RBD_ERR_INVALID_ARGUMENT, "waves must be in [1, 2^20]");
-  const int mi = mapping_of(lanes_per_knot);
-  if (b->grid[mi] != waves) {
+  // every workgroup size of the mapping
+  for (int wpg : {1, 2, 4}) {
+    const Shape sh{lanes_per_knot, wpg};
+    const int mi = mapping_of(sh);
+    if (b->grid[mi] == waves) continue;
     b->grid[mi] = waves;
     for (int small = 0; small < 2; ++small) {
-      const int gi = graph_index(lanes_per_knot, small != 0);
+      const int gi = graph_index(sh, small != 0);
       if (b->exec[gi]) { (void)hipGraphExecDestroy(b->exec[gi]); b->exec[gi] = nullptr; }
       if (b->graph[gi]) { (void)hipGraphDestroy(b->graph[gi]); b->graph[gi] = nullptr; }
     }
@@ -418,14 +429,14 @@ rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves) 
 
 rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph,
                                   rbd_stream_t stream) {
-  return rbdi::batch_run(b, trajs, n_traj, use_graph, stream, 0);
+  return rbdi::batch_run(b, trajs, n_traj, use_graph, stream, Shape{});
 }
 
 }  // extern "C"
 
-// lanes = 0: by the context (lanes_for_batch)
+// shape_req.lanes = 0: by the context (shape_for_batch)
 rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int use_graph, rbd_stream_t stream,
-                           int lanes_req) {
+                           Shape shape_req) {
   if (!b) return fail(RBD_ERR_INVALID_ARGUMENT, "null batch");
   if (n_traj < 0) return fail(RBD_ERR_INVALID_ARGUMENT, "negative trajectory count");
   if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
@@ -443,7 +454,8 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   if (total == 0) return RBD_OK;
   if (rbd_status s = ready_to_launch(b->ctx)) return s;
   const hipStream_t st = (hipStream_t)stream;
-  const int lanes = lanes_req ? lanes_req : lanes_for_batch(b->ctx, total);
+  const Shape sh = shape_req.lanes ? shape_req : shape_for_batch(b->ctx, total);
+  const int lanes = sh.lanes;
   b->last_lanes = lanes;
   const bool small = n_traj <= rbd::kSmallBatch;
   if (small) {
@@ -454,14 +466,14 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
     a.n_traj = n_traj;
     a.total_chunks = int(chunks);
     if (use_graph) {
-      hipError_t ge = replay_small(b, trajs, n_traj, a, chunks, lanes, st);
+      hipError_t ge = replay_small(b, trajs, n_traj, a, chunks, sh, st);
       if (ge != hipSuccess) return hip_fail(ge, "batch graph (small batch)");
       return RBD_OK;
     }
     Queue q{nullptr, nullptr, b->d_prefix};
     q.args = &a;
     q.chunks = chunks;
-    hipError_t le = launch_batch(b, q, lanes, true, st);
+    hipError_t le = launch_batch(b, q, sh, true, st);
     if (le != hipSuccess) return hip_fail(le, "batch kernel launch");
     return RBD_OK;
   }
@@ -475,13 +487,13 @@ rbd_status rbdi::batch_run(rbd_batch* b, const rbd_traj* trajs, int n_traj, int 
   if (e == hipSuccess) e = hipEventRecord(b->staged[r], st);
   if (e != hipSuccess) return hip_fail(e, "descriptor upload");
   if (use_graph) {
-    if (rbd_status s = ensure_graph(b, lanes, small)) return s;
-    e = hipGraphLaunch(b->exec[graph_index(lanes, small)], st);
+    if (rbd_status s = ensure_graph(b, sh, small)) return s;
+    e = hipGraphLaunch(b->exec[graph_index(sh, small)], st);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
   } else {
     Queue q = own_queue(b);
     q.chunks = chunks;
-    e = launch_batch(b, q, lanes, small, st);
+    e = launch_batch(b, q, sh, small, st);
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   }
   return RBD_OK;
@@ -494,12 +506,12 @@ int64_t rbdi::batch_stage(void* host_dst, const rbd_traj* trajs, int n_traj, int
 }
 
 rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, const rbd_traj* trajs, int n_traj,
-                                     int64_t total_knots, int64_t total_chunks, int lanes, rbd_stream_t stream,
+                                     int64_t total_knots, int64_t total_chunks, Shape sh, rbd_stream_t stream,
                                      unsigned long long* done_flag, unsigned long long done_seq) {
   if (n_traj > b->max_traj) return fail(RBD_ERR_CAPACITY, "more trajectories than the batch was created for");
   if (total_chunks > kMaxBatchChunks) return fail(RBD_ERR_CAPACITY, "batch too large (more than 2^31 chunks)");
   b->last_total_knots = total_knots;
-  b->last_lanes = lanes;
+  b->last_lanes = sh.lanes;
   if (total_knots == 0) return RBD_OK;
   const char* base = static_cast<const char*>(d_staged);
   Queue q{reinterpret_cast<const rbd::BatchHeader*>(base),
@@ -515,7 +527,7 @@ rbd_status rbdi::batch_launch_staged(rbd_batch* b, const void* d_staged, const r
     a.total_chunks = int(total_chunks);
     q.args = &a;
   }
-  hipError_t e = launch_batch(b, q, lanes, small, (hipStream_t)stream);
+  hipError_t e = launch_batch(b, q, sh, small, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
   return RBD_OK;
 }
@@ -546,7 +558,8 @@ rbd_status rbd_fdgrad_batched_noalloc_f32(rbd_context* ctx, const rbd_traj* traj
     if (ctx->device >= 0 && ctx->device < kMaxDevices) tail = g_scratch_ptr[ctx->device];
   }
   if (!tail) return fail(RBD_ERR_UNSUPPORTED, "no-allocation scratch not set up on this device");
-  const int lanes = lanes_for_batch(ctx, total);
+  const Shape sh = shape_for_batch(ctx, total);
+  const int lanes = sh.lanes;
   // the batch in the kernel arguments, the counters in the device's scratch,
   // the grid as a batch object's: nothing is allocated
   rbd_batch b;
@@ -560,11 +573,11 @@ rbd_status rbd_fdgrad_batched_noalloc_f32(rbd_context* ctx, const rbd_traj* traj
   Queue q{nullptr, nullptr, nullptr};
   q.args = &a;
   q.chunks = a.total_chunks;
-  const hipError_t e = launch_batch(&b, q, lanes, true, (hipStream_t)stream);
+  const hipError_t e = launch_batch(&b, q, sh, true, (hipStream_t)stream);
   return e == hipSuccess ? RBD_OK : hip_fail(e, "batch kernel launch (no allocation)");
 }
 
-// Times every mapping of the batched path (kAutotuneLanes) on batches of
+// Times every shape of the batched path (kAutotuneShapes) on batches of
 // 132-knot trajectories (the mean length of lengths uniform over 8..256), 8 to
 // 512 of them, and fills the context's batch mapping table with the fastest
 // at each total.  One more point is 64 trajectories of 142 knots (9,088
@@ -607,13 +620,13 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
   for (int si = 0; si < kSizes && e == hipSuccess && rs == RBD_OK; ++si) {
     place(lens[si]);
     for (int li = 0; li < kAutotuneMappings && e == hipSuccess && rs == RBD_OK; ++li) {
-      const int lanes = kAutotuneLanes[li];
+      const Shape sh = kAutotuneShapes[li];
       float best = 1e30f;
       for (int rep = 0; rep < 3 && e == hipSuccess && rs == RBD_OK; ++rep) {
-        rs = rbdi::batch_run(b, trajs.data(), counts[si], 0, stream, lanes);   // warm
+        rs = rbdi::batch_run(b, trajs.data(), counts[si], 0, stream, sh);   // warm
         if (rs == RBD_OK) e = hipEventRecord(t0, st);
         for (int it = 0; it < 4 && e == hipSuccess && rs == RBD_OK; ++it)
-          rs = rbdi::batch_run(b, trajs.data(), counts[si], 0, stream, lanes);
+          rs = rbdi::batch_run(b, trajs.data(), counts[si], 0, stream, sh);
         if (e == hipSuccess && rs == RBD_OK) e = hipEventRecord(t1, st);
         if (e == hipSuccess && rs == RBD_OK) e = hipEventSynchronize(t1);
         float ms = 0.0f;
@@ -640,7 +653,8 @@ rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* th
     for (int li = 1; li < kAutotuneMappings; ++li)
       if (best_us[li][si] < best_us[bi][si]) bi = li;
     ctx->batch_tuned.n[si] = counts[si] * lens[si];
-    ctx->batch_tuned.lanes[si] = kAutotuneLanes[bi];
+    ctx->batch_tuned.lanes[si] = kAutotuneShapes[bi].lanes;
+    ctx->batch_tuned.wpg[si] = kAutotuneShapes[bi].wpg;
     if (best_us[1][si] < best_us[0][si]) threshold = counts[si] * lens[si];
   }
   ctx->batch_tuned.count.store(kSizes, std::memory_order_release);

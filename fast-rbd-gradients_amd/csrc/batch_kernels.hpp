@@ -159,15 +159,14 @@ __device__ inline void publish_completion(const Completion& done, unsigned long 
   }
 }
 
-// The calling wave's share of a launch's `total` chunks, handed to process(c)
-// one at a time: the full rounds (all chunks when they fit one round) in a
-// fixed order with no atomics, then the last partial round through the tail
-// counters (tail: the kTailCounters counters kTailStride words apart, zero on
-// entry and left at zero).  One wave per workgroup.
+// The share of wave w (of the launch's G waves) of a launch's `total`
+// chunks, handed to process(c) one at a time: the full rounds (all chunks when
+// they fit one round) in a fixed order with no atomics, then the last partial
+// round through the tail counters (tail: the kTailCounters counters
+// kTailStride words apart, zero on entry and left at zero).  w is the same in
+// every lane of the wave.
 template <class F>
-__device__ inline void drain_chunks(int total, unsigned long long* tail, int lane, F&& process) {
-  const int G = int(gridDim.x);
-  const int w = int(blockIdx.x);
+__device__ inline void drain_chunks(int w, int G, int total, unsigned long long* tail, int lane, F&& process) {
   const int fixed = total <= G ? total : total - total % G;
   const int rest = total - fixed;
   const int S = G < kTailCounters ? G : kTailCounters;
@@ -270,7 +269,7 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
     }
   };
 
-  drain_chunks(total, tail, lane, process);
+  drain_chunks(int(blockIdx.x), int(gridDim.x), total, tail, lane, process);
   publish_completion(done, tail, unsigned(gridDim.x), lane);
 }
 
@@ -282,10 +281,12 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
 // finds its own knot's trajectory: SMALL batches from a shuffle scan of the
 // (up to 64) lengths, searched per lane; larger ones by a binary search of the
 // plan kernel's knot prefix sum (header chunk_knots = 1) for the chunk's first
-// knot, from which each lane walks on.  Queue, descriptors in the arguments
-// and completion word as in batch_persistent_kernel.
+// knot, from which each lane walks on.  Workgroups of 1, 2 or 4 waves
+// (blockDim.x = 64 x waves), each wave on its own: its own chunks and LDS
+// block.  Queue, descriptors in the arguments and completion word as in
+// batch_persistent_kernel.
 template <class P, int L, bool SMALL>
-__global__ __launch_bounds__(64, 1) void batch_group_kernel(
+__global__ __launch_bounds__(kMaxGroupWaves * 64, 1) void batch_group_kernel(
     const TrajDesc* __restrict__ descs, const BatchHeader* __restrict__ hdr,
     const typename std::conditional<SMALL, ArgBatch, NoArgBatch>::type arg, unsigned long long* __restrict__ tail,
     const long* __restrict__ prefix, Options opt, Completion done) {
@@ -304,9 +305,12 @@ __global__ __launch_bounds__(64, 1) void batch_group_kernel(
     n_traj = hdr->n_traj;
     total = int(hdr->total_chunks);
   }
-  const int lane = int(threadIdx.x);
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);   // in the workgroup
+  const int lane = int(threadIdx.x) % 64;
   const int slot = group_slot<L>(lane);
-  __shared__ typename P::Minv mi_sh[Gr::kMinvColumns ? Gr::kSlots : 1][kMinvPacked];   // Minv per knot (L >= 7)
+  const int waves_per_group = int(blockDim.x) / 64;
+  // Minv per knot (L >= 7), per wave of the workgroup
+  __shared__ typename P::Minv mi_sh[kMaxGroupWaves][Gr::kMinvColumns ? Gr::kSlots : 1][kMinvPacked];
   // SMALL: lane l holds the knots of trajectories 0..l (inclusive prefix sum)
   long incl = 0;
   if constexpr (SMALL) {
@@ -348,11 +352,12 @@ __global__ __launch_bounds__(64, 1) void batch_group_kernel(
     const long k = kk - first;
     InAoS i{d.in + kInPerKnot * k};
     OutMasked<OutAoS> o{OutAoS{d.out + kOutPerKnot * k}, valid};
-    fdgrad_knot_group<P, L>(i, o, opt, lane, mi_sh[Gr::kMinvColumns ? slot : 0]);
+    fdgrad_knot_group<P, L>(i, o, opt, lane, mi_sh[wave][Gr::kMinvColumns ? slot : 0]);
     wave_lds_sync();                            // every lane is done with mi_sh before the next chunk
   };
-  drain_chunks(total, tail, lane, process);
-  publish_completion(done, tail, unsigned(gridDim.x), lane);
+  const int G = int(gridDim.x) * waves_per_group;
+  drain_chunks(int(blockIdx.x) * waves_per_group + wave, G, total, tail, lane, process);
+  publish_completion(done, tail, unsigned(G), lane);
 }
 
 // Batches on the two-wave mapping: each workgroup of two waves takes

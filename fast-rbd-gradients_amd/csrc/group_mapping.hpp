@@ -29,6 +29,10 @@
 
 namespace rbd {
 
+// The L-lane group kernels run in workgroups of 1, 2 or 4 independent waves
+// (a launch parameter); their LDS is sized for this many.
+constexpr int kMaxGroupWaves = 4;
+
 template <int L>
 struct Group {
   static_assert(L >= 2 && L <= 16, "lanes per knot");

@@ -183,18 +183,24 @@ __global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, 
 }
 
 // L lanes per knot (L = 4, 7, 8; group_mapping.hpp): floor(64 / L) knots per
-// wave, one wave per workgroup, the waves striding over the call's knots.
-// Lanes of knots past the end compute a copy of the last knot and store nothing.
+// wave, the waves striding over the call's knots.  Workgroups of 1, 2 or 4
+// waves (blockDim.x = 64 x waves), each wave on its own: its own knots and its
+// own LDS block.  Lanes of knots past the end compute a copy of the last knot
+// and store nothing.
 template <class P, int L, bool SOA>
-__global__ __launch_bounds__(kBlock, 1) void fdgrad_group_kernel(long n, const float* __restrict__ in, long ld_in,
-                                                                float* __restrict__ out, long ld_out, Options opt) {
+__global__ __launch_bounds__(kMaxGroupWaves * kBlock, 1) void fdgrad_group_kernel(long n, const float* __restrict__ in,
+                                                                                 long ld_in, float* __restrict__ out,
+                                                                                 long ld_out, Options opt) {
   using G = Group<L>;
-  __shared__ typename P::Minv mi_sh[G::kMinvColumns ? G::kSlots : 1][kMinvPacked];   // Minv per knot (L >= 7)
-  const int lane = int(threadIdx.x);
+  // Minv per knot (L >= 7), per wave of the workgroup
+  __shared__ typename P::Minv mi_sh[kMaxGroupWaves][G::kMinvColumns ? G::kSlots : 1][kMinvPacked];
+  const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kBlock);   // the same in every lane
+  const int lane = int(threadIdx.x) % kBlock;
   const int slot = group_slot<L>(lane);
+  const long waves_per_group = long(blockDim.x) / kBlock;
   using In = typename std::conditional<SOA, InSoA, InAoS>::type;
   using O = typename std::conditional<SOA, OutSoA, OutAoS>::type;
-  for (long w = blockIdx.x; w * G::kSlots < n; w += gridDim.x) {
+  for (long w = long(blockIdx.x) * waves_per_group + wave; w * G::kSlots < n; w += long(gridDim.x) * waves_per_group) {
     const long k = w * G::kSlots + slot;
     const long kk = k < n ? k : n - 1;
     In i;
@@ -202,7 +208,7 @@ __global__ __launch_bounds__(kBlock, 1) void fdgrad_group_kernel(long n, const f
     if constexpr (SOA) { i = InSoA{in + kk, ld_in}; o0 = OutSoA{out + kk, ld_out}; }
     else { i = InAoS{in + kInPerKnot * kk}; o0 = OutAoS{out + kOutPerKnot * kk}; }
     OutMasked<O> o{o0, k < n && lane < G::kLanesUsed};
-    fdgrad_knot_group<P, L>(i, o, opt, lane, mi_sh[G::kMinvColumns ? slot : 0]);
+    fdgrad_knot_group<P, L>(i, o, opt, lane, mi_sh[wave][G::kMinvColumns ? slot : 0]);
     wave_lds_sync();                        // every lane is done with mi_sh before the next knots
   }
 }
@@ -309,9 +315,10 @@ __global__ __launch_bounds__(kBlock) void finish_kernel(long n, const float* __r
 
 namespace rbdi {
 
-// grid for n knots at `lanes` lanes per knot, capped so huge batches grid-stride
-unsigned grid_for(int64_t n, int lanes) {
-  const int64_t knots_per_block = lanes == 1 ? kBlock : kBlock / lanes;   // 9 for 7 lanes per knot
+// grid for n knots at `lanes` lanes per knot (and, for the L-lane groups, wpg
+// waves per workgroup), capped so huge calls grid-stride
+unsigned grid_for(int64_t n, int lanes, int wpg = 1) {
+  const int64_t knots_per_block = lanes == 1 ? kBlock : (kBlock / lanes) * (is_group_lanes(lanes) ? wpg : 1);
   const int64_t blocks = (n + knots_per_block - 1) / knots_per_block;
   const int64_t cap = int64_t(1) << 20;
   return unsigned(blocks < cap ? blocks : cap);
@@ -352,15 +359,17 @@ hipError_t launch_flat_p(const FlatCall& c) {
         break;
     }
   } else if (is_group_lanes(c.lanes) && c.layout != Layout::Ref) {
-    // L lanes per knot: one wave per workgroup, floor(64 / L) knots per wave
+    // L lanes per knot: floor(64 / L) knots per wave, c.wpg waves per workgroup
+    const int wpg = valid_wpg(c.wpg) ? c.wpg : 1;
+    const dim3 ggrid(grid_for(c.n, c.lanes, wpg)), gblock(kBlock * wpg);
     auto go = [&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
       if (c.layout == Layout::AoS)
-        hipLaunchKernelGGL((rbd::fdgrad_group_kernel<P, L, false>), grid, block, 0, c.stream, n, c.in, 0L, c.out, 0L,
-                           c.opt);
+        hipLaunchKernelGGL((rbd::fdgrad_group_kernel<P, L, false>), ggrid, gblock, 0, c.stream, n, c.in, 0L, c.out,
+                           0L, c.opt);
       else
-        hipLaunchKernelGGL((rbd::fdgrad_group_kernel<P, L, true>), grid, block, 0, c.stream, n, c.in, long(c.ld_in),
-                           c.out, long(c.ld_out), c.opt);
+        hipLaunchKernelGGL((rbd::fdgrad_group_kernel<P, L, true>), ggrid, gblock, 0, c.stream, n, c.in,
+                           long(c.ld_in), c.out, long(c.ld_out), c.opt);
     };
     if (c.lanes == 4) go(rbd::IC<4>{});
     else if (c.lanes == 7) go(rbd::IC<7>{});
@@ -405,7 +414,7 @@ using namespace rbdi;
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int rbd_abi_version(void) { return 2; }
+int rbd_abi_version(void) { return 3; }
 
 const char* rbd_model_name(void) { return rbd::kModelName; }
 
@@ -447,6 +456,8 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   if (c.lanes_per_knot != 0 && c.lanes_per_knot != 1 && c.lanes_per_knot != rbd::kLanesPerKnotGroup &&
       c.lanes_per_knot != kTwoWaveLanes && !is_group_lanes(c.lanes_per_knot))
     return fail(RBD_ERR_INVALID_ARGUMENT, "lanes_per_knot must be 0, 1, 4, 7, 8, 16 or 32");
+  if (c.waves_per_group != 0 && !valid_wpg(c.waves_per_group))
+    return fail(RBD_ERR_INVALID_ARGUMENT, "waves_per_group must be 0, 1, 2 or 4");
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev <= 0) return fail(RBD_ERR_NO_DEVICE, "no HIP device visible");
@@ -460,6 +471,7 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
   ctx->flags = c.flags;
   ctx->precision = c.precision;
   ctx->lanes_per_knot = c.lanes_per_knot;
+  ctx->waves_per_group = c.waves_per_group;
   ctx->two_wave_threshold.store(int64_t(16) * ctx->compute_units);
   set_default_tables(ctx);
   ctx->opt.damping = (c.flags & RBD_FLAG_VEL_DAMPING) ? 0.5f : 0.0f;
@@ -526,8 +538,9 @@ rbd_status rbd_fdgrad_traj_f32(rbd_context* ctx, int64_t n, const float* d_in, f
   if (n == 0) return RBD_OK;
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (rbd_status s = ready_to_launch(ctx)) return s;
-  const FlatCall c{Layout::AoS, lanes_for(ctx, n, true), n, d_in, 0, d_out, 0, ctx->opt,
-                   (hipStream_t)stream, 4 * ctx->compute_units};
+  const Shape sh = shape_for(ctx, n, true);
+  FlatCall c{Layout::AoS, sh.lanes, n, d_in, 0, d_out, 0, ctx->opt, (hipStream_t)stream, 4 * ctx->compute_units};
+  c.wpg = sh.wpg;
   hipError_t e = launch_flat(ctx->precision, c);
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
 }
@@ -540,8 +553,10 @@ rbd_status rbd_fdgrad_traj_soa_f32(rbd_context* ctx, int64_t n, const float* d_i
   if (!d_in || !d_out) return fail(RBD_ERR_INVALID_ARGUMENT, "null buffer");
   if (ld_in < n || ld_out < n) return fail(RBD_ERR_INVALID_ARGUMENT, "leading dimension smaller than knot count");
   if (rbd_status s = ready_to_launch(ctx)) return s;
-  const FlatCall c{Layout::SoA, lanes_for(ctx, n, true), n, d_in, ld_in, d_out, ld_out, ctx->opt,
-                   (hipStream_t)stream, 4 * ctx->compute_units};
+  const Shape sh = shape_for(ctx, n, true);
+  FlatCall c{Layout::SoA, sh.lanes, n, d_in, ld_in, d_out, ld_out, ctx->opt, (hipStream_t)stream,
+             4 * ctx->compute_units};
+  c.wpg = sh.wpg;
   hipError_t e = launch_flat(ctx->precision, c);
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad kernel launch");
 }
@@ -581,8 +596,9 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
   float* d_out = ctx->d_scratch + size_t(n) * rbd::kInPerKnot;
   hipError_t e = hipMemcpy(d_in, h_in, in_bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
-  const FlatCall c{Layout::AoS, lanes_for(ctx, n, true), n, d_in, 0, d_out, 0, ctx->opt,
-                   (hipStream_t)0, 4 * ctx->compute_units};
+  const Shape sh = shape_for(ctx, n, true);
+  FlatCall c{Layout::AoS, sh.lanes, n, d_in, 0, d_out, 0, ctx->opt, (hipStream_t)0, 4 * ctx->compute_units};
+  c.wpg = sh.wpg;
   e = launch_flat(ctx->precision, c);
   if (e != hipSuccess) return hip_fail(e, "fdgrad kernel launch");
   e = hipMemcpy(h_out, d_out, out_bytes, hipMemcpyDeviceToHost);
@@ -839,13 +855,15 @@ rbd_status rbd_set_group_threshold(rbd_context* ctx, int64_t n_knots) {
   return RBD_OK;
 }
 
-int rbd_mapping_table(rbd_context* ctx, int batched, int64_t* n_knots, int* lanes, int capacity) {
+int rbd_mapping_table(rbd_context* ctx, int batched, int64_t* n_knots, int* lanes, int* waves_per_group,
+                      int capacity) {
   if (!ctx) return -1;
   const MappingTable& t = batched ? ctx->batch_tuned : ctx->tuned;
   const int c = t.count.load(std::memory_order_acquire);
   for (int i = 0; i < c && i < capacity; ++i) {
     if (n_knots) n_knots[i] = t.n[i];
     if (lanes) lanes[i] = t.lanes[i];
+    if (waves_per_group) waves_per_group[i] = t.wpg[i] ? t.wpg[i] : 1;
   }
   return c;
 }
@@ -875,8 +893,9 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
   float best_us[kAutotuneMappings][kAutotuneSizes] = {};
   for (int si = 0; si < kAutotuneSizes && e == hipSuccess; ++si) {
     for (int li = 0; li < kAutotuneMappings && e == hipSuccess; ++li) {
-      const FlatCall c{Layout::AoS, kAutotuneLanes[li], sizes[si], in, 0, out, 0, ctx->opt,
-                       (hipStream_t)stream, 4 * ctx->compute_units};
+      FlatCall c{Layout::AoS, kAutotuneShapes[li].lanes, sizes[si], in, 0, out, 0, ctx->opt,
+                 (hipStream_t)stream, 4 * ctx->compute_units};
+      c.wpg = kAutotuneShapes[li].wpg;
       float best = 1e30f;
       for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
         e = launch_flat(ctx->precision, c);                       // warm
@@ -904,7 +923,8 @@ rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshol
     for (int li = 1; li < kAutotuneMappings; ++li)
       if (best_us[li][si] < best_us[bi][si]) bi = li;
     ctx->tuned.n[si] = sizes[si];
-    ctx->tuned.lanes[si] = kAutotuneLanes[bi];
+    ctx->tuned.lanes[si] = kAutotuneShapes[bi].lanes;
+    ctx->tuned.wpg[si] = kAutotuneShapes[bi].wpg;
     if (best_us[1][si] < best_us[0][si]) threshold = sizes[si];
     if (best_us[2][si] < best_us[1][si] && best_us[2][si] < best_us[0][si]) two_wave = sizes[si];
   }

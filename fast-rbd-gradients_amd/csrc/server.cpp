@@ -305,7 +305,8 @@ void dispatch_loop(rbd_server* s) {
       trajs.push_back(rbd_traj{reinterpret_cast<const float*>(work + r->in_off),
                                reinterpret_cast<float*>(work + r->out_off), r->n});
     }
-    const int lanes = lanes_for_batch(s->run_ctx, knots);
+    const Shape shape = shape_for_batch(s->run_ctx, knots);
+    const int lanes = shape.lanes;
     const int n_traj = int(trajs.size());
     const int64_t chunks = batch_stage(s->h_pool + off, trajs.data(), n_traj, lanes);
     const size_t out0 = off + stage_bytes + round_up(in_bytes);
@@ -322,7 +323,7 @@ void dispatch_loop(rbd_server* s) {
       e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, stage_bytes + in_bytes, hipMemcpyHostToDevice, sl.stream);
     rbd_status st = RBD_OK;
     if (e == hipSuccess)
-      st = batch_launch_staged(sl.batch, work + off, trajs.data(), n_traj, knots, chunks, lanes, sl.stream, word,
+      st = batch_launch_staged(sl.batch, work + off, trajs.data(), n_traj, knots, chunks, shape, sl.stream, word,
                                sl.seq);
     if (copy && e == hipSuccess && st == RBD_OK)
       e = hipMemcpyAsync(s->h_pool + out0, s->d_pool + out0, out_bytes, hipMemcpyDeviceToHost, sl.stream);
@@ -441,7 +442,8 @@ rbd_status warm_up(rbd_server* s) {
   const size_t in_sz = round_up(size_t(n_big) * rbd::kInPerKnot * sizeof(float));
   const size_t block = stage + in_sz + round_up(size_t(n_big) * rbd::kOutPerKnot * sizeof(float));
   if (block * s->slots.size() > pool) return RBD_OK;     // budget too small to bother
-  const int lanes = lanes_for_batch(s->run_ctx, 1);
+  const Shape shape = shape_for_batch(s->run_ctx, 1);
+  const int lanes = shape.lanes;
   // the full per-batch sequence, many times per slot, each slot in its own
   // block of the pools: the runtime sizes its internal pools (signals, command
   // buffers) on first demand
@@ -463,7 +465,7 @@ rbd_status warm_up(rbd_server* s) {
       if (copy && e == hipSuccess)
         e = hipMemcpyAsync(s->d_pool + off, s->h_pool + off, stage + in_sz, hipMemcpyHostToDevice, sl.stream);
       if (e != hipSuccess) return hip_fail(e, "server warm-up upload");
-      if (rbd_status st = batch_launch_staged(sl.batch, work + off, trajs.data(), n, n, chunks, lanes, sl.stream))
+      if (rbd_status st = batch_launch_staged(sl.batch, work + off, trajs.data(), n, n, chunks, shape, sl.stream))
         return st;
       if (copy)
         e = hipMemcpyAsync(s->h_pool + off + stage + in_sz, s->d_pool + off + stage + in_sz,
@@ -532,6 +534,7 @@ rbd_status rbd_server_create(rbd_context* ctx, const rbd_server_config* cfg, rbd
   rc.flags = ctx->flags;
   rc.precision = ctx->precision;
   rc.lanes_per_knot = c.lanes_per_knot;
+  rc.waves_per_group = ctx->waves_per_group;
   rbd_status st = rbd_create(&rc, &s->run_ctx);
   if (st == RBD_OK) s->run_ctx->opt.handoff_spins = ctx->opt.handoff_spins;   // the caller's wait bound
   const size_t budget = round_up(size_t(c.memory_budget_bytes));

@@ -60,6 +60,16 @@ EXPORTED = [
 
 # the mappings the autotuners compare, in the order of their table rows (lanes per knot)
 AUTOTUNE_LANES = (1, 16, 32, 4, 7, 8)
+# the launch shapes the autotuners time, in their table order: (lanes per
+# knot, waves per workgroup); the 4-, 7- and 8-lane groups also run in
+# workgroups of 2 and 4 waves
+AUTOTUNE_SHAPES = tuple((l, 1) for l in AUTOTUNE_LANES) + tuple((l, w) for w in (2, 4) for l in (4, 7, 8))
+
+
+def shape_key(lanes, wpg=1):
+    """Key of a launch shape in the autotune timings: "7" for 7 lanes per knot
+    in workgroups of one wave, "7x4" in workgroups of four."""
+    return str(lanes) if wpg == 1 else f"{lanes}x{wpg}"
 
 ERR_DEVICE_TIMEOUT = 7   # RBD_ERR_DEVICE_TIMEOUT
 
@@ -104,7 +114,8 @@ class ServerStats(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("flags", ctypes.c_uint), ("precision", ctypes.c_int),
-                ("lanes_per_knot", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+                ("lanes_per_knot", ctypes.c_int), ("waves_per_group", ctypes.c_int),
+                ("reserved", ctypes.c_int * 3)]
 
 
 _libs = {}
@@ -176,7 +187,7 @@ def lib(model=None):
         L.rbd_context_clear_status.argtypes = [vp]
         L.rbd_set_handoff_spins.argtypes = [vp, ctypes.c_int]
         L.rbd_mapping_table.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int),
-                                        ctypes.c_int]
+                                        ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         _libs[key] = L
     return _libs[key]
 
@@ -341,7 +352,7 @@ class Context(_Bound):
     """rbd_context: device, model flags, precision mode.  model=None is the
     KUKA iiwa 14; model="name" loads librbdgrad_name.so, built from a URDF."""
 
-    def __init__(self, device=0, flags=0, precision=PREC_FP32, lanes_per_knot=0, model=None):
+    def __init__(self, device=0, flags=0, precision=PREC_FP32, lanes_per_knot=0, model=None, waves_per_group=0):
         L = self._L = lib(model)
         self.model = L.rbd_model_name().decode()
         cfg = Config()
@@ -350,6 +361,7 @@ class Context(_Bound):
         cfg.flags = flags
         cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         cfg.lanes_per_knot = int(lanes_per_knot)
+        cfg.waves_per_group = int(waves_per_group)
         self._h = ctypes.c_void_p()
         self._check(L.rbd_create(ctypes.byref(cfg), ctypes.byref(self._h)), "rbd_create")
         self.flags = flags
@@ -407,25 +419,30 @@ class Context(_Bound):
     def kernel_name(self, n_knots, reference_mode=False):
         return self._L.rbd_kernel_name(self._h, int(bool(reference_mode)), int(n_knots)).decode()
 
-    def autotune(self, stream=None):
-        """Time the six knot-to-lane mappings on the device and install the
-        context's mapping table; returns the 16-lane crossover and the timings
-        {n: {lanes per knot: us}} (the two-wave limit is then
-        self.two_wave_threshold, the installed table self.mapping_table())."""
+    def _timings(self, fn, what, stream):
         thr = ctypes.c_int64()
-        row = 1 + len(AUTOTUNE_LANES)
+        row = 1 + len(AUTOTUNE_SHAPES)
         tab = (ctypes.c_float * (16 * row))()
-        self._check(self._L.rbd_autotune(self._h, _sh(stream), ctypes.byref(thr), tab, 16 * row), "rbd_autotune")
-        table = {int(tab[i]): {l: tab[i + 1 + j] for j, l in enumerate(AUTOTUNE_LANES)}
+        self._check(fn(self._h, _sh(stream), ctypes.byref(thr), tab, 16 * row), what)
+        table = {int(tab[i]): {shape_key(*sh): tab[i + 1 + j] for j, sh in enumerate(AUTOTUNE_SHAPES)}
                  for i in range(0, 16 * row, row) if tab[i] > 0}
         return thr.value, table
 
+    def autotune(self, stream=None):
+        """Time the twelve launch shapes (AUTOTUNE_SHAPES) on the device and
+        install the context's mapping table; returns the 16-lane crossover and
+        the timings {n: {shape_key: us}} (the two-wave limit is then
+        self.two_wave_threshold, the installed table self.mapping_table())."""
+        return self._timings(self._L.rbd_autotune, "rbd_autotune", stream)
+
     def mapping_table(self, batched=False):
-        """The installed mapping table: [(largest knots, lanes per knot), ...] ascending."""
+        """The installed mapping table: [(largest knots, lanes per knot, waves
+        per workgroup), ...] ascending."""
         n = (ctypes.c_int64 * 16)()
         lanes = (ctypes.c_int * 16)()
-        c = self._L.rbd_mapping_table(self._h, int(bool(batched)), n, lanes, 16)
-        return [(int(n[i]), int(lanes[i])) for i in range(max(c, 0))]
+        wpg = (ctypes.c_int * 16)()
+        c = self._L.rbd_mapping_table(self._h, int(bool(batched)), n, lanes, wpg, 16)
+        return [(int(n[i]), int(lanes[i]), int(wpg[i])) for i in range(max(c, 0))]
 
     @property
     def two_wave_threshold(self):
@@ -436,17 +453,10 @@ class Context(_Bound):
         self._check(self._L.rbd_set_two_wave_threshold(self._h, int(n)), "rbd_set_two_wave_threshold")
 
     def autotune_batch(self, stream=None):
-        """Time the six mappings of the batched path and install the batch
-        mapping table; returns the 16-lane crossover (total knots) and the
-        timings {total knots: {lanes per knot: us}}."""
-        thr = ctypes.c_int64()
-        row = 1 + len(AUTOTUNE_LANES)
-        tab = (ctypes.c_float * (16 * row))()
-        self._check(self._L.rbd_autotune_batch(self._h, _sh(stream), ctypes.byref(thr), tab, 16 * row),
-                    "rbd_autotune_batch")
-        table = {int(tab[i]): {l: tab[i + 1 + j] for j, l in enumerate(AUTOTUNE_LANES)}
-                 for i in range(0, 16 * row, row) if tab[i] > 0}
-        return thr.value, table
+        """Time the twelve launch shapes of the batched path and install the
+        batch mapping table; returns the 16-lane crossover (total knots) and
+        the timings {total knots: {shape_key: us}}."""
+        return self._timings(self._L.rbd_autotune_batch, "rbd_autotune_batch", stream)
 
     @property
     def batch_group_threshold(self):

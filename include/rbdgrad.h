@@ -81,7 +81,10 @@ typedef struct rbd_config {
                          32: one knot per 16-lane group in each of two waves, Minv
                          on one and the RNEA and columns on the other (lowest
                          latency; full path and batches, other calls use 16) */
-  int reserved[4];    /* must be zero */
+  int waves_per_group; /* workgroup size of the 4-, 7- and 8-lane group kernels in
+                          waves (their waves share nothing): 0 chosen per size by
+                          the mapping tables, or 1, 2, 4 */
+  int reserved[3];    /* must be zero */
 } rbd_config;
 
 /* rbd_config with device 0, flags 0 (gravity on, no damping), RBD_PREC_FP32,
@@ -245,23 +248,26 @@ rbd_status rbd_gradient_finish_f32(rbd_context* ctx, int64_t n_knots, float* d_d
  * it (template arguments abbreviated).  No reference counterpart. */
 const char* rbd_kernel_name(rbd_context* ctx, int reference_mode, int64_t n_knots);
 
-/* Autotuner for the knot-to-lane mapping (lanes_per_knot = 0): times the six
- * mappings -- 1, 16, 32 (two waves), 4, 7 and 8 lanes per knot, which also
- * fixes the knots per wave (64, 4, 4 per two waves, 16, 9, 8), the LDS each
- * wave holds (none, one Minv per knot group for 7, 8 and 16 lanes, the
- * two-wave handoff blocks) and the waves per SIMD each kernel is built for --
- * on the device at 12 call sizes from 64 to 65,536 knots (with the sizes that
- * fill the SIMDs with one wave of 9 or 16 knots), and installs the context's
- * mapping table: a call of n knots uses the mapping fastest at the smallest
+/* Autotuner for the knot-to-lane mapping (lanes_per_knot = 0): times twelve
+ * launch shapes -- 1, 16, 32 (two waves), 4, 7 and 8 lanes per knot, and the
+ * 4-, 7- and 8-lane groups also in workgroups of 2 and 4 waves.  Lanes per
+ * knot fix the knots per wave (64, 4, 4 per two waves, 16, 9, 8), the LDS
+ * each wave holds (none, one Minv per knot group for 7, 8 and 16 lanes, the
+ * two-wave handoff blocks) and the waves per SIMD each kernel is built for;
+ * the workgroup size sets the knots per workgroup.  It times them on the
+ * device at 12 call sizes from 64 to 65,536 knots (with the sizes that fill
+ * the SIMDs with one wave of 9 or 16 knots), and installs the context's
+ * mapping table: a call of n knots uses the shape fastest at the smallest
  * measured size >= n (the largest size's above it).  Also sets the largest
  * size at which the 16-lane mapping beats one lane per knot as the crossover
  * (threshold_out) and the largest at which two waves beat both as the two-wave
  * limit.  A context starts with a table of measured MI355X defaults.
  * Allocates and frees a temporary device buffer; synchronises `stream`.  Not
  * to be called while other threads launch on the same context.  table_us, if
- * not NULL, receives up to table_len floats as rows of 7: (n, us with 1, 16,
- * 32, 4, 7, 8 lanes per knot).  No reference counterpart (the reference fixes
- * dim3(32,7) per knot, time_GPU.cu:10-19). */
+ * not NULL, receives up to table_len floats as rows of 13: (n, us with 1, 16,
+ * 32, 4, 7, 8 lanes per knot, then 4, 7, 8 lanes in workgroups of 2 waves,
+ * then of 4 waves).  No reference counterpart (the reference fixes dim3(32,7)
+ * per knot, time_GPU.cu:10-19). */
 rbd_status rbd_autotune(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                         int table_len);
 /* Current crossover (knots) and its manual override; the same for the
@@ -274,21 +280,23 @@ int64_t rbd_two_wave_threshold(rbd_context* ctx);
 rbd_status rbd_set_two_wave_threshold(rbd_context* ctx, int64_t n_knots);
 
 /* The batched path (rbd_fdgrad_batched_f32) has its own mapping table on a
- * batch's total knots.  rbd_autotune_batch times the six mappings on batches
+ * batch's total knots.  rbd_autotune_batch times the twelve shapes on batches
  * of 8, 16, 32, 48, 64, 80, 96, 128, 256 and 512 trajectories of 132 knots and
  * 64 of 142 (9,088 knots) and installs the fastest at each total; it also sets the crossover (the
  * largest total at which 16-lane groups beat one lane per knot), which decides
- * alone once set by hand.  table_us receives rows of 7 as rbd_autotune's, on
+ * alone once set by hand.  table_us receives rows of 13 as rbd_autotune's, on
  * total knots.  Same conditions as rbd_autotune.  No reference counterpart. */
 rbd_status rbd_autotune_batch(rbd_context* ctx, rbd_stream_t stream, int64_t* threshold_out, float* table_us,
                               int table_len);
 int64_t rbd_batch_group_threshold(rbd_context* ctx);
 rbd_status rbd_set_batch_group_threshold(rbd_context* ctx, int64_t n_knots);
 /* The context's mapping table, per call (batched = 0) or per batch: up to
- * `capacity` entries (largest size in knots, lanes per knot), ascending;
- * returns the table's length (0: the thresholds decide), -1 for a null
- * context.  No reference counterpart. */
-int rbd_mapping_table(rbd_context* ctx, int batched, int64_t* n_knots, int* lanes, int capacity);
+ * `capacity` entries (largest size in knots, lanes per knot, waves per
+ * workgroup), ascending; any output pointer may be NULL; returns the table's
+ * length (0: the thresholds decide), -1 for a null context.  No reference
+ * counterpart. */
+int rbd_mapping_table(rbd_context* ctx, int batched, int64_t* n_knots, int* lanes, int* waves_per_group,
+                      int capacity);
 
 /* ------------------------------------------------------------------------
  * Batched trajectories of uneven length.

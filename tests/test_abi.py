@@ -57,12 +57,12 @@ def test_library_is_built_for_gfx950(model):
 def test_status_strings_and_config_default():
     import rbdgrad
     L = rbdgrad.lib()
-    assert L.rbd_abi_version() == 2
+    assert L.rbd_abi_version() == 3
     for s, text in [(0, "ok"), (1, "invalid argument"), (3, "no HIP device"), (7, "device wait timed out")]:
         assert L.rbd_status_string(s).decode() == text
     cfg = rbdgrad.Config()
     L.rbd_config_default(ctypes.byref(cfg))
-    assert (cfg.device, cfg.flags, cfg.precision) == (0, 0, 0)
+    assert (cfg.device, cfg.flags, cfg.precision, cfg.lanes_per_knot, cfg.waves_per_group) == (0, 0, 0, 0, 0)
 
 
 def test_invalid_arguments_are_reported_without_a_device():
@@ -87,9 +87,17 @@ def test_invalid_arguments_are_reported_without_a_device():
     assert L.rbd_context_status(None) == 1
     assert L.rbd_context_clear_status(None) == 1
     assert L.rbd_set_handoff_spins(None, 0) == 1
-    assert L.rbd_mapping_table(None, 0, None, None, 0) == -1
+    assert L.rbd_mapping_table(None, 0, None, None, None, 0) == -1
     cfg.flags = 0
     cfg.lanes_per_knot = 5
     assert L.rbd_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
     assert "lanes_per_knot" in L.rbd_last_error().decode()
+    cfg.lanes_per_knot = 7
+    cfg.waves_per_group = 3
+    assert L.rbd_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
+    assert "waves_per_group" in L.rbd_last_error().decode()
+    cfg.waves_per_group = 0
+    cfg.reserved[0] = 1
+    assert L.rbd_create(ctypes.byref(cfg), ctypes.byref(h)) == 1
+    assert "reserved" in L.rbd_last_error().decode()
     assert L.rbd_destroy(None) == 0

@@ -55,7 +55,7 @@ def test_uneven_batch_matches_per_trajectory_calls(rbd, ctx, oracle, use_graph):
     b.run(trajs, use_graph=use_graph)
     rbd.hip().hipDeviceSynchronize()
     total = int(lengths.sum())
-    assert b.last_lanes == next(l for n, l in ctx.mapping_table(batched=True) if total <= n)
+    assert b.last_lanes == next(l for n, l, _ in ctx.mapping_table(batched=True) if total <= n)
     got = download(trajs)
     flat_ctx = rbd.Context(precision="fp32", lanes_per_knot=b.last_lanes)
     for x, g in zip(xs, got):
@@ -233,17 +233,19 @@ def test_two_wave_batches(rbd, oracle, use_graph):
 
 def test_batch_autotune(rbd):
     c = rbd.Context(precision="fp32")
-    assert [l for _, l in c.mapping_table(batched=True)] == [7, 4, 1]   # the built-in table
+    # the built-in table: 7 lanes in workgroups of 4 waves, 4 lanes, one lane
+    assert [(l, w) for _, l, w in c.mapping_table(batched=True)] == [(7, 4), (4, 1), (1, 1)]
     thr, table = c.autotune_batch()
     sizes = sorted(table)
     assert len(sizes) == 11 and sizes[0] == 8 * 132 and sizes[-1] == 512 * 132 and 64 * 142 in sizes
-    assert all(min(t.values()) > 0 and set(t) == set(rbd.AUTOTUNE_LANES) for t in table.values())
+    keys = {rbd.shape_key(*sh) for sh in rbd.AUTOTUNE_SHAPES}
+    assert all(min(t.values()) > 0 and set(t) == keys for t in table.values())
     assert c.batch_group_threshold == thr
-    assert thr == max([0] + [n for n in sizes if table[n][16] < table[n][1]])
+    assert thr == max([0] + [n for n in sizes if table[n]["16"] < table[n]["1"]])
     installed = c.mapping_table(batched=True)
-    assert [n for n, _ in installed] == sizes
-    for n, lanes in installed:
-        assert table[n][lanes] == min(table[n].values())
+    assert [n for n, _, _ in installed] == sizes
+    for n, lanes, wpg in installed:
+        assert table[n][rbd.shape_key(lanes, wpg)] == min(table[n].values())
     say(f"batch crossover {thr} knots; table {table}")
     c.close()
 
@@ -311,6 +313,45 @@ def test_group_mapping_batches(rbd, oracle, lanes, use_graph):
         say(f"lanes {lanes}: {len(xs)} trajectories, {len(allx)} knots ok")
     b.close()
     c.close()
+
+
+@pytest.mark.parametrize("lanes", [4, 7, 8])
+def test_group_batches_in_larger_workgroups(rbd, lanes):
+    """The batch kernels of the 4-, 7- and 8-lane groups in workgroups of 2
+    and 4 waves give bitwise the results of workgroups of one wave: batches of
+    up to 64 trajectories and more, back to back without synchronisation,
+    direct and replayed, on the default grid and on grids of 3 and 5 waves
+    (not multiples of the workgroup, with more chunks than waves, so the last
+    partial round goes through the tail counters on every launch)."""
+    rng = np.random.default_rng(80 + lanes)
+    specs = [(64, 257), (3, 40), (200, 120), (65, 90), (1, 5), (64, 30)]
+    batches = []
+    for rep, (n_traj, hi) in enumerate(specs):
+        lengths = rng.integers(0, hi, n_traj)
+        lengths[0] = max(lengths[0], 1)
+        if n_traj > 3:
+            lengths[1:3] = [0, 1]
+        batches.append(make_batch(rbd, lengths, seed=800 + 10 * lanes + rep))
+    ref_ctx = rbd.Context(precision="fp32", lanes_per_knot=lanes, waves_per_group=1)
+    rb = rbd.Batch(ref_ctx, 256)
+    ref, used = run_batches(rbd, rb, batches, False)
+    assert used == [lanes] * len(specs)
+    rb.close()
+    ref_ctx.close()
+    for wpg in (2, 4):
+        c = rbd.Context(precision="fp32", lanes_per_knot=lanes, waves_per_group=wpg)
+        for grid in (None, 3, 5):
+            for use_graph in (False, True):
+                b = rbd.Batch(c, 256)
+                if grid:
+                    b.set_grid(lanes, grid)
+                got, _ = run_batches(rbd, b, batches, use_graph)
+                for g, r in zip(got, ref):
+                    for a, e in zip(g, r):
+                        np.testing.assert_array_equal(a, e)
+                b.close()
+            say(f"lanes {lanes}: workgroups of {wpg} waves, grid {grid or 'default'}: bitwise ok")
+        c.close()
 
 
 @pytest.mark.parametrize("lanes", [1, 7, 16])

@@ -93,16 +93,19 @@ def test_autotune_sets_a_crossover(rbd):
     """The autotuner measures every mapping and installs what it measured;
     which mapping wins is a timing, reported by bench.py, not asserted here."""
     ctx = rbd.Context(precision="fp32")
-    assert [l for _, l in ctx.mapping_table()] == [32, 7, 4, 1]      # the built-in table
+    # the built-in table: two waves, 7 lanes in workgroups of 4 waves, 4 lanes, one lane
+    assert [(l, w) for _, l, w in ctx.mapping_table()] == [(32, 1), (7, 4), (4, 1), (1, 1)]
     thr, table = ctx.autotune()
     say("autotune crossover", thr, table)
     assert ctx.group_threshold == thr
-    assert all(min(t.values()) > 0 and set(t) == set(rbd.AUTOTUNE_LANES) for t in table.values())
+    keys = {rbd.shape_key(*sh) for sh in rbd.AUTOTUNE_SHAPES}
+    assert len(keys) == 12
+    assert all(min(t.values()) > 0 and set(t) == keys for t in table.values())
     assert thr in (0,) + tuple(table) and ctx.two_wave_threshold in (0,) + tuple(table)
     installed = ctx.mapping_table()
-    assert [n for n, _ in installed] == sorted(table)
-    for n, lanes in installed:                                       # the fastest mapping at each size
-        assert table[n][lanes] == min(table[n].values())
+    assert [n for n, _, _ in installed] == sorted(table)
+    for n, lanes, wpg in installed:                                  # the fastest shape at each size
+        assert table[n][rbd.shape_key(lanes, wpg)] == min(table[n].values())
         assert ctx.kernel_name(n) == ctx.kernel_name(n)              # dispatch resolves
     ctx.group_threshold = 1000                                       # by hand: the thresholds decide
     assert ctx.mapping_table() == []
@@ -159,6 +162,27 @@ def test_ragged_sizes(contexts, oracle, n, lanes):
     out = ctx.run_aos(x)
     err = common.max_block_errors(out, oracle.fdgrad(x.astype(np.float64)))
     assert max(err.values()) < common.REL_TOL, err
+
+
+@pytest.mark.parametrize("lanes", [4, 7, 8])
+def test_group_workgroup_sizes_are_bitwise_equal(rbd, oracle, lanes):
+    """The 4-, 7- and 8-lane groups in workgroups of 1, 2 and 4 waves (the
+    waves of a workgroup share nothing): bitwise the same results, AoS and
+    SoA, at sizes that end inside a wave, a workgroup and the grid."""
+    ctxs = {w: rbd.Context(precision="fp32", lanes_per_knot=lanes, waves_per_group=w) for w in (1, 2, 4)}
+    for n in (1, 2, 8, 9, 10, 17, 63, 64, 65, 575, 1000, 9014, 9217, 20000):
+        x = common.knots(n, seed=700 + n)
+        ref_aos = ctxs[1].run_aos(x)
+        ref_soa = ctxs[1].run_soa(x, pad=3)
+        for w in (2, 4):
+            np.testing.assert_array_equal(ctxs[w].run_aos(x), ref_aos)
+            np.testing.assert_array_equal(ctxs[w].run_soa(x, pad=3), ref_soa)
+        if n == 1000:
+            err = common.max_block_errors(ref_aos, oracle.fdgrad(x.astype(np.float64)))
+            assert max(err.values()) < common.REL_TOL, err
+        say(f"lanes {lanes}, n={n}: workgroups of 1, 2, 4 waves agree")
+    for c in ctxs.values():
+        c.close()
 
 
 def test_zero_knots_is_a_no_op(rbd, contexts):
