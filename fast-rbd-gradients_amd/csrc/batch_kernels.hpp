@@ -409,7 +409,7 @@ __global__ __launch_bounds__(128, 1) void batch_two_wave_kernel(
     // lanes past the trajectory's end compute a copy of its last knot and store nothing
     InAoS i{in_sh + kInPerKnot * (valid ? g : knots - 1)};
     OutMasked<OutAoS> o{OutAoS{rec_sh + kOutPerKnot * g}, valid};
-    two_wave_group<P>(i, o, opt, sh[g], flags);
+    two_wave_group<P>(i, o, opt, sh[g], flags, int(threadIdx.x) / 64);
     __syncthreads();                       // both waves' parts of the records are in rec_sh
     for (int e = int(threadIdx.x); e < knots * kOutPerKnot; e += int(blockDim.x))
       d.out[kOutPerKnot * k0 + e] = rec_sh[e];

@@ -52,19 +52,6 @@ using rbdi::kBlock;
 // ---------------------------------------------------------------------------
 namespace rbd {
 
-// One lane per knot, array-of-structures, each lane loading and storing its
-// own records (64 records, so 64 cache lines, per load or store instruction).
-template <class P>
-__global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_aos_direct_kernel(long n, const float* __restrict__ in,
-                                                                   float* __restrict__ out, Options opt) {
-  const long stride = long(gridDim.x) * blockDim.x;
-  for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
-    InAoS i{in + kInPerKnot * k};
-    OutAoS o{out + kOutPerKnot * k};
-    fdgrad_knot<P>(i, o, opt);
-  }
-}
-
 // One lane per knot, array-of-structures, records staged in LDS.  A wave takes
 // 64 consecutive knots, whose 105-float records are one contiguous block of
 // 26,880 bytes: each lane writes its record into LDS at the record's own
@@ -179,7 +166,7 @@ __global__ __launch_bounds__(2 * kBlock, 1) void fdgrad_two_wave_kernel(long n, 
   if constexpr (SOA) { i = InSoA{in + kk, ld_in}; o0 = OutSoA{out + kk, ld_out}; }
   else { i = InAoS{in + kInPerKnot * kk}; o0 = OutAoS{out + kOutPerKnot * kk}; }
   OutMasked<O> o{o0, valid};
-  two_wave_group<P>(i, o, opt, sh[g], flags);
+  two_wave_group<P>(i, o, opt, sh[g], flags, int(threadIdx.x) / 64);
 }
 
 // L lanes per knot (L = 4, 7, 8; group_mapping.hpp): floor(64 / L) knots per

@@ -43,13 +43,14 @@ __device__ inline void lds_wait(int* flag, const Options& opt) {
 template <typename S>
 __device__ inline void pin(S x) { asm volatile("" ::"v"(x)); }
 
-// One knot per 16-lane group of each wave of the calling two-wave workgroup:
-// i, o are the knot of this lane's group (o masked for lanes past the last
-// knot, which compute a copy of it); sh is the group's LDS block and flags
-// the workgroup's two flags, both zero on entry.
+// One knot per 16-lane group of each wave of a pair of waves: i, o are the
+// knot of this lane's group (o masked for lanes past the last knot, which
+// compute a copy of it); sh is the group's LDS block and flags the pair's two
+// flags, both zero on entry; wave is the calling wave's place in the pair (0:
+// Minv, 1: the RNEA and the columns).
 template <class P, class In, class O>
-__device__ RBD_INLINE void two_wave_group(const In& i, O& o, const Options& opt, TwoWaveShared<P>& sh, int* flags) {
-  const int wave = threadIdx.x / 64;
+__device__ RBD_INLINE void two_wave_group(const In& i, O& o, const Options& opt, TwoWaveShared<P>& sh, int* flags,
+                                          int wave) {
   const int role = (threadIdx.x % 64) % kLanesPerKnotGroup;
   if (wave == 0) {
     CS<TwoWaveTrig<P>> tw[NQ];

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(128, 1) void single_full_two_wave(const float* in, 
     __syncthreads();
     rbd::InAoS i{in};
     rbd::OutMasked<rbd::OutAoS> o{rbd::OutAoS{out}, g == 0};
-    rbd::two_wave_group<rbd::PrecFP32>(i, o, opt, sh[g], flags);
+    rbd::two_wave_group<rbd::PrecFP32>(i, o, opt, sh[g], flags, int(threadIdx.x) / 64);
   }
 }
 
