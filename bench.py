@@ -69,10 +69,8 @@ MEASURED_TRAFFIC_BYTES = {
 # FP32 FLOPs per knot (an FMA counts 2).  The algorithmic count above is the
 # reference method's, whose dense 6x6 inertia products include structural
 # zeros the kernel never computes, so its roofline fraction can pass 1.
-EXECUTED_VALU_PER_WAVE = 7147
-EXECUTED_FLOPS_PER_KNOT = 10878
-FP32_NONPACKED_TFLOPS = 78.6  # the VALU's FP32 rate without packed instructions (switched off in these builds)
-VALU_WAVE_INSTR_PER_S = 1024 * 2.4e9 / 4   # 1,024 SIMDs at 2.4 GHz, 4 cycles per wave64 VALU instruction
+EXECUTED_VALU_PER_WAVE = 6611
+EXECUTED_FLOPS_PER_KNOT = 10900
 N_SET = (1, 10, 16, 32, 64, 128, 256)   # trajectory lengths of the reference's harness (time_GPU.cu:344-351)
 HARNESS = os.path.join(PKG, "harness", "time_gpu")
 
@@ -642,9 +640,8 @@ def main():
     if key == "fp32_soa":      # the counters are of this kernel
         t = h["ms_per_launch"] * 1e-3
         ex_tflops = EXECUTED_FLOPS_PER_KNOT * LARGE / t * 1e-12
-        executed = {"flops_per_knot": EXECUTED_FLOPS_PER_KNOT, "tflops": ex_tflops,
-                    "frac_nonpacked": ex_tflops / FP32_NONPACKED_TFLOPS,
-                    "valu_issue_frac": EXECUTED_VALU_PER_WAVE * (LARGE // 64) / (t * VALU_WAVE_INSTR_PER_S)}
+        executed = {"flops_per_knot": EXECUTED_FLOPS_PER_KNOT, "valu_instr_per_knot": EXECUTED_VALU_PER_WAVE,
+                    "tflops": ex_tflops, "frac": ex_tflops / FP32_PEAK_TFLOPS}
     mixed = detail["mixed_64_traj"]
     pc = detail["per_call_n128"]
     srv = detail["served"]["auto_two_wave"]

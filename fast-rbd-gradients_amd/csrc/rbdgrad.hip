@@ -60,7 +60,8 @@ namespace rbd {
 // covers 1 KB of contiguous output (8 cache lines) instead of 64 records'
 // lines.  Per-lane record stores had the cache and address units stalling the
 // waves (at 2^20 knots: 5.2x the L1 accesses and 4.9x the L2 write requests
-// of the structure-of-arrays layout).  The block limits a CU to 6 waves.
+// of the structure-of-arrays layout).  The block limits a CU to 6 waves.  The
+// copy uses streaming (non-temporal) stores.
 template <class P>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_aos_kernel(long n, const float* __restrict__ in,
                                                             float* __restrict__ out, Options opt) {
@@ -74,10 +75,23 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_aos_kernel(lo
     OutAoS o{rec + kOutPerKnot * lane};
     fdgrad_knot<P>(i, o, opt);
     wave_lds_sync();
-    copy_record_block(rec, out + k0 * kOutPerKnot, nk * kOutPerKnot, lane);
+    copy_record_block<true>(rec, out + k0 * kOutPerKnot, nk * kOutPerKnot, lane);
     wave_lds_sync();                        // read out before the next knots overwrite rec
   }
 }
+
+// Structure-of-arrays output of the one-lane kernel with streaming
+// (non-temporal) stores: each store writes one output row for the wave's 64
+// consecutive knots, 256 contiguous bytes, written once.  At 2^20 knots the
+// kernel measured 162 against 186 us with plain stores, whose write-allocated
+// lines kept the waves waiting.
+struct OutSoAStream {
+  float* p; long ld;
+  __device__ void st(long row, float x) const { put<true>(p + row * ld, x); }
+  __device__ void qdd(int i, float x) const { st(i, x); }
+  __device__ void dq(int r, int c, float x) const { st(NQ + NQ * c + r, x); }
+  __device__ void dqd(int r, int c, float x) const { st(NQ + NQ * NQ + NQ * c + r, x); }
+};
 
 template <class P>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_soa_kernel(long n, const float* __restrict__ in, long ld_in,
@@ -85,7 +99,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_soa_kernel(lo
   const long stride = long(gridDim.x) * blockDim.x;
   for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
     InSoA i{in + k, ld_in};
-    OutSoA o{out + k, ld_out};
+    OutSoAStream o{out + k, ld_out};
     fdgrad_knot<P>(i, o, opt);
   }
 }
