@@ -236,7 +236,7 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
   // first: a few whole-line reads instead of many small per-lane ones)
   // one lane per knot: the chunk's records (64 consecutive knots of one
   // trajectory, one contiguous block) are assembled in LDS and copied out
-  // whole (copy_record_block), as the flat one-lane kernel does
+  // whole (copy_record_block, streaming stores), as the flat one-lane kernel does
   __shared__ __align__(16) float rec_sh[chunk * kOutPerKnot];
   __shared__ float in_sh[LANES == 1 ? 1 : chunk * kInPerKnot];
   auto process = [&](int c) {
@@ -252,7 +252,9 @@ __global__ __launch_bounds__(64, kWavesPerSimd<P>) void batch_persistent_kernel(
       OutAoS o{rec_sh + kOutPerKnot * slot};
       fdgrad_knot<P>(i, o, opt);
       wave_lds_sync();
-      copy_record_block(rec_sh, d.out + kOutPerKnot * k0, knots * kOutPerKnot, lane);
+      // streaming stores, as the flat kernel's (whole lines; made visible by
+      // the kernel's end or the completion word's release like any other store)
+      copy_record_block<true>(rec_sh, d.out + kOutPerKnot * k0, knots * kOutPerKnot, lane);
       wave_lds_sync();                      // read out before the next chunk overwrites it
     } else {
       const int knots = int(d.n - k0 < chunk ? d.n - k0 : chunk);
