@@ -60,8 +60,8 @@ LARGE = 1 << 20               # knots of the large batch
 # other configurations report null.
 MEASURED_TRAFFIC_BYTES = {
     ("fdgrad_two_wave_kernel<PrecFP32>", 128): int((181.0 * 2.000 + 53.0 * 1.000) * 1024),
-    ("fdgrad_soa_kernel<PrecFP32>", LARGE): int((41337.1 * 2.000 + 430080.0) * 1024),
-    ("fdgrad_aos_kernel<PrecFP32>", LARGE): int((47763.9 * 2.000 + 454638.4) * 1024),
+    ("fdgrad_soa_kernel<PrecFP32>", LARGE): int((41326.3 * 2.000 + 430080.8) * 1024),
+    ("fdgrad_aos_kernel<PrecFP32>", LARGE): int((43358.5 * 2.000 + 430235.8) * 1024),
 }
 # Executed work of the one-lane FP32 kernel per 64-knot wave (one knot per
 # lane), from rocprofv3 --pmc SQ_INSTS_VALU and SQ_INSTS_VALU_FLOPS_FP32 at 2^20
