@@ -66,6 +66,10 @@ constexpr int64_t kDefaultBatchGroupThreshold = 8192;
 // Batches of up to this many trajectories find a chunk's trajectory inside the
 // wave, larger ones through the plan kernel (rbd::kSmallBatch, batch_kernels.hpp).
 constexpr int kSmallBatchTrajectories = 64;
+// Host-pointer calls of up to this many knots stage through pinned, mapped
+// host memory that the kernel reads and writes directly (8.3 MB, allocated at
+// the first such call); larger ones copy through a device buffer.
+constexpr int64_t kHostZeroCopyKnots = 16384;
 
 // Records msg as the calling thread's last error and returns s.
 rbd_status fail(rbd_status s, const std::string& msg);
@@ -104,10 +108,16 @@ struct rbd_context {
   rbdi::MappingTable tuned;                      // per call, set by rbd_autotune
   rbdi::MappingTable batch_tuned;                // per batch (total knots), set by rbd_autotune_batch
   rbd::Options opt;
-  // device buffer of the host-pointer convenience call
+  // buffers of the host-pointer convenience call: pinned, mapped staging that
+  // the kernel reads and writes directly (calls of up to kHostZeroCopyKnots
+  // knots), else a device buffer between two copies; its own stream
   std::mutex host_mu;
   float* d_scratch = nullptr;
   size_t scratch_bytes = 0;
+  char* h_stage = nullptr;
+  char* dh_stage = nullptr;
+  size_t stage_bytes = 0;
+  hipStream_t host_stream = nullptr;
   // the word a two-wave kernel's handoff wait sets when it gives up: pinned,
   // coherent host memory (opt.fault is its device address)
   volatile unsigned* fault_host = nullptr;

@@ -504,13 +504,18 @@ rbd_status rbd_create(const rbd_config* cfg, rbd_context** out) {
 
 rbd_status rbd_destroy(rbd_context* ctx) {
   if (!ctx) return RBD_OK;
-  if (ctx->d_scratch) {
+  if (ctx->d_scratch || ctx->host_stream) {
     int cur = -1;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(ctx->device);
-    (void)hipFree(ctx->d_scratch);
+    if (ctx->host_stream) {
+      (void)hipStreamSynchronize(ctx->host_stream);
+      (void)hipStreamDestroy(ctx->host_stream);
+    }
+    if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
     if (cur >= 0) (void)hipSetDevice(cur);
   }
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   if (ctx->fault_host) (void)hipHostFree(const_cast<unsigned*>(ctx->fault_host));
   delete ctx;
   return RBD_OK;
@@ -585,24 +590,60 @@ rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n, const float* h_
   std::lock_guard<std::mutex> lock(ctx->host_mu);
   const size_t in_bytes = size_t(n) * rbd::kInPerKnot * sizeof(float);
   const size_t out_bytes = size_t(n) * rbd::kOutPerKnot * sizeof(float);
+  hipError_t e = hipSuccess;
+  if (!ctx->host_stream) {
+    e = hipStreamCreateWithFlags(&ctx->host_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { ctx->host_stream = nullptr; return hip_fail(e, "hipStreamCreate (host call)"); }
+  }
+  const Shape sh = shape_for(ctx, n, true);
+  if (n <= kHostZeroCopyKnots) {
+    // the kernel reads the inputs from and writes the outputs to pinned,
+    // mapped host memory: one launch and one wait, no copy engine
+    if (ctx->stage_bytes < in_bytes + out_bytes) {
+      if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+      ctx->h_stage = ctx->dh_stage = nullptr;
+      ctx->stage_bytes = 0;
+      const size_t bytes = size_t(kHostZeroCopyKnots) * (rbd::kInPerKnot + rbd::kOutPerKnot) * sizeof(float);
+      void* h = nullptr;
+      e = hipHostMalloc(&h, bytes, hipHostMallocMapped | hipHostMallocCoherent);
+      if (e != hipSuccess) return fail(RBD_ERR_OUT_OF_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+      void* d = nullptr;
+      e = hipHostGetDevicePointer(&d, h, 0);
+      if (e != hipSuccess) { (void)hipHostFree(h); return hip_fail(e, "hipHostGetDevicePointer (host call)"); }
+      ctx->h_stage = static_cast<char*>(h);
+      ctx->dh_stage = static_cast<char*>(d);
+      ctx->stage_bytes = bytes;
+    }
+    std::memcpy(ctx->h_stage, h_in, in_bytes);
+    const float* d_in = reinterpret_cast<const float*>(ctx->dh_stage);
+    float* d_out = reinterpret_cast<float*>(ctx->dh_stage + in_bytes);
+    FlatCall c{Layout::AoS, sh.lanes, n, d_in, 0, d_out, 0, ctx->opt, ctx->host_stream, 4 * ctx->compute_units};
+    c.wpg = sh.wpg;
+    e = launch_flat(ctx->precision, c);
+    if (e != hipSuccess) return hip_fail(e, "fdgrad kernel launch");
+    e = hipStreamSynchronize(ctx->host_stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize (host call)");
+    std::memcpy(h_out, ctx->h_stage + in_bytes, out_bytes);
+    return RBD_OK;
+  }
   if (ctx->scratch_bytes < in_bytes + out_bytes) {
     if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
     ctx->d_scratch = nullptr;
     ctx->scratch_bytes = 0;
-    hipError_t e = hipMalloc(&ctx->d_scratch, in_bytes + out_bytes);
+    e = hipMalloc(&ctx->d_scratch, in_bytes + out_bytes);
     if (e != hipSuccess) return fail(RBD_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
     ctx->scratch_bytes = in_bytes + out_bytes;
   }
   float* d_in = ctx->d_scratch;
   float* d_out = ctx->d_scratch + size_t(n) * rbd::kInPerKnot;
-  hipError_t e = hipMemcpy(d_in, h_in, in_bytes, hipMemcpyHostToDevice);
+  e = hipMemcpyAsync(d_in, h_in, in_bytes, hipMemcpyHostToDevice, ctx->host_stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
-  const Shape sh = shape_for(ctx, n, true);
-  FlatCall c{Layout::AoS, sh.lanes, n, d_in, 0, d_out, 0, ctx->opt, (hipStream_t)0, 4 * ctx->compute_units};
+  FlatCall c{Layout::AoS, sh.lanes, n, d_in, 0, d_out, 0, ctx->opt, ctx->host_stream, 4 * ctx->compute_units};
   c.wpg = sh.wpg;
   e = launch_flat(ctx->precision, c);
   if (e != hipSuccess) return hip_fail(e, "fdgrad kernel launch");
-  e = hipMemcpy(h_out, d_out, out_bytes, hipMemcpyDeviceToHost);
+  e = hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->host_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->host_stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpy D2H");
   return RBD_OK;
 }

@@ -154,9 +154,12 @@ rbd_status rbd_dynamics_gradient_kernel_f32(rbd_context* ctx, int64_t n_knots, f
                                             const float* d_mem_vol, const float* d_mem_const,
                                             rbd_stream_t stream);
 
-/* Host-pointer convenience: copies h_in to the device, runs the full path,
- * copies the result to h_out and waits (the reference's "GPU Compute + I/O"
- * timing, time_GPU.cu:311-320).  Uses and keeps a per-context device buffer.
+/* Host-pointer convenience: runs the full path on h_in, writes h_out and
+ * waits (the reference's "GPU Compute + I/O" timing, time_GPU.cu:311-320).
+ * Calls of up to 16,384 knots copy h_in into pinned, mapped host memory that
+ * the kernel reads and writes directly (kept by the context, 8.3 MB); larger
+ * ones copy through a per-context device buffer.  Calls on one context are
+ * serialised; they run on a stream of the context's own.
  * Replaces: the H2D copy + kernel + D2H copy sequence of time_GPU.cu:313-316. */
 rbd_status rbd_fdgrad_traj_host_f32(rbd_context* ctx, int64_t n_knots, const float* h_in, float* h_out);
 

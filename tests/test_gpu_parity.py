@@ -144,11 +144,17 @@ def test_handoff_timeout_is_reported(rbd):
     ctx.close()
 
 
-def test_host_pointer_call_equals_device_call(contexts):
-    x = common.knots(777, seed=301)
-    say("host-pointer call n=777")
+@pytest.mark.parametrize("n", [1, 128, 777, 16384, 16385, 40000])
+def test_host_pointer_call_equals_device_call(contexts, n):
+    """Host buffers: up to 16,384 knots through pinned memory the kernel reads
+    and writes directly, above through device buffers; the same kernel as the
+    device-pointer call, so bitwise its results, also on repeated calls."""
+    x = common.knots(n, seed=301 + n)
+    say(f"host-pointer call n={n}")
     ctx = contexts[(0, "fp32")]
-    np.testing.assert_array_equal(ctx.fdgrad_host(x), ctx.run_aos(x))
+    ref = ctx.run_aos(x)
+    for _ in range(2):
+        np.testing.assert_array_equal(ctx.fdgrad_host(x), ref)
 
 
 @pytest.mark.parametrize("lanes", [0, 7, 4])
