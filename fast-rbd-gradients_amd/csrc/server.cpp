@@ -727,16 +727,22 @@ rbd_status rbd_server_run_poisson(rbd_server* s, double rate_per_s, double durat
   std::exponential_distribution<double> gap(rate_per_s);
   std::uniform_int_distribution<int> pick(0, n_lengths - 1);
   const int64_t expected = int64_t(rate_per_s * duration_s * 1.2) + 16;
-  // result buffers are recycled once their request completed
+  // result buffers are recycled once their request completed.  They are
+  // touched here, before the clock starts (and when more are needed, here in
+  // the load generator): an untouched buffer would take its page faults in
+  // the server's delivery copy, whose slowdown would then deepen the backlog
+  // and with it the number of fresh buffers.  About 20 ms of requests at the
+  // offered rate are prepared (at least 2,048).
   const size_t buf_floats = size_t(maxlen) * rbd::kOutPerKnot;
   std::vector<std::unique_ptr<float[]>> outbufs;
   std::deque<std::pair<uint64_t, size_t>> pending;   // (id, outbuf index)
   std::vector<size_t> free_bufs;
   (void)expected;
-  for (size_t i = 0; i < 2048; ++i) {
+  const size_t prepared = std::max<size_t>(2048, size_t(rate_per_s * 0.02));
+  for (size_t i = 0; i < prepared; ++i) {
     outbufs.emplace_back(new float[buf_floats]);
     std::memset(outbufs.back().get(), 0, buf_floats * sizeof(float));
-    free_bufs.push_back(2047 - i);
+    free_bufs.push_back(prepared - 1 - i);
   }
   const auto t0 = Clock::now();
   double t = 0.0;
@@ -762,7 +768,11 @@ rbd_status rbd_server_run_poisson(rbd_server* s, double rate_per_s, double durat
     if (cursor + n > pool_knots) cursor = 0;
     size_t bi;
     if (!free_bufs.empty()) { bi = free_bufs.back(); free_bufs.pop_back(); }
-    else { outbufs.emplace_back(new float[buf_floats]); bi = outbufs.size() - 1; }
+    else {
+      outbufs.emplace_back(new float[buf_floats]);
+      std::memset(outbufs.back().get(), 0, buf_floats * sizeof(float));
+      bi = outbufs.size() - 1;
+    }
     uint64_t id = 0;
     st = rbd_server_submit(s, pool_in + cursor * rbd::kInPerKnot, outbufs[bi].get(), n, &id);
     if (st != RBD_OK) break;
