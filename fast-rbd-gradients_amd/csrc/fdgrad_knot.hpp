@@ -280,8 +280,9 @@ using PrecFP64      = Precision<double, double, double, double>; // all FP64
 
 // Waves each SIMD keeps resident for the kernels of precision P.  With FP32
 // RNEA and columns a knot's state fits in 256 registers per lane, so two waves
-// share a SIMD and each hides the other's latency (1.65x the throughput of one
-// wave at 2^20 knots); with FP64 columns it needs about 500 and runs one.
+// share a SIMD and each hides the other's latency (1.44x the throughput of one
+// wave at 2^20 knots when measured); with FP64 columns it needs about 500 and
+// runs one.
 template <class P>
 constexpr int kWavesPerSimd = (sizeof(typename P::Col) == 4 && sizeof(typename P::Rnea) == 4) ? 2 : 1;
 
