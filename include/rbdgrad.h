@@ -360,9 +360,11 @@ rbd_status rbd_fdgrad_batched_f32(rbd_batch* b, const rbd_traj* trajs, int n_tra
 rbd_status rbd_fdgrad_batched_noalloc_f32(rbd_context* ctx, const rbd_traj* trajs, int n_traj, rbd_stream_t stream);
 
 /* Persistent grid size (waves) per mapping, lanes_per_knot 1, 4, 7, 8 or 16
- * (default: the waves the device's SIMDs keep resident, 4 per CU times 2 with
- * FP32 columns or 1 with FP64 columns for 1 and 16 lanes, 4 per CU for 4, 7
- * and 8; the two-wave mapping runs 2 workgroups of two waves per CU). */
+ * (default: the waves the device's SIMDs keep resident: per CU 6 for one lane
+ * and 8 for 16 lanes with FP32 columns, 4 for either with FP64 columns, 4 for
+ * 4, 7 and 8 lanes in every workgroup size; the two-wave mapping runs 2
+ * workgroups of two waves per CU).  The grid of the 4-, 7- and 8-lane groups
+ * is set for all three of their workgroup sizes at once. */
 rbd_status rbd_batch_set_grid(rbd_batch* b, int lanes_per_knot, unsigned waves);
 
 /* Lanes per knot used by the last rbd_fdgrad_batched_f32 call (1, 4, 7, 8, 16 or 32). */
