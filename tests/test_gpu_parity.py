@@ -269,6 +269,41 @@ def test_large_batch_every_knot(rbd, large_batch, prec, bar):
     ctx.close()
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64_columns"])
+def test_one_lane_calls_beyond_the_resident_waves(rbd, oracle, prec):
+    """A one-lane call of more 64-knot blocks than the device keeps resident
+    runs one round of resident waves, each looping over blocks w, w + G, ...
+    Here 6,145 blocks (a ragged size) on 2,048 or 1,536 waves with FP32
+    columns, 1,024 with FP64 columns: every knot is written (the output starts
+    as NaN), AoS and SoA agree bit for bit, knots computed alone are bitwise
+    the same, sampled knots (the last block among them) hold the bar, and the
+    reference-compatible layout gives each tiled golden knot its 128-knot
+    result."""
+    n = 3 * (1 << 17) + 37
+    x = common.knots(n, seed=306, near_singular_frac=0.1)
+    say(f"one lane beyond resident waves n={n} prec={prec}")
+    ctx = rbd.Context(precision=prec, lanes_per_knot=1)
+    d_in = rbd.DeviceBuffer.from_array(x)
+    d_out = rbd.DeviceBuffer.from_array(np.full((n, rbd.OUT_PER_KNOT), np.nan, np.float32))
+    ctx.fdgrad(n, d_in, d_out)
+    out = d_out.download((n, rbd.OUT_PER_KNOT))
+    d_in.free()
+    d_out.free()
+    assert np.isfinite(out).all()
+    np.testing.assert_array_equal(ctx.run_soa(x), out)
+    rng = np.random.default_rng(7)
+    sample = np.concatenate([rng.choice(n - 64, 2000, replace=False), np.arange(n - 64, n)])
+    for i in sample[::50]:
+        np.testing.assert_array_equal(ctx.run_aos(x[i:i + 1])[0], out[i])
+    err = common.max_block_errors(out[sample], oracle.fdgrad(x[sample].astype(np.float64)))
+    assert max(err.values()) < BOUND[prec], err
+    vol = GOLDEN["refmode_n128__in"]
+    one = ctx.run_refmode(vol)
+    tiled = ctx.run_refmode(np.tile(vol, (n // len(vol) + 1, 1))[:n])
+    np.testing.assert_array_equal(tiled, np.tile(one, (n // len(vol) + 1, 1))[:n])
+    ctx.close()
+
+
 def test_fp32_error_statistics_near_singular(contexts, oracle):
     """Every knot at a stretched-arm pose: each precision mode holds its bar at
     the largest error over the knots."""
