@@ -60,16 +60,16 @@ LARGE = 1 << 20               # knots of the large batch
 # other configurations report null.
 MEASURED_TRAFFIC_BYTES = {
     ("fdgrad_two_wave_kernel<PrecFP32>", 128): int((181.0 * 2.000 + 53.0 * 1.000) * 1024),
-    ("fdgrad_soa_kernel<PrecFP32>", LARGE): int((41326.3 * 2.000 + 430080.8) * 1024),
-    ("fdgrad_aos_kernel<PrecFP32>", LARGE): int((43358.5 * 2.000 + 430235.8) * 1024),
+    ("fdgrad_soa_kernel<PrecFP32>", LARGE): int((41326.0 * 2.000 + 430081.4) * 1024),
+    ("fdgrad_aos_kernel<PrecFP32>", LARGE): int((43358.5 * 2.000 + 430325.9) * 1024),
 }
-# Executed work of the one-lane FP32 kernel per 64-knot wave (one knot per
+# Executed work of the one-lane FP32 kernel per 64-knot block (one knot per
 # lane), from rocprofv3 --pmc SQ_INSTS_VALU and SQ_INSTS_VALU_FLOPS_FP32 at 2^20
 # knots (profiles/fdgrad_large_batch_2p20_soa_aos.md): VALU instructions, and
 # FP32 FLOPs per knot (an FMA counts 2).  The algorithmic count above is the
 # reference method's, whose dense 6x6 inertia products include structural
 # zeros the kernel never computes, so its roofline fraction can pass 1.
-EXECUTED_VALU_PER_WAVE = 6611
+EXECUTED_VALU_PER_KNOT = 6592
 EXECUTED_FLOPS_PER_KNOT = 10900
 N_SET = (1, 10, 16, 32, 64, 128, 256)   # trajectory lengths of the reference's harness (time_GPU.cu:344-351)
 HARNESS = os.path.join(PKG, "harness", "time_gpu")
@@ -547,7 +547,8 @@ def profile_one(args, stream):
         n, layout = LARGE, args.only[3:]
         x = knots(n, seed=104, near_singular_frac=0.1)
     ms, _ = run_flat(ctx, x, args.steps, min(args.warmup, 10), stream, layout)
-    log(f"{args.only} ({ctx.model}, {layout}, {args.precision}): {ctx.kernel_name(n)}, n={n}: "
+    kname = ctx.kernel_name(n).replace("_aos_", f"_{layout}_")    # the library names the AoS kernel
+    log(f"{args.only} ({ctx.model}, {layout}, {args.precision}): {kname}, n={n}: "
         f"{ms / args.steps * 1e3:.3f} us per launch over {args.steps}")
 
 
@@ -640,7 +641,7 @@ def main():
     if key == "fp32_soa":      # the counters are of this kernel
         t = h["ms_per_launch"] * 1e-3
         ex_tflops = EXECUTED_FLOPS_PER_KNOT * LARGE / t * 1e-12
-        executed = {"flops_per_knot": EXECUTED_FLOPS_PER_KNOT, "valu_instr_per_knot": EXECUTED_VALU_PER_WAVE,
+        executed = {"flops_per_knot": EXECUTED_FLOPS_PER_KNOT, "valu_instr_per_knot": EXECUTED_VALU_PER_KNOT,
                     "tflops": ex_tflops, "frac": ex_tflops / FP32_PEAK_TFLOPS}
     mixed = detail["mixed_64_traj"]
     pc = detail["per_call_n128"]
