@@ -85,23 +85,59 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_aos_kernel(lo
 // consecutive knots, 256 contiguous bytes, written once.  At 2^20 knots the
 // kernel measured 162 against 186 us with plain stores, whose write-allocated
 // lines kept the waves waiting.
+// The last column computed (d/dqd column 6, the record's last 7 rows) is kept
+// in `last` instead of stored: the kernel stores it after the next knot's
+// input loads have issued.
 struct OutSoAStream {
   float* p; long ld;
+  float* last;
   __device__ void st(long row, float x) const { put<true>(p + row * ld, x); }
   __device__ void qdd(int i, float x) const { st(i, x); }
   __device__ void dq(int r, int c, float x) const { st(NQ + NQ * c + r, x); }
-  __device__ void dqd(int r, int c, float x) const { st(NQ + NQ * NQ + NQ * c + r, x); }
+  __device__ void dqd(int r, int c, float x) const {
+    if (c == NQ - 1) last[r] = x;
+    else st(NQ + NQ * NQ + NQ * c + r, x);
+  }
 };
 
+// A knot's 21 inputs read into registers before anything else of its loop
+// iteration.
+struct InSoARegs {
+  float x[kInPerKnot];
+  const float* fx = nullptr;
+  __device__ float q(int i) const { return x[i]; }
+  __device__ float qd(int i) const { return x[NQ + i]; }
+  __device__ float u(int i) const { return x[2 * NQ + i]; }
+};
+
+// One lane per knot, structure of arrays.  A wave's vector-memory counter
+// retires loads and stores in the order they were issued, so the wait for a
+// knot's input loads also waits for every store issued before them.  Each
+// loop iteration therefore issues its knot's loads first and only then stores
+// the previous knot's last column, which the wait does not cover; the stores
+// before it were issued a column's work earlier and have retired.  At 2^20
+// knots: 156.0-157.0 against 162.0-162.2 us with the last column stored at
+// the end of its own iteration, bitwise the same results.
 template <class P>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_soa_kernel(long n, const float* __restrict__ in, long ld_in,
                                                             float* __restrict__ out, long ld_out, Options opt) {
+  constexpr int kLastRow = kOutPerKnot - NQ;
   const long stride = long(gridDim.x) * blockDim.x;
+  float last[NQ];
+  long prev = -1;
   for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
-    InSoA i{in + k, ld_in};
-    OutSoAStream o{out + k, ld_out};
+    InSoARegs i;
+    for (int j = 0; j < kInPerKnot; ++j) i.x[j] = in[j * ld_in + k];
+    __builtin_amdgcn_sched_barrier(0);
+    if (prev >= 0)
+      for (int r = 0; r < NQ; ++r) put<true>(out + (kLastRow + r) * ld_out + prev, last[r]);
+    __builtin_amdgcn_sched_barrier(0);
+    OutSoAStream o{out + k, ld_out, last};
     fdgrad_knot<P>(i, o, opt);
+    prev = k;
   }
+  if (prev >= 0)
+    for (int r = 0; r < NQ; ++r) put<true>(out + (kLastRow + r) * ld_out + prev, last[r]);
 }
 
 template <class P>
