@@ -52,6 +52,16 @@ using rbdi::kBlock;
 // ---------------------------------------------------------------------------
 namespace rbd {
 
+// A knot's 21 inputs read into registers before anything else of its loop
+// iteration (the one-lane kernels below).
+struct InRegs {
+  float x[kInPerKnot];
+  const float* fx = nullptr;
+  __device__ float q(int i) const { return x[i]; }
+  __device__ float qd(int i) const { return x[NQ + i]; }
+  __device__ float u(int i) const { return x[2 * NQ + i]; }
+};
+
 // One lane per knot, array-of-structures, records staged in LDS.  A wave takes
 // 64 consecutive knots, whose 105-float records are one contiguous block of
 // 26,880 bytes: each lane writes its record into LDS at the record's own
@@ -62,32 +72,49 @@ namespace rbd {
 // waves (at 2^20 knots: 5.2x the L1 accesses and 4.9x the L2 write requests
 // of the structure-of-arrays layout).  The block limits a CU to 6 waves.  The
 // copy uses streaming (non-temporal) stores.
+//
+// A wave's vector-memory counter retires loads and stores in the order they
+// were issued, so waiting for a block's input loads also waits for every store
+// issued before them.  Each loop iteration therefore issues its block's loads
+// first and only then copies out the previous block's records (2^20 knots:
+// 179.5-181.4 against 183.3-184.9 us).  The copy's LDS reads precede the next
+// records' LDS writes in program order, on the same array, and a wave's LDS
+// operations execute in order, so no fence separates them; a fence there
+// changed how the compiler contracted the knot's products into FMAs, and with
+// it the results' last bits.
 template <class P>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_aos_kernel(long n, const float* __restrict__ in,
                                                             float* __restrict__ out, Options opt) {
   __shared__ __align__(16) float rec[kRecordBlockFloats];
   const int lane = int(threadIdx.x);
   const long stride = long(gridDim.x) * kBlock;
+  long prev = 0;          // first knot of the block whose records rec holds
+  int prev_words = 0;     // and their words (0: none yet)
   for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
     const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
     // lanes past the last knot compute a copy of it; only nk records are written out
-    InAoS i{in + kInPerKnot * (k0 + (lane < nk ? lane : nk - 1))};
+    const float* src = in + kInPerKnot * (k0 + (lane < nk ? lane : nk - 1));
+    InRegs i;
+    for (int j = 0; j < kInPerKnot; ++j) i.x[j] = src[j];
+    __builtin_amdgcn_sched_barrier(0);
+    copy_record_block<true>(rec, out + prev * kOutPerKnot, prev_words, lane);
+    __builtin_amdgcn_sched_barrier(0);
     OutAoS o{rec + kOutPerKnot * lane};
     fdgrad_knot<P>(i, o, opt);
     wave_lds_sync();
-    copy_record_block<true>(rec, out + k0 * kOutPerKnot, nk * kOutPerKnot, lane);
-    wave_lds_sync();                        // read out before the next knots overwrite rec
+    prev = k0;
+    prev_words = nk * kOutPerKnot;
   }
+  copy_record_block<true>(rec, out + prev * kOutPerKnot, prev_words, lane);
 }
 
 // Structure-of-arrays output of the one-lane kernel with streaming
 // (non-temporal) stores: each store writes one output row for the wave's 64
 // consecutive knots, 256 contiguous bytes, written once.  At 2^20 knots the
 // kernel measured 162 against 186 us with plain stores, whose write-allocated
-// lines kept the waves waiting.
-// The last column computed (d/dqd column 6, the record's last 7 rows) is kept
-// in `last` instead of stored: the kernel stores it after the next knot's
-// input loads have issued.
+// lines kept the waves waiting.  The last column computed (d/dqd column 6,
+// the record's last 7 rows) is kept in `last` instead of stored: the kernel
+// stores it after the next knot's input loads have issued.
 struct OutSoAStream {
   float* p; long ld;
   float* last;
@@ -100,24 +127,12 @@ struct OutSoAStream {
   }
 };
 
-// A knot's 21 inputs read into registers before anything else of its loop
-// iteration.
-struct InSoARegs {
-  float x[kInPerKnot];
-  const float* fx = nullptr;
-  __device__ float q(int i) const { return x[i]; }
-  __device__ float qd(int i) const { return x[NQ + i]; }
-  __device__ float u(int i) const { return x[2 * NQ + i]; }
-};
-
-// One lane per knot, structure of arrays.  A wave's vector-memory counter
-// retires loads and stores in the order they were issued, so the wait for a
-// knot's input loads also waits for every store issued before them.  Each
-// loop iteration therefore issues its knot's loads first and only then stores
-// the previous knot's last column, which the wait does not cover; the stores
-// before it were issued a column's work earlier and have retired.  At 2^20
-// knots: 156.0-157.0 against 162.0-162.2 us with the last column stored at
-// the end of its own iteration, bitwise the same results.
+// One lane per knot, structure of arrays.  As in the AoS kernel, each loop
+// iteration issues its knot's input loads before any store: here the previous
+// knot's last column, so the wait for the loads covers only stores issued a
+// column's work earlier, which have retired.  At 2^20 knots: 156.0-157.0
+// against 162.0-162.2 us with the last column stored at the end of its own
+// iteration, bitwise the same results.
 template <class P>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_soa_kernel(long n, const float* __restrict__ in, long ld_in,
                                                             float* __restrict__ out, long ld_out, Options opt) {
@@ -126,7 +141,7 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_soa_kernel(lo
   float last[NQ];
   long prev = -1;
   for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
-    InSoARegs i;
+    InRegs i;
     for (int j = 0; j < kInPerKnot; ++j) i.x[j] = in[j * ld_in + k];
     __builtin_amdgcn_sched_barrier(0);
     if (prev >= 0)
