@@ -157,17 +157,20 @@ def test_host_pointer_call_equals_device_call(contexts, n):
         np.testing.assert_array_equal(ctx.fdgrad_host(x), ref)
 
 
-@pytest.mark.parametrize("lanes", [0, 7, 4])
+@pytest.mark.parametrize("lanes", [0, 1, 7, 4])
 @pytest.mark.parametrize("n", [1, 2, 8, 9, 10, 17, 63, 64, 65, 127, 129, 1000])
 def test_ragged_sizes(contexts, oracle, n, lanes):
-    """Sizes that end inside a wave's last knot group (9 knots per wave with 7
-    lanes per knot, 16 with 4; 0: the mapping the context picks)."""
+    """Sizes that end inside a wave's last knot group (64 knots per wave with
+    one lane per knot, 9 with 7 lanes, 16 with 4; 0: the mapping the context
+    picks); with one lane, SoA equals AoS bit for bit."""
     x = common.knots(n, seed=400 + n)
     say(f"ragged n={n} lanes={lanes}")
     ctx = contexts[(0, "fp32", lanes)] if lanes else contexts[(0, "fp32")]
     out = ctx.run_aos(x)
     err = common.max_block_errors(out, oracle.fdgrad(x.astype(np.float64)))
     assert max(err.values()) < common.REL_TOL, err
+    if lanes == 1:
+        np.testing.assert_array_equal(ctx.run_soa(x, pad=3), out)
 
 
 @pytest.mark.parametrize("lanes", [4, 7, 8])
