@@ -1466,6 +1466,15 @@ struct InSoA {
   RBD_HD float qd(int i) const { return p[(NQ + i) * ld]; }
   RBD_HD float u(int i) const { return p[(2 * NQ + i) * ld]; }
 };
+// A knot's 21 inputs already in registers: the one-lane kernels load them
+// before anything else of a loop iteration, ahead of the stores they defer.
+struct InRegs {
+  float x[kInPerKnot];
+  const float* fx = nullptr;
+  RBD_HD float q(int i) const { return x[i]; }
+  RBD_HD float qd(int i) const { return x[NQ + i]; }
+  RBD_HD float u(int i) const { return x[2 * NQ + i]; }
+};
 struct OutSoA {
   float* p; long ld;
   RBD_HD void qdd(int i, float x) const { p[i * ld] = x; }

@@ -52,16 +52,6 @@ using rbdi::kBlock;
 // ---------------------------------------------------------------------------
 namespace rbd {
 
-// A knot's 21 inputs read into registers before anything else of its loop
-// iteration (the one-lane kernels below).
-struct InRegs {
-  float x[kInPerKnot];
-  const float* fx = nullptr;
-  __device__ float q(int i) const { return x[i]; }
-  __device__ float qd(int i) const { return x[NQ + i]; }
-  __device__ float u(int i) const { return x[2 * NQ + i]; }
-};
-
 // One lane per knot, array-of-structures, records staged in LDS.  A wave takes
 // 64 consecutive knots, whose 105-float records are one contiguous block of
 // 26,880 bytes: each lane writes its record into LDS at the record's own
