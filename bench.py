@@ -59,7 +59,7 @@ LARGE = 1 << 20               # knots of the large batch
 # profiles/fdgrad_per_call_n128.md.  Keyed by (kernel, knots per launch);
 # other configurations report null.
 MEASURED_TRAFFIC_BYTES = {
-    ("fdgrad_two_wave_kernel<PrecFP32>", 128): int((181.0 * 2.000 + 53.0 * 1.000) * 1024),
+    ("fdgrad_two_wave_kernel<PrecFP32>", 128): int((176.0 * 2.000 + 53.0 * 1.000) * 1024),
     ("fdgrad_soa_kernel<PrecFP32>", LARGE): int((41326.1 * 2.000 + 430108.3) * 1024),
     ("fdgrad_aos_kernel<PrecFP32>", LARGE): int((43356.5 * 2.000 + 430334.7) * 1024),
 }
