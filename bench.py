@@ -20,8 +20,8 @@ fields: the per-call kernel on one trajectory of 128 knots, the 64-trajectory
 mixed-length batch (direct launch and graph replay), served Poisson request
 streams, the reference harness's per-call latency statistics over its
 trajectory lengths (when fast-rbd-gradients_amd/harness/time_gpu is built).
-The full record of every workload goes to a JSON file (--detail, by default
-gpurun_out/bench_detail.json); stdout gets one JSON line.
+The full record of every workload goes to a JSON file (--detail); stdout gets
+one JSON line.
 
 The CPU leg comes last: the CPU baseline is the oracle's FP32 restatement of
 the reference's multithreaded path (time_CPU.cpp ReusableThreads loop) timed
