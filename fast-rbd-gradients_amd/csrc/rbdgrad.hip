@@ -274,17 +274,42 @@ template <class P, int LANES, bool WAFR = false>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_minv_kernel(long n, const float* __restrict__ in,
                                                              float* __restrict__ out, float* __restrict__ minv,
                                                              Options opt) {
-  const int role = threadIdx.x % LANES;
-  __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
-  const long groups = long(gridDim.x) * (blockDim.x / LANES);
-  for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
-    InAoS i{in + kInPerKnot * k};
-    using O = typename std::conditional<WAFR, OutWafr, OutAoSMinv>::type;
-    O o;
-    if constexpr (WAFR) o = OutWafr{minv ? minv + NQ * k : nullptr, out + kWafrPerKnot * k};
-    else o = OutAoSMinv{out + kOutPerKnot * k, minv + NQ * NQ * k};
-    if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
-    else fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
+  using O = typename std::conditional<WAFR, OutWafr, OutAoSMinv>::type;
+  if constexpr (LANES == 1) {
+    // the wave's 64 consecutive knots: both outputs staged in LDS, each copied
+    // out as one contiguous block (as fdgrad_aos_kernel does); at 2^20 knots
+    // 262 against 385 us (+ Minv) and 262 against 444 us (WAFR layout) with
+    // per-lane stores.  The two blocks (39,424 bytes) leave a CU 4 waves.
+    constexpr int kRec = WAFR ? kWafrPerKnot : kOutPerKnot;
+    constexpr int kAux = WAFR ? NQ : NQ * NQ;     // qdd (WAFR, optional) or Minv per knot
+    __shared__ __align__(16) float rec[64 * kRec];
+    __shared__ __align__(16) float aux[64 * kAux];
+    const int lane = int(threadIdx.x);
+    const long stride = long(gridDim.x) * kBlock;
+    for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
+      const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
+      // lanes past the last knot compute a copy of it; only nk knots are written out
+      InAoS i{in + kInPerKnot * (k0 + (lane < nk ? lane : nk - 1))};
+      O o;
+      if constexpr (WAFR) o = OutWafr{minv ? aux + kAux * lane : nullptr, rec + kRec * lane};
+      else o = OutAoSMinv{rec + kRec * lane, aux + kAux * lane};
+      fdgrad_knot<P>(i, o, opt);
+      wave_lds_sync();
+      copy_record_block<true>(rec, out + k0 * kRec, nk * kRec, lane);
+      if (minv) copy_record_block<true>(aux, minv + k0 * kAux, nk * kAux, lane);
+      wave_lds_sync();                      // read out before the next knots overwrite rec, aux
+    }
+  } else {
+    const int role = threadIdx.x % LANES;
+    __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
+    const long groups = long(gridDim.x) * (blockDim.x / LANES);
+    for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
+      InAoS i{in + kInPerKnot * k};
+      O o;
+      if constexpr (WAFR) o = OutWafr{minv ? minv + NQ * k : nullptr, out + kWafrPerKnot * k};
+      else o = OutAoSMinv{out + kOutPerKnot * k, minv + NQ * NQ * k};
+      fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
+    }
   }
 }
 
@@ -294,14 +319,31 @@ template <class P, int LANES>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_fext_kernel(long n, const float* __restrict__ in,
                                                              const float* __restrict__ fext, long fext_stride,
                                                              float* __restrict__ out, Options opt) {
-  const int role = threadIdx.x % LANES;
-  __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
-  const long groups = long(gridDim.x) * (blockDim.x / LANES);
-  for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
-    InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
-    OutAoS o{out + kOutPerKnot * k};
-    if constexpr (LANES == 1) fdgrad_knot<P>(i, o, opt);
-    else fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
+  if constexpr (LANES == 1) {
+    // records staged in LDS and copied out whole, as fdgrad_aos_kernel does
+    // (2^20 knots: 188 against 273 us with per-lane stores)
+    __shared__ __align__(16) float rec[kRecordBlockFloats];
+    const int lane = int(threadIdx.x);
+    const long stride = long(gridDim.x) * kBlock;
+    for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
+      const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
+      const long k = k0 + (lane < nk ? lane : nk - 1);
+      InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
+      OutAoS o{rec + kOutPerKnot * lane};
+      fdgrad_knot<P>(i, o, opt);
+      wave_lds_sync();
+      copy_record_block<true>(rec, out + k0 * kOutPerKnot, nk * kOutPerKnot, lane);
+      wave_lds_sync();                      // read out before the next knots overwrite rec
+    }
+  } else {
+    const int role = threadIdx.x % LANES;
+    __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
+    const long groups = long(gridDim.x) * (blockDim.x / LANES);
+    for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
+      InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
+      OutAoS o{out + kOutPerKnot * k};
+      fdgrad_knot_lanes<P>(i, o, opt, role, mi_sh[threadIdx.x / kLanesPerKnotGroup]);
+    }
   }
 }
 
@@ -311,12 +353,24 @@ template <class P, bool FEXT>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fd_kernel(long n, const float* __restrict__ in,
                                                     const float* __restrict__ fext, long fext_stride,
                                                     float* __restrict__ qdd, float* __restrict__ minv, Options opt) {
-  const long stride = long(gridDim.x) * blockDim.x;
-  for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
+  // qdd and Minv of the wave's 64 consecutive knots staged in LDS and copied
+  // out as two contiguous blocks (2^20 knots: 63 against 119 us with per-lane
+  // stores)
+  __shared__ __align__(16) float qrec[64 * NQ];
+  __shared__ __align__(16) float mrec[64 * NQ * NQ];
+  const int lane = int(threadIdx.x);
+  const long stride = long(gridDim.x) * kBlock;
+  for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
+    const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
+    const long k = k0 + (lane < nk ? lane : nk - 1);
     InAoS i{in + kInPerKnot * k};
     if constexpr (FEXT) i.fx = fext + fext_stride * k;
-    OutFD o{qdd + NQ * k, minv + NQ * NQ * k};
+    OutFD o{qrec + NQ * lane, mrec + NQ * NQ * lane};
     fd_knot<P>(i, o, opt);
+    wave_lds_sync();
+    copy_record_block<true>(qrec, qdd + NQ * k0, nk * NQ, lane);
+    copy_record_block<true>(mrec, minv + NQ * NQ * k0, nk * NQ * NQ, lane);
+    wave_lds_sync();                        // read out before the next knots overwrite them
   }
 }
 
