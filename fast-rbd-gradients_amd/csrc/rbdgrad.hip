@@ -277,9 +277,10 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_minv_kernel(l
   using O = typename std::conditional<WAFR, OutWafr, OutAoSMinv>::type;
   if constexpr (LANES == 1) {
     // the wave's 64 consecutive knots: both outputs staged in LDS, each copied
-    // out as one contiguous block (as fdgrad_aos_kernel does); at 2^20 knots
-    // 262 against 385 us (+ Minv) and 262 against 444 us (WAFR layout) with
-    // per-lane stores.  The two blocks (39,424 bytes) leave a CU 4 waves.
+    // out as one contiguous block (as fdgrad_aos_kernel does), by one round of
+    // resident waves: at 2^20 knots 252 against 385 us (+ Minv) and 252
+    // against 444 us (WAFR layout) with per-lane stores and a wave per block.
+    // The two blocks (39,424 bytes) leave a CU 4 waves.
     constexpr int kRec = WAFR ? kWafrPerKnot : kOutPerKnot;
     constexpr int kAux = WAFR ? NQ : NQ * NQ;     // qdd (WAFR, optional) or Minv per knot
     __shared__ __align__(16) float rec[64 * kRec];
@@ -321,7 +322,8 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_fext_kernel(l
                                                              float* __restrict__ out, Options opt) {
   if constexpr (LANES == 1) {
     // records staged in LDS and copied out whole, as fdgrad_aos_kernel does
-    // (2^20 knots: 188 against 273 us with per-lane stores)
+    // by one round of resident waves (2^20 knots: 181 against 273 us with
+    // per-lane stores and a wave per block)
     __shared__ __align__(16) float rec[kRecordBlockFloats];
     const int lane = int(threadIdx.x);
     const long stride = long(gridDim.x) * kBlock;
@@ -450,14 +452,15 @@ void launch_lanes(const FlatCall& c, dim3 grid, dim3 block) {
 }
 
 // Workgroups (one wave each) of a one-lane kernel that a CU keeps resident,
-// asked of the runtime once per kernel: 8 for the FP32-column kernels (two
-// waves per SIMD), 6 for the array-of-structures one (its LDS record block).
-template <class P, Layout L, class K>
-hipError_t resident_per_cu(K kernel, int* per_cu) {
+// asked of the runtime once per kernel: 8 for the FP32-column SoA kernel (two
+// waves per SIMD), 6 for the array-of-structures one (its LDS record block),
+// 4 for the kernels that stage two output blocks.
+template <auto Kernel>
+hipError_t resident_per_cu(int* per_cu) {
   static std::atomic<int> cached{0};
   int v = cached.load(std::memory_order_relaxed);
   if (v <= 0) {
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, kBlock, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, Kernel, kBlock, 0);
     if (e != hipSuccess) return e;
     cached.store(v, std::memory_order_relaxed);
   }
@@ -469,13 +472,14 @@ hipError_t resident_per_cu(K kernel, int* per_cu) {
 // runs one round of resident waves, each looping over blocks w, w + G, ...
 // instead of dispatching a new wave per block: at 2^20 knots 157 against 160
 // us (structure of arrays) and 176 against 183 us (array of structures, whose
-// last round of dispatched waves left SIMDs idle).
-template <class P, Layout L, class K>
-hipError_t one_lane_grid(K kernel, const FlatCall& c, dim3* grid) {
-  if (c.simds <= 0) return hipSuccess;
+// last round of dispatched waves left SIMDs idle).  cus: the device's compute
+// units (0: unknown, the grid stays as it is).
+template <auto Kernel>
+hipError_t one_lane_grid(int cus, dim3* grid) {
+  if (cus <= 0) return hipSuccess;
   int per_cu = 0;
-  if (hipError_t e = resident_per_cu<P, L>(kernel, &per_cu)) return e;
-  const unsigned resident = unsigned(per_cu) * unsigned(c.simds / 4);   // 4 SIMDs per CU
+  if (hipError_t e = resident_per_cu<Kernel>(&per_cu)) return e;
+  const unsigned resident = unsigned(per_cu) * unsigned(cus);
   if (resident > 0 && grid->x > resident) grid->x = resident;
   return hipSuccess;
 }
@@ -487,9 +491,9 @@ hipError_t launch_flat_p(const FlatCall& c) {
   if (c.lanes == 1) {
     hipError_t e = hipSuccess;
     switch (c.layout) {
-      case Layout::AoS: e = one_lane_grid<P, Layout::AoS>(rbd::fdgrad_aos_kernel<P>, c, &grid); break;
-      case Layout::SoA: e = one_lane_grid<P, Layout::SoA>(rbd::fdgrad_soa_kernel<P>, c, &grid); break;
-      case Layout::Ref: e = one_lane_grid<P, Layout::Ref>(rbd::fdgrad_refmode_kernel<P>, c, &grid); break;
+      case Layout::AoS: e = one_lane_grid<rbd::fdgrad_aos_kernel<P>>(c.simds / 4, &grid); break;   // 4 SIMDs per CU
+      case Layout::SoA: e = one_lane_grid<rbd::fdgrad_soa_kernel<P>>(c.simds / 4, &grid); break;
+      case Layout::Ref: e = one_lane_grid<rbd::fdgrad_refmode_kernel<P>>(c.simds / 4, &grid); break;
     }
     if (e != hipSuccess) return e;
     switch (c.layout) {
@@ -819,12 +823,16 @@ rbd_status launch_minv(rbd_context* ctx, int64_t n, const float* d_in, float* d_
   const int lanes = lanes_for(ctx, n);
   const dim3 grid(grid_for(n, lanes)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
+  hipError_t ge = hipSuccess;
   auto go = [&](auto prec_tag) {
     using P = decltype(prec_tag);
-    if (lanes == 1)
-      hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, 1, WAFR>), grid, block, 0, st, long(n), d_in, d_out, d_minv,
-                         ctx->opt);
-    else
+    if (lanes == 1) {
+      dim3 g = grid;
+      ge = one_lane_grid<rbd::fdgrad_minv_kernel<P, 1, WAFR>>(ctx->compute_units, &g);
+      if (ge == hipSuccess)
+        hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, 1, WAFR>), g, block, 0, st, long(n), d_in, d_out, d_minv,
+                           ctx->opt);
+    } else
       hipLaunchKernelGGL((rbd::fdgrad_minv_kernel<P, rbd::kLanesPerKnotGroup, WAFR>), grid, block, 0, st, long(n),
                          d_in, d_out, d_minv, ctx->opt);
   };
@@ -835,6 +843,7 @@ rbd_status launch_minv(rbd_context* ctx, int64_t n, const float* d_in, float* d_
     case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP64Col{}); break;
     default: go(rbd::PrecFP64{}); break;
   }
+  if (ge != hipSuccess) return hip_fail(ge, "occupancy query");
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_minv_kernel launch");
 }
@@ -878,6 +887,9 @@ rbd_status forward_dynamics(rbd_context* ctx, int64_t n, const float* d_in, cons
   if (rbd_status s = ready_to_launch(ctx)) return s;
   const dim3 grid(grid_for(n, 1)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
+  // one wave per 64-knot block: a grid of resident waves looping over the
+  // blocks took 77.6 against 63 us at 2^20 knots here (11 resident per CU,
+  // so the blocks do not divide evenly over them)
   auto go = [&](auto prec_tag) {
     using P = decltype(prec_tag);
     if (d_fext)
@@ -923,12 +935,16 @@ rbd_status rbd_fdgrad_traj_fext_f32(rbd_context* ctx, int64_t n, const float* d_
   const int lanes = lanes_for(ctx, n);
   const dim3 grid(grid_for(n, lanes)), block(kBlock);
   const hipStream_t st = (hipStream_t)stream;
+  hipError_t ge = hipSuccess;
   auto go = [&](auto prec_tag) {
     using P = decltype(prec_tag);
-    if (lanes == 1)
-      hipLaunchKernelGGL((rbd::fdgrad_fext_kernel<P, 1>), grid, block, 0, st, long(n), d_in, d_fext,
-                         long(fext_stride), d_out, ctx->opt);
-    else
+    if (lanes == 1) {
+      dim3 g = grid;
+      ge = one_lane_grid<rbd::fdgrad_fext_kernel<P, 1>>(ctx->compute_units, &g);
+      if (ge == hipSuccess)
+        hipLaunchKernelGGL((rbd::fdgrad_fext_kernel<P, 1>), g, block, 0, st, long(n), d_in, d_fext,
+                           long(fext_stride), d_out, ctx->opt);
+    } else
       hipLaunchKernelGGL((rbd::fdgrad_fext_kernel<P, rbd::kLanesPerKnotGroup>), grid, block, 0, st, long(n), d_in,
                          d_fext, long(fext_stride), d_out, ctx->opt);
   };
@@ -939,6 +955,7 @@ rbd_status rbd_fdgrad_traj_fext_f32(rbd_context* ctx, int64_t n, const float* d_
     case RBD_PREC_FP64_COLUMNS: go(rbd::PrecFP64Col{}); break;
     default: go(rbd::PrecFP64{}); break;
   }
+  if (ge != hipSuccess) return hip_fail(ge, "occupancy query");
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? RBD_OK : hip_fail(e, "fdgrad_fext_kernel launch");
 }

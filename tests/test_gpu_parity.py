@@ -336,3 +336,35 @@ def test_hardware_trig_mode_holds_the_bar(contexts, oracle, flags):
         err = {k: float(v.max()) for k, v in e.items()}
         say(f"hw trig flags={flags} lanes={lanes}", err)
         assert max(err.values()) < common.REL_TOL, err
+
+
+def test_secondary_entry_points_beyond_the_resident_waves(rbd):
+    """The one-lane kernels of the other entry points (+ Minv, the WAFR
+    layout, external forces, forward dynamics) stage their outputs in LDS and,
+    for calls of more 64-knot blocks than the device keeps resident, run one
+    round of resident waves that loop over the blocks.  A ragged call of
+    196,645 knots (3,073 blocks, more than any of them keeps resident) equals,
+    bit for bit, the same knots in calls of at most 40,000 knots (fewer blocks
+    than resident waves, one block per wave)."""
+    n = 3 * (1 << 16) + 37
+    x = common.knots(n, seed=307, near_singular_frac=0.1)
+    fext = np.random.default_rng(308).uniform(-20, 20, (n, 42)).astype(np.float32)
+    chunks = [slice(a, min(a + 40000, n)) for a in range(0, n, 40000)]
+    say(f"secondary entry points n={n}")
+    ctx = rbd.Context(precision="fp32", lanes_per_knot=1)
+    whole = ctx.run_with_minv(x)
+    parts = [ctx.run_with_minv(x[c]) for c in chunks]
+    for j in range(2):
+        np.testing.assert_array_equal(whole[j], np.concatenate([p[j] for p in parts]))
+    whole = ctx.run_wafr(x)
+    parts = [ctx.run_wafr(x[c]) for c in chunks]
+    for j in range(2):
+        np.testing.assert_array_equal(whole[j], np.concatenate([p[j] for p in parts]))
+    whole = ctx.run_fext(x, fext)
+    np.testing.assert_array_equal(whole, np.concatenate([ctx.run_fext(x[c], fext[c]) for c in chunks]))
+    whole = ctx.forward_dynamics(x)
+    parts = [ctx.forward_dynamics(x[c]) for c in chunks]
+    for j in range(2):
+        np.testing.assert_array_equal(whole[j], np.concatenate([p[j] for p in parts]))
+    assert np.isfinite(whole[1]).all()
+    ctx.close()
