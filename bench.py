@@ -10,7 +10,9 @@ layout.  Every one of its 2^20 outputs is checked against the FP64 oracle on
 the same FP32 inputs; should the FP32 mode break the 1e-4 bar the headline
 moves to the fastest precision mode that holds it, and says so.  Beside it the
 same knots run in the other precision modes (FP64 pivots, FP64 derivative
-columns, hardware sin/cos) and in the array-of-structures layout.
+columns, hardware sin/cos) and in the array-of-structures layout, and through
+the library's other entry points (the reference-compatible mode, the full path
+with Minv, the WAFR layout, external forces, forward dynamics alone).
 
 Knot points/s is given as a fraction of the FP32 roofline (31,340 algorithmic
 FLOP per knot, the op count of the oracle's restatement of the reference
@@ -124,12 +126,18 @@ def time_launches(launch, steps, warmup, stream):
     return ms
 
 
-def error_stats(o, ref):
+GRADIENT_BLOCKS = (("qdd", slice(0, 7)), ("dqdd_dq", slice(7, 56)), ("dqdd_dqd", slice(56, 105)))
+ORACLE_BLOCKS = {"fdgrad": GRADIENT_BLOCKS,
+                 "fdgrad_minv": GRADIENT_BLOCKS + (("minv", slice(105, 154)),),   # + dqdd/du, column-major
+                 "fd": (("qdd", slice(0, 7)), ("minv", slice(7, 56)))}
+
+
+def error_stats(o, ref, blocks=GRADIENT_BLOCKS):
     """Per output block: the normwise error per knot max|d| / max(max|ref|, 1e-6),
     its max and p99.9 over knots, and the elementwise max of |d| / (|ref| +
     1e-3 max|ref|)."""
     res = {}
-    for name, sl in (("qdd", slice(0, 7)), ("dqdd_dq", slice(7, 56)), ("dqdd_dqd", slice(56, 105))):
+    for name, sl in blocks:
         if sl.stop > ref.shape[1]:
             continue
         d = np.abs(o[:, sl].astype(np.float64) - ref[:, sl])
@@ -151,33 +159,44 @@ class OracleChecks:
     def __init__(self):
         self.items = []
 
-    def add(self, target, key, out, x, sample=None, model=None):
+    def add(self, target, key, out, x, sample=None, model=None, kind="fdgrad", fext=None):
+        """kind: "fdgrad" (105 per knot: qdd, dqdd/dq, dqdd/dqd), "fdgrad_minv"
+        (those and Minv, 154), "fd" (qdd and Minv, 56); Minv column-major.
+        fext: the per-knot external forces (n, 42) the outputs were computed with."""
         if sample is None or sample >= len(x):
-            o, xi = out, x
+            o, xi, fi = out, x, fext
         else:
             idx = np.random.default_rng(5).choice(len(x), sample, replace=False)
-            o, xi = out[idx], x[idx]
+            o, xi, fi = out[idx], x[idx], None if fext is None else fext[idx]
         target[key] = {"pending": "checked against the FP64 oracle in the CPU leg"}
-        self.items.append((target, key, o, xi, model))
+        self.items.append((target, key, o, xi, model, kind, fi))
 
     def run(self, threads):
         from oracle import pyoracle
         refs = {}
-        for target, key, o, x, model in self.items:
-            rk = (id(x), None if model is None else id(model[0]))
+        for target, key, o, x, model, kind, fext in self.items:
+            rk = (id(x), None if model is None else id(model[0]), kind, None if fext is None else id(fext))
             if rk not in refs:
                 if x.shape[1] == rbdgrad.VOL_PER_KNOT:          # reference-compatible mode: 98 out, no qdd
                     refs[rk] = np.concatenate([x[:, 21:28].astype(np.float64),
                                                pyoracle.fdgrad_refmode(x.astype(np.float64))], axis=1)
+                elif kind == "fd":
+                    fd = [pyoracle.fd(k[:7], k[7:14], k[14:21]) for k in x.astype(np.float64)]
+                    refs[rk] = np.array([np.concatenate([qdd, m.T.reshape(49)]) for qdd, m in fd])
+                elif fext is not None:
+                    refs[rk] = pyoracle.fdgrad(x.astype(np.float64), fext=fext.astype(np.float64))
                 elif model is not None:
                     with pyoracle.using_model(*model):
                         refs[rk] = pyoracle.fdgrad(x.astype(np.float64), threads=threads)
                 else:
                     refs[rk] = pyoracle.fdgrad(x.astype(np.float64), threads=threads)
+                if kind == "fdgrad_minv":
+                    minv = [pyoracle.fd(k[:7], k[7:14], k[14:21])[1].T.reshape(49) for k in x.astype(np.float64)]
+                    refs[rk] = np.concatenate([refs[rk], np.array(minv)], axis=1)
             ref = refs[rk]
             if x.shape[1] == rbdgrad.VOL_PER_KNOT:
                 o = np.concatenate([x[:, 21:28], o], axis=1)
-            res = error_stats(o, ref)
+            res = error_stats(o, ref, ORACLE_BLOCKS[kind])
             res["knots_checked"] = int(len(x))
             target[key] = res
         self.items = []
@@ -246,6 +265,71 @@ def large_batch(stream, steps, warmup):
             f"FP32 frac {r['fp32_frac']:.3f}")
         ctx.close()
     return legs, x4
+
+
+# The library's other entry points, each a one-lane kernel over the same knots:
+# (name, floats read per knot, floats written per knot).  External forces are
+# one set of 42 per knot.
+OTHER_ENTRIES = (("reference_mode", 77, 98), ("with_minv", 21, 105 + 49), ("wafr_layout", 21, 7 + 147),
+                 ("external_forces", 21 + 42, 105), ("forward_dynamics", 21, 7 + 49))
+ORACLE_SAMPLE = 4096   # knots of each of these outputs checked against the FP64 oracle
+
+
+def other_entries(stream, steps, warmup, x):
+    """The other entry points on the large batch's knots, on device-resident
+    buffers, one launch per step: the reference-compatible mode (q, qd, u,
+    qdd and Minv in, 98 out), the full path with dqdd/du = Minv, the WAFR
+    output layout, external forces, and forward dynamics alone.  Each is
+    timed as the headline is and priced against HBM by the floats it reads
+    and writes; a sample of its outputs goes to the oracle check."""
+    n = len(x)
+    ctx = rbdgrad.Context(device=0, precision="fp32", lanes_per_knot=1)
+    d_in = rbdgrad.DeviceBuffer.from_array(x)
+    d_qdd = rbdgrad.DeviceBuffer(n * 7 * 4)
+    d_minv = rbdgrad.DeviceBuffer(n * 49 * 4)
+    ctx.forward_dynamics_dev(n, d_in, d_qdd, d_minv, stream)
+    rbdgrad.hip().hipDeviceSynchronize()
+    qdd, minv = d_qdd.download((n, 7)), d_minv.download((n, 49))
+    vol = np.ascontiguousarray(np.concatenate([x, qdd, minv], axis=1), dtype=np.float32)   # Minv column-major
+    fext = np.random.default_rng(105).uniform(-20.0, 20.0, (n, 42)).astype(np.float32)
+    d_vol = rbdgrad.DeviceBuffer.from_array(vol)
+    d_fext = rbdgrad.DeviceBuffer.from_array(fext)
+    d_out = rbdgrad.DeviceBuffer(n * 147 * 4)
+    launches = {
+        "reference_mode": lambda: ctx.dynamics_gradient_kernel(n, d_out, d_vol, stream=stream),
+        "with_minv": lambda: ctx.fdgrad_minv(n, d_in, d_out, d_minv, stream),
+        "wafr_layout": lambda: ctx.fdgrad_wafr(n, d_in, d_qdd, d_out, stream),
+        "external_forces": lambda: ctx.fdgrad_fext(n, d_in, d_fext, 42, d_out, stream),
+        "forward_dynamics": lambda: ctx.forward_dynamics_dev(n, d_in, d_qdd, d_minv, stream),
+    }
+    res = {"knots": n, "oracle_sample": ORACLE_SAMPLE}
+    for name, fin, fout in OTHER_ENTRIES:
+        ms = time_launches(launches[name], steps, warmup, stream)
+        per = ms / steps
+        gbs = n * (fin + fout) * 4 / (per * 1e-3) / 1e9
+        r = res[name] = {"us_per_launch": per * 1e3, "knots_per_s": n / (per * 1e-3),
+                         "bytes_per_knot": (fin + fout) * 4, "hbm_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+        # outputs of the last launch, in the oracle's layouts
+        if name == "reference_mode":
+            CHECKS.add(r, "accuracy", d_out.download((n, rbdgrad.DQDD_PER_KNOT)), vol, sample=ORACLE_SAMPLE)
+        elif name == "with_minv":
+            o = np.concatenate([d_out.download((n, rbdgrad.OUT_PER_KNOT)), d_minv.download((n, 49))], axis=1)
+            CHECKS.add(r, "accuracy", o, x, sample=ORACLE_SAMPLE, kind="fdgrad_minv")
+        elif name == "wafr_layout":
+            o = np.concatenate([d_qdd.download((n, 7)), d_out.download((n, 147))], axis=1)
+            CHECKS.add(r, "accuracy", o, x, sample=ORACLE_SAMPLE, kind="fdgrad_minv")
+        elif name == "external_forces":
+            CHECKS.add(r, "accuracy", d_out.download((n, rbdgrad.OUT_PER_KNOT)), x, sample=ORACLE_SAMPLE,
+                       fext=fext)
+        else:
+            o = np.concatenate([d_qdd.download((n, 7)), d_minv.download((n, 49))], axis=1)
+            CHECKS.add(r, "accuracy", o, x, sample=ORACLE_SAMPLE, kind="fd")
+        log(f"[other entries] {name}: {per * 1e3:.1f} us/launch, {r['knots_per_s']:.3e} knots/s, "
+            f"HBM frac {r['hbm_frac']:.3f}")
+    for b in (d_in, d_qdd, d_minv, d_vol, d_fext, d_out):
+        b.free()
+    ctx.close()
+    return res
 
 
 def choose_headline(legs):
@@ -534,6 +618,9 @@ def profile_one(args, stream):
         ctx = rbdgrad.Context(device=0, precision=args.precision, model=args.model)
         log(json.dumps(mixed_batch(ctx, stream, steps=args.steps)))
         return
+    if args.only == "c4-other":
+        other_entries(stream, args.steps, min(args.warmup, 10), knots(LARGE, seed=104, near_singular_frac=0.1))
+        return
     if args.only == "c3":
         ctx = rbdgrad.Context(device=0, precision=args.precision, model=args.model)
         mode = {16: "auto", 32: "auto_two_wave"}[args.server_lanes]
@@ -555,7 +642,7 @@ def profile_one(args, stream):
 def compact(line, limit=4000):
     """Keep the printed line under `limit` characters: secondary fields go
     first (they are all in the detail file)."""
-    for k in ("c4_modes", "latency_us", "c3_served", "c1_per_call_n128", "c2_mixed_batch"):
+    for k in ("other_entries_2p20", "c4_modes", "latency_us", "c3_served", "c1_per_call_n128", "c2_mixed_batch"):
         if len(json.dumps(line)) <= limit:
             break
         line.pop(k, None)
@@ -573,7 +660,8 @@ def main():
     ap.add_argument("--full", action="store_true", help="also every serving mode, at more rates")
     ap.add_argument("--detail", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
                     help="where the full record of every workload is written")
-    ap.add_argument("--only", choices=["c1", "c1-one-lane", "c1-lane-group", "c4-aos", "c4-soa", "c2", "c3", "copy"],
+    ap.add_argument("--only", choices=["c1", "c1-one-lane", "c1-lane-group", "c4-aos", "c4-soa", "c4-other", "c2", "c3",
+                                       "copy"],
                     help="run one workload only and print its timing (for profiler runs)")
     ap.add_argument("--precision", default="fp32", choices=list(rbdgrad.PRECISIONS), help="with --only")
     ap.add_argument("--rate", type=int, default=0, help="with --only c3: requests/s")
@@ -599,6 +687,7 @@ def main():
     # ---- headline: the large batch, every precision mode ----
     legs, x4 = large_batch(stream, args.steps, args.warmup)
     detail["large_batch_2p20"] = legs
+    detail["other_entries_2p20"] = other_entries(stream, args.steps, args.warmup, x4)
     # ---- secondary workloads ----
     ctx = rbdgrad.Context(device=0, precision="fp32")
     thr, tune = ctx.autotune(stream)
@@ -682,6 +771,10 @@ def main():
                          "max_err": max((r["accuracy"][b]["max"] for b in ("qdd", "dqdd_dq", "dqdd_dqd")),
                                         default=None) if "qdd" in r.get("accuracy", {}) else None}
                      for k, r in legs.items()},
+        "other_entries_2p20": {k: {"us": round(r["us_per_launch"], 1), "hbm_frac": round(r["hbm_frac"], 3),
+                                   "max_err": max((v["max"] for v in r["accuracy"].values() if isinstance(v, dict)),
+                                                  default=None)}
+                               for k, r in detail["other_entries_2p20"].items() if isinstance(r, dict)},
         "c1_per_call_n128": {"us": round(pc["auto"]["us_per_launch"], 2), "knots_per_s": pc["auto"]["knots_per_s"],
                              "kernel": pc["auto"]["kernel"]},
         "c2_mixed_batch": {"knots": mixed["knots"], "us_direct": round(mixed["direct"]["us_per_batch"], 2),

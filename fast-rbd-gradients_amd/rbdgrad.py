@@ -396,6 +396,22 @@ class Context(_Bound):
                                                       d_const.handle if d_const is not None else None,
                                                       _sh(stream)), "rbd_dynamics_gradient_kernel_f32")
 
+    def fdgrad_minv(self, n, d_in, d_out, d_minv, stream=None):
+        self._check(self._L.rbd_fdgrad_minv_traj_f32(self._h, int(n), d_in.handle, d_out.handle, d_minv.handle,
+                                                     _sh(stream)), "rbd_fdgrad_minv_traj_f32")
+
+    def fdgrad_wafr(self, n, d_in, d_qdd, d_dqdd, stream=None):
+        self._check(self._L.rbd_fdgrad_wafr_f32(self._h, int(n), d_in.handle, d_qdd.handle if d_qdd else None,
+                                                d_dqdd.handle, _sh(stream)), "rbd_fdgrad_wafr_f32")
+
+    def fdgrad_fext(self, n, d_in, d_fext, fext_stride, d_out, stream=None):
+        self._check(self._L.rbd_fdgrad_traj_fext_f32(self._h, int(n), d_in.handle, d_fext.handle, int(fext_stride),
+                                                     d_out.handle, _sh(stream)), "rbd_fdgrad_traj_fext_f32")
+
+    def forward_dynamics_dev(self, n, d_in, d_qdd, d_minv, stream=None):
+        self._check(self._L.rbd_forward_dynamics_f32(self._h, int(n), d_in.handle, d_qdd.handle, d_minv.handle,
+                                                     _sh(stream)), "rbd_forward_dynamics_f32")
+
     def status(self):
         """rbd_context_status: 0, or ERR_DEVICE_TIMEOUT once a two-wave kernel
         of this context gave up a handoff wait (synchronise first)."""
@@ -511,8 +527,7 @@ class Context(_Bound):
         d_in = DeviceBuffer.from_array(x)
         d_out = DeviceBuffer(n * OUT_PER_KNOT * 4)
         d_m = DeviceBuffer(n * 49 * 4)
-        self._check(self._L.rbd_fdgrad_minv_traj_f32(self._h, n, d_in.handle, d_out.handle, d_m.handle, None),
-               "rbd_fdgrad_minv_traj_f32")
+        self.fdgrad_minv(n, d_in, d_out, d_m)
         return d_out.download((n, OUT_PER_KNOT)), d_m.download((n, 7, 7)).transpose(0, 2, 1).copy()
 
     def run_wafr(self, x, with_qdd=True):
@@ -524,8 +539,7 @@ class Context(_Bound):
         d_in = DeviceBuffer.from_array(x)
         d_q = DeviceBuffer(n * 7 * 4) if with_qdd else None
         d_d = DeviceBuffer(n * 147 * 4)
-        self._check(self._L.rbd_fdgrad_wafr_f32(self._h, n, d_in.handle, d_q.handle if d_q else None, d_d.handle,
-                                                None), "rbd_fdgrad_wafr_f32")
+        self.fdgrad_wafr(n, d_in, d_q, d_d)
         return (d_q.download((n, 7)) if d_q else None), d_d.download((n, 147))
 
     def forward_dynamics(self, x, fext=None):
@@ -537,8 +551,7 @@ class Context(_Bound):
         d_q = DeviceBuffer(n * 7 * 4)
         d_m = DeviceBuffer(n * 49 * 4)
         if fext is None:
-            self._check(self._L.rbd_forward_dynamics_f32(self._h, n, d_in.handle, d_q.handle, d_m.handle, None),
-                   "rbd_forward_dynamics_f32")
+            self.forward_dynamics_dev(n, d_in, d_q, d_m)
         else:
             d_f, stride = _fext_buffer(fext, n)
             self._check(self._L.rbd_forward_dynamics_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_q.handle,
@@ -552,8 +565,7 @@ class Context(_Bound):
         d_in = DeviceBuffer.from_array(x)
         d_out = DeviceBuffer(n * OUT_PER_KNOT * 4)
         d_f, stride = _fext_buffer(fext, n)
-        self._check(self._L.rbd_fdgrad_traj_fext_f32(self._h, n, d_in.handle, d_f.handle, stride, d_out.handle, None),
-               "rbd_fdgrad_traj_fext_f32")
+        self.fdgrad_fext(n, d_in, d_f, stride, d_out)
         return d_out.download((n, OUT_PER_KNOT))
 
     def id_gradient(self, vol):
