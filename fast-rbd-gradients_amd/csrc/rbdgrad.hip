@@ -148,24 +148,59 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_soa_kernel(lo
 // Reference-compatible mode (qdd and Minv given), one lane per knot: a wave's
 // 64 consecutive knots write one contiguous block of 64 x 98 floats, staged in
 // LDS and copied out whole as in the AoS kernel above: 215.6 us at 2^20
-// knots, against 381 us with per-lane stores of the 98-float records.
+// knots, against 381 us with per-lane stores of the 98-float records.  As
+// there, each iteration issues its block's input loads (the 49 of a record's
+// 77 floats the gradient reads: q, qd, qdd and Minv's upper triangle) before
+// it copies out the previous block's records: 200 against 216-218 us, with
+// the same FP instruction mix and bitwise the same results.
 constexpr int kDqddBlockFloats = 64 * kDqddPerKnot;
+struct InVolRegs {
+  static constexpr int kUpper = NQ * (NQ + 1) / 2;
+  float x[3 * NQ + kUpper];
+  const float* fx = nullptr;
+  // row r of the upper triangle starts at r * NQ - r * (r - 1) / 2
+  static constexpr int upper(int r, int c) { return 3 * NQ + r * NQ - r * (r - 1) / 2 + (c - r); }
+  __device__ void load(const float* p) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      x[k] = p[k];
+      x[NQ + k] = p[NQ + k];
+      x[2 * NQ + k] = p[3 * NQ + k];
+    }
+#pragma unroll
+    for (int r = 0; r < NQ; ++r)
+#pragma unroll
+      for (int c = r; c < NQ; ++c) x[upper(r, c)] = p[4 * NQ + r + NQ * c];
+  }
+  __device__ float q(int i) const { return x[i]; }
+  __device__ float qd(int i) const { return x[NQ + i]; }
+  __device__ float qdd(int i) const { return x[2 * NQ + i]; }
+  __device__ float minv(int r, int c) const { return x[upper(r, c)]; }   // r <= c: only the upper triangle is read
+};
+
 template <class P>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_refmode_kernel(long n, const float* __restrict__ vol,
                                                                 float* __restrict__ dqdd, Options opt) {
   __shared__ __align__(16) float rec[kDqddBlockFloats];
   const int lane = int(threadIdx.x);
   const long stride = long(gridDim.x) * kBlock;
+  long prev = 0;          // first knot of the block whose records rec holds
+  int prev_words = 0;     // and their words (0: none yet)
   for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
     const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
     // lanes past the last knot compute a copy of it; only nk records are written out
-    InAoS i{vol + kVolPerKnot * (k0 + (lane < nk ? lane : nk - 1))};
+    InVolRegs i;
+    i.load(vol + kVolPerKnot * (k0 + (lane < nk ? lane : nk - 1)));
+    __builtin_amdgcn_sched_barrier(0);
+    copy_record_block<true>(rec, dqdd + prev * kDqddPerKnot, prev_words, lane);
+    __builtin_amdgcn_sched_barrier(0);
     OutDqddAoS o{rec + kDqddPerKnot * lane};
     fdgrad_knot_given_qdd_minv<P>(i, o, opt);
     wave_lds_sync();
-    copy_record_block<true>(rec, dqdd + k0 * kDqddPerKnot, nk * kDqddPerKnot, lane);
-    wave_lds_sync();                        // read out before the next knots overwrite rec
+    prev = k0;
+    prev_words = nk * kDqddPerKnot;
   }
+  copy_record_block<true>(rec, dqdd + prev * kDqddPerKnot, prev_words, lane);
 }
 
 // One knot per group of kLanesPerKnotGroup lanes (4 knots per wave): the
