@@ -320,21 +320,33 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_minv_kernel(l
     constexpr int kAux = WAFR ? NQ : NQ * NQ;     // qdd (WAFR, optional) or Minv per knot
     __shared__ __align__(16) float rec[64 * kRec];
     __shared__ __align__(16) float aux[64 * kAux];
+    // Each iteration issues its block's input loads before it copies out the
+    // previous block's records, as fdgrad_aos_kernel does: 236 against 252 us
+    // for either output, with bitwise the same results.
     const int lane = int(threadIdx.x);
     const long stride = long(gridDim.x) * kBlock;
+    long prev = 0;          // first knot of the block whose records rec, aux hold
+    int prev_nk = 0;        // and their knots (0: none yet)
     for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
       const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
       // lanes past the last knot compute a copy of it; only nk knots are written out
-      InAoS i{in + kInPerKnot * (k0 + (lane < nk ? lane : nk - 1))};
+      const float* src = in + kInPerKnot * (k0 + (lane < nk ? lane : nk - 1));
+      InRegs i;
+      for (int j = 0; j < kInPerKnot; ++j) i.x[j] = src[j];
+      __builtin_amdgcn_sched_barrier(0);
+      copy_record_block<true>(rec, out + prev * kRec, prev_nk * kRec, lane);
+      if (minv) copy_record_block<true>(aux, minv + prev * kAux, prev_nk * kAux, lane);
+      __builtin_amdgcn_sched_barrier(0);
       O o;
       if constexpr (WAFR) o = OutWafr{minv ? aux + kAux * lane : nullptr, rec + kRec * lane};
       else o = OutAoSMinv{rec + kRec * lane, aux + kAux * lane};
       fdgrad_knot<P>(i, o, opt);
       wave_lds_sync();
-      copy_record_block<true>(rec, out + k0 * kRec, nk * kRec, lane);
-      if (minv) copy_record_block<true>(aux, minv + k0 * kAux, nk * kAux, lane);
-      wave_lds_sync();                      // read out before the next knots overwrite rec, aux
+      prev = k0;
+      prev_nk = nk;
     }
+    copy_record_block<true>(rec, out + prev * kRec, prev_nk * kRec, lane);
+    if (minv) copy_record_block<true>(aux, minv + prev * kAux, prev_nk * kAux, lane);
   } else {
     const int role = threadIdx.x % LANES;
     __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
@@ -359,19 +371,31 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fdgrad_fext_kernel(l
     // records staged in LDS and copied out whole, as fdgrad_aos_kernel does
     // by one round of resident waves (2^20 knots: 181 against 273 us with
     // per-lane stores and a wave per block)
+    // (q, qd, u issued before the previous block's copy, as fdgrad_aos_kernel
+    // does; the forces are read where the RNEA needs them): 186 against 191 us
+    // with one set of forces per knot, bitwise the same results
     __shared__ __align__(16) float rec[kRecordBlockFloats];
     const int lane = int(threadIdx.x);
     const long stride = long(gridDim.x) * kBlock;
+    long prev = 0;          // first knot of the block whose records rec holds
+    int prev_words = 0;     // and their words (0: none yet)
     for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
       const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
       const long k = k0 + (lane < nk ? lane : nk - 1);
-      InAoS i{in + kInPerKnot * k, fext + fext_stride * k};
+      const float* src = in + kInPerKnot * k;
+      InRegs i;
+      for (int j = 0; j < kInPerKnot; ++j) i.x[j] = src[j];
+      i.fx = fext + fext_stride * k;
+      __builtin_amdgcn_sched_barrier(0);
+      copy_record_block<true>(rec, out + prev * kOutPerKnot, prev_words, lane);
+      __builtin_amdgcn_sched_barrier(0);
       OutAoS o{rec + kOutPerKnot * lane};
       fdgrad_knot<P>(i, o, opt);
       wave_lds_sync();
-      copy_record_block<true>(rec, out + k0 * kOutPerKnot, nk * kOutPerKnot, lane);
-      wave_lds_sync();                      // read out before the next knots overwrite rec
+      prev = k0;
+      prev_words = nk * kOutPerKnot;
     }
+    copy_record_block<true>(rec, out + prev * kOutPerKnot, prev_words, lane);
   } else {
     const int role = threadIdx.x % LANES;
     __shared__ typename P::Minv mi_sh[kBlock / kLanesPerKnotGroup][kMinvPacked];   // Minv per lane group
