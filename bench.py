@@ -129,7 +129,10 @@ def time_launches(launch, steps, warmup, stream):
 GRADIENT_BLOCKS = (("qdd", slice(0, 7)), ("dqdd_dq", slice(7, 56)), ("dqdd_dqd", slice(56, 105)))
 ORACLE_BLOCKS = {"fdgrad": GRADIENT_BLOCKS,
                  "fdgrad_minv": GRADIENT_BLOCKS + (("minv", slice(105, 154)),),   # + dqdd/du, column-major
-                 "fd": (("qdd", slice(0, 7)), ("minv", slice(7, 56)))}
+                 "fd": (("qdd", slice(0, 7)), ("minv", slice(7, 56))),
+                 "cdu": (("dtau_dq", slice(0, 49)), ("dtau_dqd", slice(49, 98))),
+                 "dqdd": (("dqdd_dq", slice(0, 49)), ("dqdd_dqd", slice(49, 98))),
+                 "vaf": (("v", slice(28, 70)), ("a", slice(70, 112)), ("f", slice(112, 154)))}
 
 
 def error_stats(o, ref, blocks=GRADIENT_BLOCKS):
@@ -161,7 +164,9 @@ class OracleChecks:
 
     def add(self, target, key, out, x, sample=None, model=None, kind="fdgrad", fext=None):
         """kind: "fdgrad" (105 per knot: qdd, dqdd/dq, dqdd/dqd), "fdgrad_minv"
-        (those and Minv, 154), "fd" (qdd and Minv, 56); Minv column-major.
+        (those and Minv, 154), "fd" (qdd and Minv, 56), "dqdd" (98, no qdd),
+        "cdu" (the inverse-dynamics gradient, 98, of x = q, qd, u, qdd), "vaf"
+        (the split layout, 154, of the same x); Minv column-major.
         fext: the per-knot external forces (n, 42) the outputs were computed with."""
         if sample is None or sample >= len(x):
             o, xi, fi = out, x, fext
@@ -180,6 +185,12 @@ class OracleChecks:
                 if x.shape[1] == rbdgrad.VOL_PER_KNOT:          # reference-compatible mode: 98 out, no qdd
                     refs[rk] = np.concatenate([x[:, 21:28].astype(np.float64),
                                                pyoracle.fdgrad_refmode(x.astype(np.float64))], axis=1)
+                elif kind == "cdu":                                 # x: q, qd, u, qdd (28 per knot)
+                    refs[rk] = pyoracle.id_gradient(x.astype(np.float64))
+                elif kind == "vaf":
+                    refs[rk] = pyoracle.vaf(np.concatenate([x.astype(np.float64), np.zeros((len(x), 126))], axis=1))
+                elif kind == "dqdd":                                # x: q, qd, u; the gradient without qdd
+                    refs[rk] = pyoracle.fdgrad(x.astype(np.float64), threads=threads)[:, 7:]
                 elif kind == "fd":
                     fd = [pyoracle.fd(k[:7], k[7:14], k[14:21]) for k in x.astype(np.float64)]
                     refs[rk] = np.array([np.concatenate([qdd, m.T.reshape(49)]) for qdd, m in fd])
@@ -271,7 +282,12 @@ def large_batch(stream, steps, warmup):
 # (name, floats read per knot, floats written per knot).  External forces are
 # one set of 42 per knot.
 OTHER_ENTRIES = (("reference_mode", 77, 98), ("with_minv", 21, 105 + 49), ("wafr_layout", 21, 7 + 147),
-                 ("external_forces", 21 + 42, 105), ("forward_dynamics", 21, 7 + 49))
+                 ("external_forces", 21 + 42, 105), ("forward_dynamics", 21, 7 + 49),
+                 # the reference's staged variants: inverse-dynamics gradient from the fused
+                 # (q, qd, u, qdd) and split (+ v, a, f) layouts, v, a, f written into the
+                 # split layout, and the finish stage -Minv cdu
+                 ("fused_cdu", 28, 98), ("split_cdu", 154, 98), ("setup_split", 28, 126),
+                 ("finish", 49 + 98, 98))
 ORACLE_SAMPLE = 4096   # knots of each of these outputs checked against the FP64 oracle
 
 
@@ -295,15 +311,29 @@ def other_entries(stream, steps, warmup, x):
     d_vol = rbdgrad.DeviceBuffer.from_array(vol)
     d_fext = rbdgrad.DeviceBuffer.from_array(fext)
     d_out = rbdgrad.DeviceBuffer(n * 147 * 4)
+    vol28 = np.ascontiguousarray(vol[:, :28])
+    d_vol28 = rbdgrad.DeviceBuffer.from_array(vol28)
+    split = np.zeros((n, 154), np.float32)
+    split[:, :28] = vol28
+    d_split = rbdgrad.DeviceBuffer.from_array(split)
+    d_cdu = rbdgrad.DeviceBuffer(n * 98 * 4)
     launches = {
         "reference_mode": lambda: ctx.dynamics_gradient_kernel(n, d_out, d_vol, stream=stream),
         "with_minv": lambda: ctx.fdgrad_minv(n, d_in, d_out, d_minv, stream),
         "wafr_layout": lambda: ctx.fdgrad_wafr(n, d_in, d_qdd, d_out, stream),
         "external_forces": lambda: ctx.fdgrad_fext(n, d_in, d_fext, 42, d_out, stream),
         "forward_dynamics": lambda: ctx.forward_dynamics_dev(n, d_in, d_qdd, d_minv, stream),
+        "fused_cdu": lambda: ctx.dynamics_gradient_kernel_fused(n, d_cdu, d_vol28, stream),
+        "setup_split": lambda: ctx.gradient_setup_split_dev(n, d_split, stream),
+        "split_cdu": lambda: ctx.dynamics_gradient_kernel_split(n, d_cdu, d_split, stream),
+        "finish": lambda: ctx.gradient_finish_dev(n, d_out, d_cdu, d_minv, stream),
     }
+    order = ("reference_mode", "with_minv", "wafr_layout", "external_forces", "forward_dynamics", "fused_cdu",
+             "setup_split", "split_cdu", "finish")     # setup before split; forward dynamics leaves Minv for finish
+    sizes = {name: (fin, fout) for name, fin, fout in OTHER_ENTRIES}
     res = {"knots": n, "oracle_sample": ORACLE_SAMPLE}
-    for name, fin, fout in OTHER_ENTRIES:
+    for name in order:
+        fin, fout = sizes[name]
         ms = time_launches(launches[name], steps, warmup, stream)
         per = ms / steps
         gbs = n * (fin + fout) * 4 / (per * 1e-3) / 1e9
@@ -321,12 +351,18 @@ def other_entries(stream, steps, warmup, x):
         elif name == "external_forces":
             CHECKS.add(r, "accuracy", d_out.download((n, rbdgrad.OUT_PER_KNOT)), x, sample=ORACLE_SAMPLE,
                        fext=fext)
-        else:
+        elif name == "forward_dynamics":
             o = np.concatenate([d_qdd.download((n, 7)), d_minv.download((n, 49))], axis=1)
             CHECKS.add(r, "accuracy", o, x, sample=ORACLE_SAMPLE, kind="fd")
+        elif name in ("fused_cdu", "split_cdu"):
+            CHECKS.add(r, "accuracy", d_cdu.download((n, 98)), vol28, sample=ORACLE_SAMPLE, kind="cdu")
+        elif name == "setup_split":
+            CHECKS.add(r, "accuracy", d_split.download((n, 154)), vol28, sample=ORACLE_SAMPLE, kind="vaf")
+        else:
+            CHECKS.add(r, "accuracy", d_out.download((n, 98)), x, sample=ORACLE_SAMPLE, kind="dqdd")
         log(f"[other entries] {name}: {per * 1e3:.1f} us/launch, {r['knots_per_s']:.3e} knots/s, "
             f"HBM frac {r['hbm_frac']:.3f}")
-    for b in (d_in, d_qdd, d_minv, d_vol, d_fext, d_out):
+    for b in (d_in, d_qdd, d_minv, d_vol, d_fext, d_out, d_vol28, d_split, d_cdu):
         b.free()
     ctx.close()
     return res
@@ -772,8 +808,8 @@ def main():
                                         default=None) if "qdd" in r.get("accuracy", {}) else None}
                      for k, r in legs.items()},
         "other_entries_2p20": {k: {"us": round(r["us_per_launch"], 1), "hbm_frac": round(r["hbm_frac"], 3),
-                                   "max_err": max((v["max"] for v in r["accuracy"].values() if isinstance(v, dict)),
-                                                  default=None)}
+                                   "max_err": max((v["max"] for v in r.get("accuracy", {}).values()
+                                                   if isinstance(v, dict)), default=None)}
                                for k, r in detail["other_entries_2p20"].items() if isinstance(r, dict)},
         "c1_per_call_n128": {"us": round(pc["auto"]["us_per_launch"], 2), "knots_per_s": pc["auto"]["knots_per_s"],
                              "kernel": pc["auto"]["kernel"]},

@@ -469,14 +469,24 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void vaf_kernel(long n, f
   }
 }
 
-// dqdd = -Minv cdu per knot.  One lane per knot.
+// dqdd = -Minv cdu per knot.  One lane per knot; the wave's 64 consecutive
+// knots' results are staged in LDS and copied out as one contiguous block of
+// 64 x 98 floats, as fdgrad_refmode_kernel does.
 __global__ __launch_bounds__(kBlock) void finish_kernel(long n, const float* __restrict__ minv,
                                                         const float* __restrict__ cdu, float* __restrict__ dqdd) {
-  const long stride = long(gridDim.x) * blockDim.x;
-  for (long k = long(blockIdx.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
+  __shared__ __align__(16) float rec[kDqddBlockFloats];
+  const int lane = int(threadIdx.x);
+  const long stride = long(gridDim.x) * kBlock;
+  for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
+    const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
+    // lanes past the last knot compute a copy of it; only nk records are written out
+    const long k = k0 + (lane < nk ? lane : nk - 1);
     InFinish i{minv + NQ * NQ * k, cdu + kDqddPerKnot * k};
-    OutFinish o{dqdd + kDqddPerKnot * k};
+    OutFinish o{rec + kDqddPerKnot * lane};
     finish_knot(i, o);
+    wave_lds_sync();
+    copy_record_block<true>(rec, dqdd + k0 * kDqddPerKnot, nk * kDqddPerKnot, lane);
+    wave_lds_sync();                        // read out before the next knots overwrite rec
   }
 }
 

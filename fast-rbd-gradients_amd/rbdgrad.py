@@ -408,6 +408,24 @@ class Context(_Bound):
         self._check(self._L.rbd_fdgrad_traj_fext_f32(self._h, int(n), d_in.handle, d_fext.handle, int(fext_stride),
                                                      d_out.handle, _sh(stream)), "rbd_fdgrad_traj_fext_f32")
 
+    def dynamics_gradient_kernel_fused(self, n, d_cdu, d_vol, stream=None):
+        self._check(self._L.rbd_dynamics_gradient_kernel_fused_f32(self._h, int(n), d_cdu.handle, d_vol.handle, None,
+                                                                   _sh(stream)),
+                    "rbd_dynamics_gradient_kernel_fused_f32")
+
+    def dynamics_gradient_kernel_split(self, n, d_cdu, d_vol, stream=None):
+        self._check(self._L.rbd_dynamics_gradient_kernel_split_f32(self._h, int(n), d_cdu.handle, d_vol.handle, None,
+                                                                   _sh(stream)),
+                    "rbd_dynamics_gradient_kernel_split_f32")
+
+    def gradient_setup_split_dev(self, n, d_vol, stream=None):
+        self._check(self._L.rbd_gradient_setup_split_f32(self._h, int(n), d_vol.handle, _sh(stream)),
+                    "rbd_gradient_setup_split_f32")
+
+    def gradient_finish_dev(self, n, d_dqdd, d_cdu, d_minv, stream=None):
+        self._check(self._L.rbd_gradient_finish_f32(self._h, int(n), d_dqdd.handle, d_cdu.handle, d_minv.handle,
+                                                    _sh(stream)), "rbd_gradient_finish_f32")
+
     def forward_dynamics_dev(self, n, d_in, d_qdd, d_minv, stream=None):
         self._check(self._L.rbd_forward_dynamics_f32(self._h, int(n), d_in.handle, d_qdd.handle, d_minv.handle,
                                                      _sh(stream)), "rbd_forward_dynamics_f32")
@@ -574,8 +592,7 @@ class Context(_Bound):
         n = vol.shape[0]
         d_vol = DeviceBuffer.from_array(vol)
         d_cdu = DeviceBuffer(n * DQDD_PER_KNOT * 4)
-        self._check(self._L.rbd_dynamics_gradient_kernel_fused_f32(self._h, n, d_cdu.handle, d_vol.handle, None, None),
-               "rbd_dynamics_gradient_kernel_fused_f32")
+        self.dynamics_gradient_kernel_fused(n, d_cdu, d_vol)
         return d_cdu.download((n, DQDD_PER_KNOT))
 
     def id_gradient_split(self, vol):
@@ -584,8 +601,7 @@ class Context(_Bound):
         n = vol.shape[0]
         d_vol = DeviceBuffer.from_array(vol)
         d_cdu = DeviceBuffer(n * DQDD_PER_KNOT * 4)
-        self._check(self._L.rbd_dynamics_gradient_kernel_split_f32(self._h, n, d_cdu.handle, d_vol.handle, None, None),
-               "rbd_dynamics_gradient_kernel_split_f32")
+        self.dynamics_gradient_kernel_split(n, d_cdu, d_vol)
         return d_cdu.download((n, DQDD_PER_KNOT))
 
     def gradient_setup_split(self, vol28):
@@ -595,7 +611,7 @@ class Context(_Bound):
         vol = np.zeros((n, 154), np.float32)
         vol[:, :28] = vol28
         d_vol = DeviceBuffer.from_array(vol)
-        self._check(self._L.rbd_gradient_setup_split_f32(self._h, n, d_vol.handle, None), "rbd_gradient_setup_split_f32")
+        self.gradient_setup_split_dev(n, d_vol)
         return d_vol.download((n, 154))
 
     def gradient_finish(self, minv, cdu):
@@ -605,8 +621,7 @@ class Context(_Bound):
         n = cdu.shape[0]
         d_m, d_c = DeviceBuffer.from_array(minv), DeviceBuffer.from_array(cdu)
         d_out = DeviceBuffer(n * DQDD_PER_KNOT * 4)
-        self._check(self._L.rbd_gradient_finish_f32(self._h, n, d_out.handle, d_c.handle, d_m.handle, None),
-               "rbd_gradient_finish_f32")
+        self.gradient_finish_dev(n, d_out, d_c, d_m)
         return d_out.download((n, DQDD_PER_KNOT))
 
     def run_refmode(self, vol):

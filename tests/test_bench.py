@@ -53,6 +53,9 @@ def test_oracle_check_kinds_of_the_other_entry_points():
     fext = np.random.default_rng(1).uniform(-20.0, 20.0, (64, 42)).astype(np.float32)
     with_fext = pyoracle.fdgrad(x64, fext=fext.astype(np.float64))
     moved = np.roll(minv.reshape(-1, 7, 7), 1, axis=1).reshape(-1, 49)
+    vol28 = np.concatenate([x, qdd.astype(np.float32)], axis=1)
+    cdu = pyoracle.id_gradient(vol28.astype(np.float64))
+    split = pyoracle.vaf(np.concatenate([vol28.astype(np.float64), np.zeros((64, 126))], axis=1))
     cases = {
         "fdgrad_minv": (np.concatenate([full, minv], 1), {"kind": "fdgrad_minv"}, True),
         "fd": (np.concatenate([qdd, minv], 1), {"kind": "fd"}, True),
@@ -60,11 +63,18 @@ def test_oracle_check_kinds_of_the_other_entry_points():
         "fext_dropped": (full, {"fext": fext}, False),
         "minv_moved": (np.concatenate([full, moved], 1), {"kind": "fdgrad_minv"}, False),
         "qdd_sign": (np.concatenate([-qdd, minv], 1), {"kind": "fd"}, False),
+        "dqdd": (full[:, 7:], {"kind": "dqdd"}, True),
+        "dqdd_shifted": (full[:, 6:104], {"kind": "dqdd"}, False),
+        "cdu": (cdu, {"kind": "cdu", "x": vol28}, True),
+        "cdu_sign": (-cdu, {"kind": "cdu", "x": vol28}, False),
+        "vaf": (split, {"kind": "vaf", "x": vol28}, True),
+        "vaf_f_dropped": (np.concatenate([split[:, :112], np.zeros((64, 42))], 1), {"kind": "vaf", "x": vol28}, False),
     }
     checks, res = bench.OracleChecks(), {}
     for name, (out, kw, _) in cases.items():
         res[name] = {}
-        checks.add(res[name], "accuracy", out.astype(np.float32), x, sample=48, **kw)
+        kw = dict(kw)
+        checks.add(res[name], "accuracy", out.astype(np.float32), kw.pop("x", x), sample=48, **kw)
     checks.run(threads=2)
     for name, (_, _, good) in cases.items():
         acc = res[name]["accuracy"]
