@@ -364,11 +364,12 @@ def test_small_graph_replays_back_to_back(rbd, lanes):
     list, and three rounds of 20,000 calls alternating between two lists (a
     new capture each), leave the device's free memory where it was to within
     the runtime's pool steps: the replays within 8 MiB, all captures within 64
-    MiB (the runtime's pools grow in steps while earlier graphs wait to be
-    destroyed), and the last round within 8 MiB of the one before, so nothing
-    is allocated per replay or per capture (rewriting a graph's kernel
-    arguments instead kept about 34 KB per replay, 3.9 GB over 100,000; a leak
-    of 420 bytes or more per capture would fail the last round)."""
+    MiB (the runtime's pools grow in steps of 10-22 MiB while earlier graphs
+    wait to be destroyed, in any of the rounds), and at least one round
+    within 8 MiB of the one before, so nothing is allocated per replay or per
+    capture (rewriting a graph's kernel arguments instead kept about 34 KB per
+    replay, 3.9 GB over 100,000; a leak of 420 bytes or more per capture would
+    take more than 8 MiB in every round)."""
     c = rbd.Context(precision="fp32", lanes_per_knot=lanes)
     b = rbd.Batch(c, 64)
     s = rbd.Stream()
@@ -402,8 +403,10 @@ def test_small_graph_replays_back_to_back(rbd, lanes):
         free.append(rbd.mem_info()[0])
     say(f"lanes {lanes}: free device memory {free0} -> {free1} (100,000 replays) -> {free} "
         f"(20,000 captures each)")
+    losses = [before - after for before, after in zip([free1] + free[:-1], free)]
     assert free1 >= free0 - (8 << 20)
-    assert free[-1] >= free1 - (64 << 20) and free[-1] >= free[-2] - (8 << 20)
+    assert free[-1] >= free1 - (64 << 20), losses
+    assert min(losses) <= (8 << 20), losses
     s.destroy()
     b.close()
     c.close()
