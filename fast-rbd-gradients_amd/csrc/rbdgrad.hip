@@ -441,6 +441,36 @@ __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void fd_kernel(long n, co
 template <class P, int LANES, bool SPLIT>
 __global__ __launch_bounds__(kBlock, kWavesPerSimd<P>) void id_gradient_kernel(long n, const float* __restrict__ vol,
                                                              float* __restrict__ cdu, Options opt) {
+  if constexpr (LANES == 1 && SPLIT && std::is_same<P, PrecFP32>::value) {
+    // FP32, split layout: the wave's 64 consecutive input records (154 floats
+    // each, one contiguous block) are read into LDS as whole lines, 16 bytes
+    // per lane, and each lane then reads its record from there.  Read by each
+    // lane at the 616-byte record stride, they were fetched 5x over.
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    __shared__ __align__(16) float inrec[64 * kSplitPerKnot];
+    const int lane = int(threadIdx.x);
+    const long stride = long(gridDim.x) * kBlock;
+    for (long k0 = long(blockIdx.x) * kBlock; k0 < n; k0 += stride) {
+      const int nk = int(n - k0 < kBlock ? n - k0 : kBlock);
+      const float* src = vol + k0 * kSplitPerKnot;
+      const int words = nk * kSplitPerKnot;
+      if ((reinterpret_cast<uintptr_t>(src) & 15u) == 0) {
+        for (int e = lane; e < words / 4; e += 64)
+          reinterpret_cast<v4f*>(inrec)[e] = reinterpret_cast<const v4f*>(src)[e];
+        for (int e = (words / 4) * 4 + lane; e < words; e += 64) inrec[e] = src[e];
+      } else {
+        for (int e = lane; e < words; e += 64) inrec[e] = src[e];
+      }
+      wave_lds_sync();
+      if (lane < nk) {
+        InSplit i{inrec + kSplitPerKnot * lane};
+        OutDqddAoS o{cdu + kDqddPerKnot * (k0 + lane)};
+        id_gradient_split_knot<P>(i, o, opt);
+      }
+      wave_lds_sync();                      // every lane has read its record before the next block overwrites inrec
+    }
+    return;
+  }
   const int role = threadIdx.x % LANES;
   const long groups = long(gridDim.x) * (blockDim.x / LANES);
   for (long k = (long(blockIdx.x) * blockDim.x + threadIdx.x) / LANES; k < n; k += groups) {
